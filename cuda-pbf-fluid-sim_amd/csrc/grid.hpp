@@ -1,0 +1,23 @@
+// grid.hpp — internal grid-build entry points shared by the launchers and the
+// headless pipeline.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pbf_params.h"
+
+namespace pbfx {
+
+// Segment compaction of sorted keys into unique keys / offsets / count, with an
+// optional per-slot segment index (segOf[s] = index of the cell holding slot s).
+// blockCounts/blockOffsets hold compact_scratch_elems(N) entries each.
+uint32_t compact_scratch_elems(uint32_t N);
+void compact_segments(uint32_t* uniq, uint32_t* offsets, uint32_t* count, uint32_t* segOf,
+                      const uint32_t* keysSorted, uint32_t N, uint32_t* blockCounts, uint32_t* blockOffsets,
+                      hipStream_t s);
+
+// Fill n words with a value (a kernel, so it is safe inside captured graphs).
+void fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
+
+}  // namespace pbfx

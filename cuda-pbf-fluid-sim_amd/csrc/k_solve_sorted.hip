@@ -1,0 +1,347 @@
+// k_solve_sorted.hip — the cell-reordered PBF step.
+//
+// Layout inside a step: Ps (float4, cell-sorted) holds the predicted position
+// in xyz and, between the lambda and delta passes, lambda in w, so a delta
+// neighbour costs one 16-byte load; Vs (float4, cell-sorted) holds velocities
+// for XSPH.  Every occupied cell carries its (2*reach+1)^2 neighbour-row spans,
+// built once per step, so the five neighbour passes read spans instead of
+// searching.  Slot s of a neighbour span is visited in the same order as the
+// reference's per-cell loops (rows dz-major, then dy; cells ascending x; slots
+// ascending), and slot identity replaces the j != i test, so every sum is
+// formed in the reference's order.
+//
+// Launch shape: 256-lane blocks over sorted slots, block index remapped so each
+// XCD owns one contiguous run of slots (its neighbours' rows then sit in the
+// same L2).
+#include "pbf_common.hpp"
+#include "pbf_hip.h"
+#include "pbf_math.hpp"
+#include "sorted_path.hpp"
+
+namespace pbfx {
+namespace {
+
+constexpr int kBS = 256;
+
+__device__ __forceinline__ uint32_t slot_of_thread() {
+    return xcd_tile(blockIdx.x, gridDim.x) * kBS + threadIdx.x;
+}
+
+__global__ __launch_bounds__(kBS) void k_predict_hash(const float4* __restrict__ X, const float4* __restrict__ V,
+                                                      float4* __restrict__ Xn, uint32_t* __restrict__ keys,
+                                                      uint32_t* __restrict__ epoch, float3 g, float dt,
+                                                      sim::GridBounds grid, uint32_t N) {
+    const uint32_t i = blockIdx.x * kBS + threadIdx.x;
+    if (epoch && i == 0) epoch[0] = epoch[0] + 1u;
+    if (i >= N) return;
+    const float4 x = X[i];
+    const float4 v = V[i];
+    // reference: integrate.cu:43-51, then boundary.cu:11-25 with e = 0 (the
+    // velocity side of that clamp is overwritten by the velocity phase)
+    const float vx = v.x + g.x * dt;
+    const float vy = v.y + g.y * dt;
+    const float vz = v.z + g.z * dt;
+    float px = x.x + vx * dt, py = x.y + vy * dt, pz = x.z + vz * dt;
+    float ux = 0.f, uy = 0.f, uz = 0.f;
+    clamp_box(px, py, pz, ux, uy, uz, grid, 0.0f);
+    Xn[i] = make_float4(px, py, pz, 1.0f);
+    keys[i] = cell_key(px, py, pz, grid);
+}
+
+__device__ __forceinline__ uint32_t table_home(uint32_t key, uint32_t mask) { return key & mask; }
+
+__global__ __launch_bounds__(kBS) void k_hash_insert(HashTable t, const uint32_t* __restrict__ uniq,
+                                                     const uint32_t* __restrict__ off,
+                                                     const uint32_t* __restrict__ count) {
+    const uint32_t m = blockIdx.x * kBS + threadIdx.x;
+    const uint32_t M = *count;
+    if (m >= M) return;
+    const uint32_t key = uniq[m];
+    const uint32_t ep = *t.epoch;
+    const unsigned long long mine = ((unsigned long long)ep << 32) | key;
+    uint32_t slot = table_home(key, t.mask);
+    for (uint32_t probe = 0; probe <= t.mask; ++probe, slot = (slot + 1) & t.mask) {
+        unsigned long long* w = reinterpret_cast<unsigned long long*>(&t.slots[slot]);
+        unsigned long long cur = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while ((uint32_t)(cur >> 32) != ep) {
+            const unsigned long long prev = atomicCAS(w, cur, mine);
+            if (prev == cur) {
+                t.slots[slot].beg = off[m];
+                t.slots[slot].end = off[m + 1];
+                return;
+            }
+            cur = prev;
+        }
+        // slot taken this step by another cell: keep probing
+    }
+    *t.overflow = 1u;
+}
+
+struct HashRows {
+    const HashSlot* __restrict__ slots;
+    uint32_t mask;
+    uint32_t ep;
+    __device__ __forceinline__ bool cell(uint32_t key, uint32_t& b, uint32_t& e) const {
+        uint32_t slot = table_home(key, mask);
+        for (uint32_t probe = 0; probe <= mask; ++probe, slot = (slot + 1) & mask) {
+            const uint4 v = *reinterpret_cast<const uint4*>(&slots[slot]);
+            if (v.y != ep) return false;
+            if (v.x == key) { b = v.z; e = v.w; return b < e; }
+        }
+        return false;
+    }
+    __device__ __forceinline__ void span(uint32_t k0, uint32_t k1, uint32_t& lo, uint32_t& hi) const {
+        lo = 0xFFFFFFFFu;
+        hi = 0u;
+        for (uint32_t c = k0; c <= k1; ++c) {
+            uint32_t b, e;
+            if (!cell(c, b, e)) continue;
+            lo = min(lo, b);
+            hi = max(hi, e);
+        }
+        if (lo == 0xFFFFFFFFu) lo = hi = 0;
+    }
+};
+
+template <class Rows>
+__device__ __forceinline__ void cell_spans(uint32_t key, uint32_t m, const Rows& rows, const SpanTable& st,
+                                           const sim::GridBounds& g) {
+    int cx, cy, cz;
+    decode_key(key, g.dim, cx, cy, cz);
+    const int R = st.reach;
+    const int x0 = max(0, cx - R), x1 = min(g.dim.x - 1, cx + R);
+    int r = 0;
+    for (int dz = -R; dz <= R; ++dz) {
+        const int z = cz + dz;
+        for (int dy = -R; dy <= R; ++dy, ++r) {
+            const int y = cy + dy;
+            uint32_t lo = 0, hi = 0;
+            if (z >= 0 && z < g.dim.z && y >= 0 && y < g.dim.y && x0 <= x1) {
+                const uint32_t base = (uint32_t)((z * g.dim.y + y) * g.dim.x);
+                rows.span(base + (uint32_t)x0, base + (uint32_t)x1, lo, hi);
+            }
+            st.spans[(uint64_t)r * st.stride + m] = make_uint2(lo, hi);
+        }
+    }
+}
+
+template <class Rows>
+__global__ __launch_bounds__(kBS) void k_spans_gather(const float4* __restrict__ Xn, const uint32_t* __restrict__ idxS,
+                                                      float4* __restrict__ Ps, uint32_t N,
+                                                      const uint32_t* __restrict__ uniq,
+                                                      const uint32_t* __restrict__ count, Rows rows, SpanTable st,
+                                                      sim::GridBounds g) {
+    const uint32_t t = blockIdx.x * kBS + threadIdx.x;
+    if (t < N) {
+        const float4 p = Xn[idxS[t]];
+        Ps[t] = make_float4(p.x, p.y, p.z, 0.0f);
+    }
+    const uint32_t M = *count;
+    if (t < M) cell_spans(uniq[t], t, rows, st, g);
+}
+
+struct CompactRowsCount {
+    const uint32_t* __restrict__ uniq;
+    const uint32_t* __restrict__ off;
+    const uint32_t* __restrict__ count;
+    __device__ __forceinline__ void span(uint32_t k0, uint32_t k1, uint32_t& lo, uint32_t& hi) const {
+        CompactRows r{uniq, off, *count};
+        r.span(k0, k1, lo, hi);
+    }
+};
+
+struct HashRowsDev {
+    const HashSlot* __restrict__ slots;
+    uint32_t mask;
+    const uint32_t* __restrict__ epoch;
+    __device__ __forceinline__ void span(uint32_t k0, uint32_t k1, uint32_t& lo, uint32_t& hi) const {
+        HashRows r{slots, mask, *epoch};
+        r.span(k0, k1, lo, hi);
+    }
+};
+
+__global__ __launch_bounds__(kBS) void k_lambda_sorted(float4* Ps, const uint32_t* __restrict__ segOf, SpanTable st,
+                                                       sim::DeviceParams dp, uint32_t N) {
+    const uint32_t s = slot_of_thread();
+    if (s >= N) return;
+    const float4 pi = Ps[s];
+    const uint32_t m = segOf[s];
+    const PairConsts c = make_pair_consts(dp);
+    const int cap = dp.maxNeighbors;
+    const bool hasCap = cap > 0;
+    LambdaAcc a;
+    for (int r = 0; r < st.rows; ++r) {
+        const uint2 sp = st.spans[(uint64_t)r * st.stride + m];
+        for (uint32_t k = sp.x; k < sp.y; ++k) {
+            if (hasCap && a.n >= cap) goto done;
+            if (k == s) continue;
+            const float4 pj = Ps[k];
+            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+            const float r2 = rx * rx + ry * ry + rz * rz;
+            if (r2 > c.h2) continue;
+            lambda_pair(a, rx, ry, rz, r2, c);
+        }
+    }
+done:
+    reinterpret_cast<float*>(Ps)[4ull * s + 3] = lambda_finish(a, dp);
+}
+
+__global__ __launch_bounds__(kBS) void k_delta_sorted(const float4* __restrict__ Pa, float4* __restrict__ Pb,
+                                                      const uint32_t* __restrict__ segOf, SpanTable st,
+                                                      sim::DeviceParams dp, uint32_t N, int last,
+                                                      const float4* __restrict__ X,
+                                                      const uint32_t* __restrict__ idxS, float4* __restrict__ Vs,
+                                                      float invDt) {
+    const uint32_t s = slot_of_thread();
+    if (s >= N) return;
+    const float4 pi = Pa[s];
+    const uint32_t m = segOf[s];
+    const PairConsts c = make_pair_consts(dp);
+    float dx = 0.f, dy = 0.f, dz = 0.f;
+    for (int r = 0; r < st.rows; ++r) {
+        const uint2 sp = st.spans[(uint64_t)r * st.stride + m];
+        for (uint32_t k = sp.x; k < sp.y; ++k) {
+            if (k == s) continue;
+            const float4 pj = Pa[k];
+            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+            const float r2 = rx * rx + ry * ry + rz * rz;
+            if (r2 > c.h2) continue;
+            delta_pair(dx, dy, dz, rx, ry, rz, r2, pi.w, pj.w, c);
+        }
+    }
+    const float3 d = delta_finish(dx, dy, dz, dp);
+    // reference: pbf_delta_apply.cu:232-233 (P += delta)
+    float px = pi.x + d.x, py = pi.y + d.y, pz = pi.z + d.z;
+    if (!last) {
+        Pb[s] = make_float4(px, py, pz, 0.0f);
+        return;
+    }
+    // post-solve boundary (phases.cpp:94) and velocity (phases.cpp:102-105)
+    float ux = 0.f, uy = 0.f, uz = 0.f;
+    clamp_box(px, py, pz, ux, uy, uz, dp.grid, 0.0f);
+    Pb[s] = make_float4(px, py, pz, 1.0f);
+    const float4 x = X[idxS[s]];
+    Vs[s] = make_float4((px - x.x) * invDt, (py - x.y) * invDt, (pz - x.z) * invDt, 0.0f);
+}
+
+__global__ __launch_bounds__(kBS) void k_finalize_sorted(float4* __restrict__ Ps, const float4* __restrict__ X,
+                                                         const uint32_t* __restrict__ idxS, float4* __restrict__ Vs,
+                                                         sim::GridBounds g, float invDt, uint32_t N) {
+    const uint32_t s = blockIdx.x * kBS + threadIdx.x;
+    if (s >= N) return;
+    const float4 p = Ps[s];
+    float px = p.x, py = p.y, pz = p.z, ux = 0.f, uy = 0.f, uz = 0.f;
+    clamp_box(px, py, pz, ux, uy, uz, g, 0.0f);
+    Ps[s] = make_float4(px, py, pz, 1.0f);
+    const float4 x = X[idxS[s]];
+    Vs[s] = make_float4((px - x.x) * invDt, (py - x.y) * invDt, (pz - x.z) * invDt, 0.0f);
+}
+
+__global__ __launch_bounds__(kBS) void k_xsph_scatter(const float4* __restrict__ Ps, const float4* __restrict__ Vs,
+                                                      const uint32_t* __restrict__ segOf, SpanTable st,
+                                                      sim::DeviceParams dp, uint32_t N,
+                                                      const uint32_t* __restrict__ idxS, float4* __restrict__ Xn,
+                                                      float4* __restrict__ V) {
+    const uint32_t s = slot_of_thread();
+    if (s >= N) return;
+    const float4 pi = Ps[s];
+    const float4 vi = Vs[s];
+    float vx = vi.x, vy = vi.y, vz = vi.z;
+    if (dp.xsph_c > 0.f) {
+        const uint32_t m = segOf[s];
+        const float h2 = dp.kernel.h2, poly6 = dp.kernel.poly6;
+        float sx = 0.f, sy = 0.f, sz = 0.f;
+        int n = 0;
+        for (int r = 0; r < st.rows; ++r) {
+            const uint2 sp = st.spans[(uint64_t)r * st.stride + m];
+            for (uint32_t k = sp.x; k < sp.y; ++k) {
+                if (k == s) continue;
+                const float4 pj = Ps[k];
+                const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+                const float r2 = rx * rx + ry * ry + rz * rz;
+                if (r2 > h2) continue;
+                const float hr2 = h2 - r2;
+                const float w = poly6 * hr2 * hr2 * hr2;
+                const float4 vj = Vs[k];
+                sx += (vj.x - vi.x) * w;
+                sy += (vj.y - vi.y) * w;
+                sz += (vj.z - vi.z) * w;
+                ++n;
+            }
+        }
+        const float3 v = xsph_finish(vi.x, vi.y, vi.z, sx, sy, sz, n, dp);
+        vx = v.x; vy = v.y; vz = v.z;
+    }
+    // BoundaryOp with restitution (post_ops.cpp:25-29, boundary.cu:11-25)
+    float px = pi.x, py = pi.y, pz = pi.z;
+    clamp_box(px, py, pz, vx, vy, vz, dp.grid, clamp_restitution(dp.boundaryRestitution));
+    const uint32_t i = idxS[s];
+    Xn[i] = make_float4(px, py, pz, 1.0f);
+    V[i] = make_float4(vx, vy, vz, 0.0f);
+}
+
+inline uint32_t blocks_for(uint32_t n) { return ceil_div(n > 0 ? n : 1, kBS); }
+
+}  // namespace
+
+void launch_predict_hash(const float4* X, const float4* V, float4* Xn, uint32_t* keys, uint32_t* epoch,
+                         const sim::SimParams& p, hipStream_t s) {
+    const uint32_t N = p.numParticles;
+    k_predict_hash<<<blocks_for(N), kBS, 0, s>>>(X, V, Xn, keys, epoch, p.gravity, p.dt, p.grid, N);
+    check_launch("predict_hash");
+}
+
+void launch_hash_insert(const HashTable& t, const uint32_t* uniq, const uint32_t* off, const uint32_t* count,
+                        uint32_t maxCells, hipStream_t s) {
+    k_hash_insert<<<blocks_for(maxCells), kBS, 0, s>>>(t, uniq, off, count);
+    check_launch("hash_insert");
+}
+
+void launch_spans_gather(CellLookup lookup, const float4* Xn, const uint32_t* idxS, float4* Ps, uint32_t N,
+                         const uint32_t* uniq, const uint32_t* off, const uint32_t* count, const uint32_t* cellStart,
+                         const uint32_t* cellEnd, const HashTable& table, const SpanTable& st,
+                         const sim::GridBounds& grid, hipStream_t s) {
+    const uint32_t b = blocks_for(N);
+    switch (lookup) {
+        case CellLookup::Dense:
+            k_spans_gather<<<b, kBS, 0, s>>>(Xn, idxS, Ps, N, uniq, count, DenseRows{cellStart, cellEnd}, st, grid);
+            break;
+        case CellLookup::Compact:
+            k_spans_gather<<<b, kBS, 0, s>>>(Xn, idxS, Ps, N, uniq, count, CompactRowsCount{uniq, off, count}, st,
+                                             grid);
+            break;
+        case CellLookup::Hashed:
+            k_spans_gather<<<b, kBS, 0, s>>>(Xn, idxS, Ps, N, uniq, count,
+                                             HashRowsDev{table.slots, table.mask, table.epoch}, st, grid);
+            break;
+    }
+    check_launch("spans_gather");
+}
+
+void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st, const sim::DeviceParams& dp,
+                          uint32_t N, hipStream_t s) {
+    k_lambda_sorted<<<blocks_for(N), kBS, 0, s>>>(Ps, segOf, st, dp, N);
+    check_launch("lambda_sorted");
+}
+
+void launch_delta_sorted(const float4* Pa, float4* Pb, const uint32_t* segOf, const SpanTable& st,
+                         const sim::DeviceParams& dp, uint32_t N, bool last, const float4* X, const uint32_t* idxS,
+                         float4* Vs, float invDt, hipStream_t s) {
+    k_delta_sorted<<<blocks_for(N), kBS, 0, s>>>(Pa, Pb, segOf, st, dp, N, last ? 1 : 0, X, idxS, Vs, invDt);
+    check_launch("delta_sorted");
+}
+
+void launch_finalize_sorted(float4* Ps, const float4* X, const uint32_t* idxS, float4* Vs,
+                            const sim::GridBounds& grid, float invDt, uint32_t N, hipStream_t s) {
+    k_finalize_sorted<<<blocks_for(N), kBS, 0, s>>>(Ps, X, idxS, Vs, grid, invDt, N);
+    check_launch("finalize_sorted");
+}
+
+void launch_xsph_scatter(const float4* Ps, const float4* Vs, const uint32_t* segOf, const SpanTable& st,
+                         const sim::DeviceParams& dp, uint32_t N, const uint32_t* idxS, float4* Xn, float4* V,
+                         hipStream_t s) {
+    k_xsph_scatter<<<blocks_for(N), kBS, 0, s>>>(Ps, Vs, segOf, st, dp, N, idxS, Xn, V);
+    check_launch("xsph_scatter");
+}
+
+}  // namespace pbfx

@@ -1,0 +1,184 @@
+// pbf_math.hpp — per-pair PBF arithmetic shared by every neighbour kernel.
+//
+// Expression association follows the reference exactly (the code is built
+// without FMA contraction, with IEEE division and square root), so the
+// reordered-layout kernels and the drop-in kernels produce the same bits:
+//   density += ((m * poly6) * hr2) * hr2 * hr2          pbf_lambda.cu:73-74
+//   coeff    = ((-3 * spiky) * (t * t)) / max(r, eps)  pbf_lambda.cu:76-79
+//   lambda   = -(rho/rho0 - 1) / max(denom, eps)       pbf_lambda.cu:86-107
+//   s_corr   = -k * pow(W / W(dq), n), floored         pbf_delta_apply.cu:77-84
+//   delta   += (lambda_i + lambda_j + s_corr) * grad   pbf_delta_apply.cu:86-89
+//   dv      += (v_j - v_i) * W                         xsph.cu:66-72
+#pragma once
+
+#include "pbf_common.hpp"
+
+namespace pbfx {
+
+// Constants hoisted out of the neighbour loops (each is the first product the
+// reference's left-to-right expression forms, so hoisting does not change bits).
+struct PairConsts {
+    float h, h2, poly6, m_poly6, neg3_spiky, grad_r_eps;
+    float w_q, neg_scorr_k, scorr_n, scorr_min;
+    int scorr_enable;
+};
+
+__device__ __forceinline__ PairConsts make_pair_consts(const sim::DeviceParams& dp) {
+    PairConsts c;
+    c.h = dp.kernel.h;
+    c.h2 = dp.kernel.h2;
+    c.poly6 = dp.kernel.poly6;
+    c.m_poly6 = dp.particleMass * dp.kernel.poly6;
+    c.neg3_spiky = -3.0f * dp.kernel.spiky;
+    c.grad_r_eps = dp.pbf.grad_r_eps;
+    const float dq = dp.pbf.scorr_dq_h * dp.kernel.h;
+    const float hr2_q = dp.kernel.h2 - dq * dq;
+    c.w_q = (hr2_q > 0.f) ? (dp.kernel.poly6 * hr2_q * hr2_q * hr2_q) : dp.pbf.wq_min;
+    c.neg_scorr_k = -dp.pbf.scorr_k;
+    c.scorr_n = dp.pbf.scorr_n;
+    c.scorr_min = dp.pbf.scorr_min;
+    c.scorr_enable = dp.pbf.scorr_enable;
+    return c;
+}
+
+struct LambdaAcc {
+    float density = 0.f, gx = 0.f, gy = 0.f, gz = 0.f, sumGrad2 = 0.f;
+    int n = 0;
+};
+
+// One in-range neighbour (r2 <= h2 already tested) of the lambda sum.
+__device__ __forceinline__ void lambda_pair(LambdaAcc& a, float rx, float ry, float rz, float r2,
+                                            const PairConsts& c) {
+    a.n++;
+    const float hr2 = c.h2 - r2;
+    a.density += c.m_poly6 * hr2 * hr2 * hr2;
+    const float r = sqrtf(r2);
+    const float r_safe = fmaxf(r, c.grad_r_eps);
+    const float t = c.h - r;
+    const float coeff = c.neg3_spiky * (t * t) / r_safe;
+    const float ax = coeff * rx, ay = coeff * ry, az = coeff * rz;
+    a.gx += ax;
+    a.gy += ay;
+    a.gz += az;
+    a.sumGrad2 += ax * ax + ay * ay + az * az;
+}
+
+// Lambda from the accumulated sums (reference: pbf_lambda.cu:85-107).
+__device__ __forceinline__ float lambda_finish(const LambdaAcc& a, const sim::DeviceParams& dp) {
+    const float gradSum2 = a.gx * a.gx + a.gy * a.gy + a.gz * a.gz;
+    const float scale = (dp.restDensity > 0.f) ? (dp.particleMass / dp.restDensity) : 0.f;
+    float denom = (scale * scale) * (a.sumGrad2 + gradSum2) + dp.pbf.lambda_denom_eps;
+    if (dp.pbf.compliance > 0.f && dp.dt > 0.f) denom += dp.pbf.compliance / (dp.dt * dp.dt);
+    denom = fmaxf(denom, dp.pbf.lambda_denom_eps);
+    if (a.n < 2 || (a.sumGrad2 + gradSum2) < 1e-12f) return 0.0f;
+    const float C = (dp.restDensity > 0.f) ? (a.density / dp.restDensity - 1.f) : 0.f;
+    float lam = -C / denom;
+    if (!isfinite(lam)) lam = 0.f;
+    if (dp.pbf.enable_lambda_clamp) {
+        if (fabsf(lam) > dp.pbf.lambda_max_abs) lam = copysignf(dp.pbf.lambda_max_abs, lam);
+    }
+    return lam;
+}
+
+// One in-range neighbour of the position-correction sum.
+__device__ __forceinline__ void delta_pair(float& dx, float& dy, float& dz, float rx, float ry, float rz, float r2,
+                                           float lam_i, float lam_j, const PairConsts& c) {
+    const float r = sqrtf(r2);
+    const float t = c.h - r;
+    const float r_safe = fmaxf(r, c.grad_r_eps);
+    const float coeff = c.neg3_spiky * (t * t) / r_safe;
+    const float ax = coeff * rx, ay = coeff * ry, az = coeff * rz;
+    float scorr = 0.0f;
+    if (c.scorr_enable) {
+        const float hr2 = c.h2 - r2;
+        const float w = c.poly6 * hr2 * hr2 * hr2;
+        const float ratio = (c.w_q > 0.f) ? (w / c.w_q) : 0.0f;
+        scorr = c.neg_scorr_k * powf(ratio, c.scorr_n);
+        if (c.scorr_min < 0.f) scorr = fmaxf(scorr, c.scorr_min);
+    }
+    const float lij = (lam_i + lam_j + scorr);
+    dx += lij * ax;
+    dy += lij * ay;
+    dz += lij * az;
+}
+
+// Scale, relax and clamp of the summed correction (reference: pbf_delta_apply.cu:93-110).
+__device__ __forceinline__ float3 delta_finish(float dx, float dy, float dz, const sim::DeviceParams& dp) {
+    const float invRest = (dp.restDensity > 0.f) ? (1.0f / dp.restDensity) : 0.f;
+    float ex = invRest * dx, ey = invRest * dy, ez = invRest * dz;
+    if (dp.pbf.enable_relax) {
+        ex *= dp.pbf.relax_omega;
+        ey *= dp.pbf.relax_omega;
+        ez *= dp.pbf.relax_omega;
+    }
+    if (dp.pbf.enable_disp_clamp) {
+        const float maxDisp = dp.pbf.disp_clamp_max_h * dp.kernel.h;
+        const float maxDisp2 = maxDisp * maxDisp;
+        const float len2 = ex * ex + ey * ey + ez * ez;
+        if (len2 > maxDisp2) {
+            const float len = sqrtf(len2);
+            const float s = maxDisp / (len + 1e-20f);
+            ex *= s;
+            ey *= s;
+            ez *= s;
+        }
+    }
+    return make_float3(ex, ey, ez);
+}
+
+// XSPH update of v_i from the summed (v_j - v_i) W (reference: xsph.cu:80-93).
+__device__ __forceinline__ float3 xsph_finish(float vx, float vy, float vz, float sx, float sy, float sz, int n,
+                                              const sim::DeviceParams& dp) {
+    float gate = 1.0f;
+    if (dp.pbf.xsph_gate_enable) {
+        const int nMin = max(0, dp.pbf.xsph_n_min);
+        const int nMax = max(nMin + 1, dp.pbf.xsph_n_max);
+        const float t = (float(n) - float(nMin)) / float(nMax - nMin);
+        gate = fminf(1.f, fmaxf(0.f, t));
+    }
+    const float mor = (dp.restDensity > 0.f) ? (dp.particleMass / dp.restDensity) : 0.f;
+    const float cg = dp.xsph_c * gate;
+    return make_float3(vx + cg * mor * sx, vy + cg * mor * sy, vz + cg * mor * sz);
+}
+
+// --------------------------------------------------------- row enumeration
+// For a neighbour row (dz, dy) the cells x-reach..x+reach (clipped to the grid)
+// hold consecutive keys, so their sorted slots form one contiguous span that
+// visits them in the reference's dx-then-slot order.
+
+// Dense provider over start/end arrays with sentinel empties.
+struct DenseRows {
+    const uint32_t* __restrict__ start;
+    const uint32_t* __restrict__ end;
+    __device__ __forceinline__ void span(uint32_t k0, uint32_t k1, uint32_t& lo, uint32_t& hi) const {
+        lo = 0xFFFFFFFFu;
+        hi = 0u;
+        for (uint32_t c = k0; c <= k1; ++c) {
+            const uint32_t b = start[c], e = end[c];
+            if (b == 0xFFFFFFFFu || b >= e) continue;
+            lo = min(lo, b);
+            hi = max(hi, e);
+        }
+        if (lo == 0xFFFFFFFFu) lo = hi = 0;
+    }
+};
+
+// Compact provider: one lower_bound over the unique keys per row
+// (reference: cuda_grid_utils.cuh:34-67 searches once per cell).
+struct CompactRows {
+    const uint32_t* __restrict__ uniq;
+    const uint32_t* __restrict__ off;
+    uint32_t M;
+    __device__ __forceinline__ void span(uint32_t k0, uint32_t k1, uint32_t& lo, uint32_t& hi) const {
+        uint32_t a = 0, b = M;
+        while (a < b) {
+            const uint32_t mid = (a + b) >> 1;
+            if (uniq[mid] < k0) a = mid + 1; else b = mid;
+        }
+        uint32_t e = a;
+        while (e < M && uniq[e] <= k1) ++e;
+        if (e > a) { lo = off[a]; hi = off[e]; } else { lo = hi = 0; }
+    }
+};
+
+}  // namespace pbfx

@@ -1,0 +1,579 @@
+// pipeline.cpp — headless phase pipeline, dispatcher registry and Simulator.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "grid.hpp"
+#include "pbf_common.hpp"
+#include "pbf_hip.h"
+#include "radix_sort.hpp"
+#include "sim_host.h"
+
+namespace sim {
+
+using pbfx::set_error;
+
+namespace {
+
+// Event marks recorded after every launch while a profiled (eager) step runs.
+struct LaunchTimer {
+    hipStream_t stream = nullptr;
+    std::vector<const char*> names;
+    std::vector<hipEvent_t> events;
+    void mark(const char* n) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        (void)hipEventRecord(e, stream);
+        names.push_back(n);
+        events.push_back(e);
+    }
+};
+thread_local LaunchTimer* t_timer = nullptr;
+inline void mark(const char* n) {
+    if (t_timer) t_timer->mark(n);
+}
+
+template <class T>
+bool dalloc(T*& p, size_t count) {
+    p = nullptr;
+    if (count == 0) count = 1;
+    if (hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * count) != hipSuccess) {
+        p = nullptr;
+        set_error(PBFX_E_CAPACITY, "device allocation failed");
+        return false;
+    }
+    return true;
+}
+
+template <class T>
+void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+int reach_for(const SimParams& p) {
+    const float cs = p.grid.cellSize;
+    if (!(cs > 0.f)) return 1;
+    const float q = std::ceil(p.kernel.h / cs);
+    return std::max(1, q >= 1e6f ? 1000000 : (int)q);
+}
+
+__global__ void k_copy4(float4* __restrict__ dst, const float4* __restrict__ src, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+void copy4(float4* dst, const float4* src, uint32_t n, hipStream_t s) {
+    if (n == 0) return;
+    k_copy4<<<pbfx::ceil_div(n, 256), 256, 0, s>>>(dst, src, n);
+    pbfx::check_launch("copy4");
+}
+
+// ------------------------------------------------------------ reference path
+// Phase bodies restate sim/phases.cpp:33-105 and sim/post_ops.cpp:25-71 over the
+// drop-in launchers.
+void RefIntegrate(DeviceBuffers& b, GridBuffers&, const SimParams& p, hipStream_t s) {
+    LaunchIntegratePred(b.d_pos, b.d_vel, b.d_pos_pred, p.gravity, p.dt, p.numParticles, s);
+    mark("integrate");
+    LaunchBoundary(b.d_pos_pred, b.d_vel, p.grid, 0.0f, p.numParticles, s);
+    mark("boundary");
+}
+
+void RefGridBuild(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_t s) {
+    LaunchHashKeys(g.d_cellKeys, g.d_indices, b.d_pos_pred, p.grid, p.numParticles, s);
+    mark("hash");
+    LaunchSortPairs(g.d_sortTemp, g.sortTempBytes, g.d_cellKeys, g.d_cellKeys_sorted, g.d_indices,
+                    g.d_indices_sorted, p.numParticles, s);
+    mark("sort");
+    if (g.kind == GridKind::Dense) {
+        LaunchCellRanges(g.d_cellStart, g.d_cellEnd, g.d_cellKeys_sorted, p.numParticles, g.numCells, s);
+    } else {
+        LaunchCellRangesCompact(g.d_cellUniqueKeys, g.d_cellOffsets, g.d_compactCount, g.d_cellKeys_sorted,
+                                p.numParticles, s);
+    }
+    mark("cell_ranges");
+}
+
+void RefSolve(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_t s) {
+    for (int it = 0; it < p.solverIters; ++it) {
+        const DeviceParams dp = MakeDeviceParams(p);
+        if (g.kind == GridKind::Dense) {
+            LaunchLambda(b.d_lambda, b.d_pos_pred, g.d_indices_sorted, g.d_cellKeys_sorted, g.d_cellStart,
+                         g.d_cellEnd, dp, p.numParticles, s);
+            mark("lambda");
+            LaunchDeltaApply(b.d_pos_pred, b.d_delta, b.d_lambda, g.d_indices_sorted, g.d_cellKeys_sorted,
+                             g.d_cellStart, g.d_cellEnd, dp, p.numParticles, s);
+        } else {
+            LaunchLambdaCompact(b.d_lambda, b.d_pos_pred, g.d_indices_sorted, g.d_cellKeys_sorted,
+                                g.d_cellUniqueKeys, g.d_cellOffsets, g.d_compactCount, dp, p.numParticles, s);
+            mark("lambda");
+            LaunchDeltaApplyCompact(b.d_pos_pred, b.d_delta, b.d_lambda, g.d_indices_sorted, g.d_cellKeys_sorted,
+                                    g.d_cellUniqueKeys, g.d_cellOffsets, g.d_compactCount, dp, p.numParticles, s);
+        }
+        mark("delta_apply");
+    }
+    LaunchBoundary(b.d_pos_pred, b.d_vel, p.grid, 0.0f, p.numParticles, s);
+    mark("boundary");
+}
+
+void RefVelocity(DeviceBuffers& b, GridBuffers&, const SimParams& p, hipStream_t s) {
+    LaunchVelocity(b.d_vel, b.d_pos, b.d_pos_pred, 1.0f / p.dt, p.numParticles, s);
+    mark("velocity");
+}
+
+void RefPost(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_t s) {
+    if (p.xsph_c > 0.f && p.numParticles > 0) {
+        const DeviceParams dp = MakeDeviceParams(p);
+        copy4(b.d_vel_snapshot, b.d_vel, p.numParticles, s);
+        if (g.kind == GridKind::Dense) {
+            LaunchXSPH(b.d_vel, b.d_vel_snapshot, b.d_pos_pred, g.d_indices_sorted, g.d_cellKeys_sorted,
+                       g.d_cellStart, g.d_cellEnd, dp, p.numParticles, s);
+        } else {
+            LaunchXSPHCompact(b.d_vel, b.d_vel_snapshot, b.d_pos_pred, g.d_indices_sorted, g.d_cellKeys_sorted,
+                              g.d_cellUniqueKeys, g.d_cellOffsets, g.d_compactCount, dp, p.numParticles, s);
+        }
+        mark("xsph");
+    }
+    LaunchBoundary(b.d_pos_pred, b.d_vel, p.grid, p.boundaryRestitution, p.numParticles, s);
+    mark("boundary");
+}
+
+// ------------------------------------------------------------ reordered path
+void SortedPredict(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_t s) {
+    pbfx::launch_predict_hash(b.d_pos, b.d_vel, b.d_pos_pred, g.d_cellKeys,
+                              g.kind == GridKind::Hashed ? g.table.epoch : nullptr, p, s);
+    mark("predict_hash");
+}
+
+void SortedGridBuild(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_t s) {
+    const uint32_t N = p.numParticles;
+    pbfx::radix_sort_pairs(g.d_sortTemp, g.sortTempBytes, g.d_cellKeys, g.d_cellKeys_sorted, nullptr,
+                           g.d_indices_sorted, N, 0, g.keyBits, s);
+    mark("sort");
+    pbfx::compact_segments(g.d_cellUniqueKeys, g.d_cellOffsets, g.d_compactCount, g.d_segOf, g.d_cellKeys_sorted, N,
+                           g.d_blockCounts, g.d_blockOffsets, s);
+    mark("compact");
+    pbfx::CellLookup lk = pbfx::CellLookup::Compact;
+    if (g.kind == GridKind::Dense) {
+        LaunchCellRanges(g.d_cellStart, g.d_cellEnd, g.d_cellKeys_sorted, N, g.numCells, s);
+        lk = pbfx::CellLookup::Dense;
+    } else if (g.kind == GridKind::Hashed) {
+        pbfx::launch_hash_insert(g.table, g.d_cellUniqueKeys, g.d_cellOffsets, g.d_compactCount, N, s);
+        lk = pbfx::CellLookup::Hashed;
+    }
+    if (lk != pbfx::CellLookup::Compact) mark("cell_lookup");
+    pbfx::launch_spans_gather(lk, b.d_pos_pred, g.d_indices_sorted, b.d_pos_sorted[0], N, g.d_cellUniqueKeys,
+                              g.d_cellOffsets, g.d_compactCount, g.d_cellStart, g.d_cellEnd, g.table, g.spans,
+                              p.grid, s);
+    mark("spans_gather");
+}
+
+void SortedSolve(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_t s) {
+    const uint32_t N = p.numParticles;
+    const DeviceParams dp = MakeDeviceParams(p);
+    const float invDt = 1.0f / p.dt;
+    if (p.solverIters <= 0) {
+        pbfx::launch_finalize_sorted(b.d_pos_sorted[0], b.d_pos, g.d_indices_sorted, b.d_vel_sorted, p.grid, invDt,
+                                     N, s);
+        mark("finalize");
+        return;
+    }
+    int cur = 0;
+    for (int it = 0; it < p.solverIters; ++it) {
+        pbfx::launch_lambda_sorted(b.d_pos_sorted[cur], g.d_segOf, g.spans, dp, N, s);
+        mark("lambda");
+        pbfx::launch_delta_sorted(b.d_pos_sorted[cur], b.d_pos_sorted[cur ^ 1], g.d_segOf, g.spans, dp, N,
+                                  it == p.solverIters - 1, b.d_pos, g.d_indices_sorted, b.d_vel_sorted, invDt, s);
+        mark(it == p.solverIters - 1 ? "delta_last" : "delta");
+        cur ^= 1;
+    }
+}
+
+void SortedPost(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_t s) {
+    const int fin = p.solverIters > 0 ? (p.solverIters & 1) : 0;
+    pbfx::launch_xsph_scatter(b.d_pos_sorted[fin], b.d_vel_sorted, g.d_segOf, g.spans, MakeDeviceParams(p),
+                              p.numParticles, g.d_indices_sorted, b.d_pos_pred, b.d_vel, s);
+    mark("xsph_scatter");
+}
+
+// Phase objects look their launch function up in the dispatcher.
+class DispatchedPhase : public IPhase {
+public:
+    DispatchedPhase(PhaseType t, const char* n) : m_type(t), m_name(n) {}
+    PhaseType type() const override { return m_type; }
+    const char* name() const override { return m_name; }
+    void run(SimulationContext& ctx, const SimParams& p, hipStream_t s) override {
+        if (p.numParticles == 0) return;
+        KernelLaunchFn fn = ctx.dispatcher->find(KernelVariant{ctx.grid->kind, ctx.path}, m_type);
+        if (!fn) {
+            set_error(PBFX_E_INVALID_ARGUMENT, "no kernel registered for phase");
+            return;
+        }
+        fn(*ctx.bufs, *ctx.grid, p, s);
+    }
+
+private:
+    PhaseType m_type;
+    const char* m_name;
+};
+
+}  // namespace
+
+// ------------------------------------------------------------------ buffers
+bool DeviceBuffers::allocate(uint32_t cap) {
+    release();
+    if (cap == 0) cap = 1;
+    bool ok = dalloc(d_pos_curr, cap) && dalloc(d_pos_next, cap) && dalloc(d_vel, cap) && dalloc(d_lambda, cap) &&
+              dalloc(d_delta, cap) && dalloc(d_vel_snapshot, cap) && dalloc(d_pos_sorted[0], cap) &&
+              dalloc(d_pos_sorted[1], cap) && dalloc(d_vel_sorted, cap);
+    if (!ok) {
+        release();
+        return false;
+    }
+    capacity = cap;
+    d_pos = d_pos_curr;
+    d_pos_pred = d_pos_next;
+    (void)hipMemset(d_pos_curr, 0, sizeof(float4) * cap);
+    (void)hipMemset(d_pos_next, 0, sizeof(float4) * cap);
+    (void)hipMemset(d_vel, 0, sizeof(float4) * cap);
+    return true;
+}
+
+void DeviceBuffers::release() {
+    dfree(d_pos_curr); dfree(d_pos_next); dfree(d_vel); dfree(d_lambda); dfree(d_delta); dfree(d_vel_snapshot);
+    dfree(d_pos_sorted[0]); dfree(d_pos_sorted[1]); dfree(d_vel_sorted);
+    d_pos = d_pos_pred = nullptr;
+    capacity = 0;
+}
+
+void DeviceBuffers::swapPositionPingPong() {
+    std::swap(d_pos_curr, d_pos_next);
+    d_pos = d_pos_curr;
+    d_pos_pred = d_pos_next;
+}
+
+bool GridBuffers::allocate(uint32_t cap, uint32_t nCells, GridKind k, uint32_t tLog2, int reach, bool reordered) {
+    release();
+    if (cap == 0) cap = 1;
+    kind = k;
+    capacity = cap;
+    numCells = nCells;
+    keyBits = reordered ? pbfx::key_bits_for(nCells) : 32;
+    bool ok = dalloc(d_cellKeys, cap) && dalloc(d_cellKeys_sorted, cap) && dalloc(d_indices, cap) &&
+              dalloc(d_indices_sorted, cap);
+    sortTempBytes = pbfx::radix_sort_temp_bytes(cap, 0, 32);
+    ok = ok && (hipMalloc(&d_sortTemp, sortTempBytes) == hipSuccess);
+    if (k == GridKind::Dense) ok = ok && dalloc(d_cellStart, nCells) && dalloc(d_cellEnd, nCells);
+    ok = ok && dalloc(d_cellUniqueKeys, cap) && dalloc(d_cellOffsets, (size_t)cap + 1) && dalloc(d_compactCount, 1);
+    if (reordered) {
+        const uint32_t tiles = pbfx::compact_scratch_elems(cap);
+        spans.reach = reach;
+        spans.rows = (2 * reach + 1) * (2 * reach + 1);
+        spans.stride = cap;
+        ok = ok && dalloc(d_segOf, cap) && dalloc(d_blockCounts, tiles) && dalloc(d_blockOffsets, tiles) &&
+             dalloc(spans.spans, (size_t)spans.rows * cap);
+        if (k == GridKind::Hashed) {
+            tableLog2 = tLog2;
+            const size_t slots = size_t(1) << tLog2;
+            table.mask = (uint32_t)(slots - 1);
+            ok = ok && dalloc(table.slots, slots) && dalloc(table.epoch, 1) && dalloc(table.overflow, 1);
+            if (ok) {
+                (void)hipMemset(table.slots, 0, sizeof(pbfx::HashSlot) * slots);
+                (void)hipMemset(table.epoch, 0, sizeof(uint32_t));
+                (void)hipMemset(table.overflow, 0, sizeof(uint32_t));
+            }
+        }
+    }
+    if (!ok) {
+        release();
+        return false;
+    }
+    (void)hipMemset(d_compactCount, 0, sizeof(uint32_t));
+    return true;
+}
+
+void GridBuffers::release() {
+    dfree(d_cellKeys); dfree(d_cellKeys_sorted); dfree(d_indices); dfree(d_indices_sorted);
+    dfree(d_cellStart); dfree(d_cellEnd); dfree(d_cellUniqueKeys); dfree(d_cellOffsets); dfree(d_compactCount);
+    dfree(d_segOf); dfree(d_blockCounts); dfree(d_blockOffsets);
+    if (d_sortTemp) (void)hipFree(d_sortTemp);
+    d_sortTemp = nullptr;
+    sortTempBytes = 0;
+    dfree(spans.spans);
+    dfree(table.slots); dfree(table.epoch); dfree(table.overflow);
+    capacity = 0;
+}
+
+// --------------------------------------------------------------- dispatcher
+KernelLaunchFn KernelDispatcher::find(const KernelVariant& v, PhaseType ph) const {
+    auto it = m_map.find(key(ph, v));
+    if (it != m_map.end()) return it->second;
+    if (v.grid == GridKind::Hashed) {
+        it = m_map.find(key(ph, KernelVariant{GridKind::Compact, v.path}));
+        if (it != m_map.end()) return it->second;
+    }
+    it = m_map.find(key(ph, KernelVariant{GridKind::Dense, v.path}));
+    return it != m_map.end() ? it->second : nullptr;
+}
+
+void InitializeKernelDispatcher(KernelDispatcher& kd) {
+    // Grid-kind differences live inside the launch functions (GridBuffers::kind);
+    // the registry is keyed so a specialised variant can be slotted in per kind.
+    kd.registerKernel(PhaseType::Integrate, {GridKind::Dense, PathKind::Reference}, RefIntegrate);
+    kd.registerKernel(PhaseType::GridBuild, {GridKind::Dense, PathKind::Reference}, RefGridBuild);
+    kd.registerKernel(PhaseType::SolveIterations, {GridKind::Dense, PathKind::Reference}, RefSolve);
+    kd.registerKernel(PhaseType::Velocity, {GridKind::Dense, PathKind::Reference}, RefVelocity);
+    kd.registerKernel(PhaseType::Post, {GridKind::Dense, PathKind::Reference}, RefPost);
+    kd.registerKernel(PhaseType::Integrate, {GridKind::Dense, PathKind::Reordered}, SortedPredict);
+    kd.registerKernel(PhaseType::GridBuild, {GridKind::Dense, PathKind::Reordered}, SortedGridBuild);
+    kd.registerKernel(PhaseType::SolveIterations, {GridKind::Dense, PathKind::Reordered}, SortedSolve);
+    kd.registerKernel(PhaseType::Post, {GridKind::Dense, PathKind::Reordered}, SortedPost);
+}
+
+void BuildDefaultPipelines(PhasePipeline& pl, PathKind path) {
+    pl.clear();
+    pl.add(std::make_unique<DispatchedPhase>(PhaseType::Integrate, "integrate"));
+    pl.add(std::make_unique<DispatchedPhase>(PhaseType::GridBuild, "grid_build"));
+    pl.add(std::make_unique<DispatchedPhase>(PhaseType::SolveIterations, "solve"));
+    // the reordered path fuses the velocity phase into the last delta pass
+    if (path == PathKind::Reference) pl.add(std::make_unique<DispatchedPhase>(PhaseType::Velocity, "velocity"));
+    pl.add(std::make_unique<DispatchedPhase>(PhaseType::Post, "post"));
+}
+
+// ---------------------------------------------------------------- simulator
+float JitterUniform(uint64_t seed, uint64_t particle, uint32_t axis) {
+    // SplitMix64 finaliser (Steele, Lea, Flood 2014) over (seed, 3*particle+axis+1).
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * (particle * 3ull + (uint64_t)axis + 1ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    return (float)(uint32_t)(z >> 40) * (1.0f / 16777216.0f);
+}
+
+bool Simulator::initialize(const SimParams& p, const SimOptions& opt) {
+    shutdown();
+    if (p.grid.dim.x <= 0 || p.grid.dim.y <= 0 || p.grid.dim.z <= 0 || !(p.grid.cellSize > 0.f)) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "Simulator::initialize: empty grid");
+        return false;
+    }
+    const uint64_t cells = (uint64_t)p.grid.dim.x * (uint64_t)p.grid.dim.y * (uint64_t)p.grid.dim.z;
+    if (cells > 0xFFFFFFFFull) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "Simulator::initialize: more than 2^32 cells");
+        return false;
+    }
+    if (opt.grid == GridKind::Hashed) {
+        if (opt.path != PathKind::Reordered || opt.tableLog2 < 4 || opt.tableLog2 > 30 ||
+            (uint64_t(1) << opt.tableLog2) < std::max<uint64_t>(p.numParticles, 1)) {
+            set_error(PBFX_E_INVALID_ARGUMENT, "Simulator::initialize: hash table smaller than the particle count");
+            return false;
+        }
+    }
+    m_params = p;
+    m_opt = opt;
+    if (hipStreamCreateWithFlags(&m_stream, hipStreamNonBlocking) != hipSuccess) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "stream creation failed");
+        return false;
+    }
+    const uint32_t cap = std::max<uint32_t>(p.maxParticles, p.numParticles);
+    if (!m_bufs.allocate(cap)) return false;
+    if (!m_grid.allocate(cap, (uint32_t)cells, opt.grid, opt.tableLog2, reach_for(p),
+                         opt.path == PathKind::Reordered))
+        return false;
+    if (opt.path == PathKind::Reference && opt.grid != GridKind::Dense) {
+        if (!EnsureCellCompactScratch(cap, 256)) return false;
+    }
+    InitializeKernelDispatcher(m_dispatcher);
+    BuildDefaultPipelines(m_pipeline, opt.path);
+    m_ctx.bufs = &m_bufs;
+    m_ctx.grid = &m_grid;
+    m_ctx.useHashedGrid = opt.grid != GridKind::Dense;
+    m_ctx.path = opt.path;
+    m_ctx.dispatcher = &m_dispatcher;
+    m_parity = 0;
+    m_frameIndex = 0;
+    m_graphDirty = true;
+    m_initialized = true;
+    return true;
+}
+
+void Simulator::destroyGraphs() {
+    for (auto& e : m_exec) {
+        if (e) (void)hipGraphExecDestroy(e);
+        e = nullptr;
+    }
+}
+
+void Simulator::shutdown() {
+    destroyGraphs();
+    m_bufs.release();
+    m_grid.release();
+    if (m_stream) (void)hipStreamDestroy(m_stream);
+    m_stream = nullptr;
+    m_initialized = false;
+}
+
+void Simulator::runPipeline(hipStream_t s) { m_pipeline.runFull(m_ctx, m_params, s); }
+
+bool Simulator::captureGraphs() {
+    destroyGraphs();
+    for (int k = 0; k < 2; ++k) {
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(m_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
+        runPipeline(m_stream);
+        if (hipStreamEndCapture(m_stream, &g) != hipSuccess || !g) return false;
+        const int slot = m_parity ^ k;
+        const hipError_t e = hipGraphInstantiate(&m_exec[slot], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            set_error((int)e, "hipGraphInstantiate");
+            return false;
+        }
+        m_bufs.swapPositionPingPong();
+    }
+    // two swaps: orientation restored
+    m_graphDirty = false;
+    return true;
+}
+
+bool Simulator::step(const SimParams& p) {
+    if (!m_initialized) return false;
+    if (std::memcmp(&p, &m_params, sizeof(SimParams)) != 0) {
+        if (p.numParticles > m_bufs.capacity || p.grid.dim.x != m_params.grid.dim.x ||
+            p.grid.dim.y != m_params.grid.dim.y || p.grid.dim.z != m_params.grid.dim.z ||
+            p.grid.cellSize != m_params.grid.cellSize || p.kernel.h != m_params.kernel.h) {
+            set_error(PBFX_E_INVALID_ARGUMENT, "Simulator::step: structural change needs initialize()");
+            return false;
+        }
+        m_params = p;
+        m_graphDirty = true;
+    }
+    return stepMany(1);
+}
+
+bool Simulator::stepMany(int steps) {
+    if (!m_initialized) return false;
+    for (int k = 0; k < steps; ++k) {
+        if (m_opt.useGraph) {
+            if (m_graphDirty && !captureGraphs()) {
+                set_error(PBFX_E_INVALID_ARGUMENT, "graph capture failed");
+                return false;
+            }
+            const hipError_t e = hipGraphLaunch(m_exec[m_parity], m_stream);
+            if (e != hipSuccess) {
+                set_error((int)e, "hipGraphLaunch");
+                return false;
+            }
+        } else {
+            runPipeline(m_stream);
+        }
+        m_bufs.swapPositionPingPong();
+        m_parity ^= 1;
+        ++m_frameIndex;
+    }
+    return true;
+}
+
+bool Simulator::profilePhases(std::vector<std::pair<std::string, float>>& out) {
+    out.clear();
+    if (!m_initialized) return false;
+    LaunchTimer timer;
+    timer.stream = m_stream;
+    timer.mark("start");
+    t_timer = &timer;
+    runPipeline(m_stream);
+    t_timer = nullptr;
+    bool ok = !timer.events.empty() && hipEventSynchronize(timer.events.back()) == hipSuccess;
+    for (size_t i = 1; ok && i < timer.events.size(); ++i) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, timer.events[i - 1], timer.events[i]);
+        out.emplace_back(timer.names[i], ms);
+    }
+    for (auto& e : timer.events) (void)hipEventDestroy(e);
+    if (!ok) return false;
+    m_bufs.swapPositionPingPong();
+    m_parity ^= 1;
+    ++m_frameIndex;
+    return true;
+}
+
+bool Simulator::seedBoxLattice(uint32_t first, uint32_t nx, uint32_t ny, uint32_t nz, float3 o, float spacing,
+                               float amp, uint64_t seed) {
+    const uint64_t n64 = (uint64_t)nx * ny * nz;
+    if (!m_initialized || first + n64 > m_bufs.capacity) {
+        set_error(PBFX_E_CAPACITY, "seedBoxLattice: block exceeds capacity");
+        return false;
+    }
+    const uint32_t n = (uint32_t)n64;
+    const GridBounds& g = m_params.grid;
+    std::vector<float4> h(n);
+    uint32_t idx = 0;
+    for (uint32_t iz = 0; iz < nz; ++iz)
+        for (uint32_t iy = 0; iy < ny; ++iy)
+            for (uint32_t ix = 0; ix < nx; ++ix) {
+                float x = o.x + (float)ix * spacing;
+                float y = o.y + (float)iy * spacing;
+                float z = o.z + (float)iz * spacing;
+                x = std::fmin(std::fmax(x, g.mins.x), g.maxs.x);
+                y = std::fmin(std::fmax(y, g.mins.y), g.maxs.y);
+                z = std::fmin(std::fmax(z, g.mins.z), g.maxs.z);
+                h[idx++] = make_float4(x, y, z, 1.0f);
+            }
+    if (amp > 0.f) {
+        for (uint32_t i = 0; i < n; ++i) {
+            const float jx = (2.0f * JitterUniform(seed, i, 0) - 1.0f) * amp;
+            const float jy = (2.0f * JitterUniform(seed, i, 1) - 1.0f) * amp;
+            const float jz = (2.0f * JitterUniform(seed, i, 2) - 1.0f) * amp;
+            h[i].x = std::fmin(std::fmax(h[i].x + jx, g.mins.x), g.maxs.x);
+            h[i].y = std::fmin(std::fmax(h[i].y + jy, g.mins.y), g.maxs.y);
+            h[i].z = std::fmin(std::fmax(h[i].z + jz, g.mins.z), g.maxs.z);
+        }
+    }
+    const size_t bytes = sizeof(float4) * n;
+    bool ok = hipMemcpy(m_bufs.d_pos + first, h.data(), bytes, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(m_bufs.d_pos_pred + first, h.data(), bytes, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemset(m_bufs.d_vel + first, 0, bytes) == hipSuccess;
+    if (!ok) set_error(PBFX_E_INVALID_ARGUMENT, "seedBoxLattice: copy failed");
+    return ok;
+}
+
+bool Simulator::setState(const float4* pos, const float4* vel, uint32_t N, bool fromDevice) {
+    if (!m_initialized || N > m_bufs.capacity) {
+        set_error(PBFX_E_CAPACITY, "setState: N exceeds capacity");
+        return false;
+    }
+    const hipMemcpyKind k = fromDevice ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const size_t bytes = sizeof(float4) * N;
+    bool ok = true;
+    if (pos) {
+        ok = ok && hipMemcpy(m_bufs.d_pos, pos, bytes, k) == hipSuccess;
+        ok = ok && hipMemcpy(m_bufs.d_pos_pred, pos, bytes, k) == hipSuccess;
+    }
+    if (vel) ok = ok && hipMemcpy(m_bufs.d_vel, vel, bytes, k) == hipSuccess;
+    if (!ok) set_error(PBFX_E_INVALID_ARGUMENT, "setState: copy failed");
+    return ok;
+}
+
+bool Simulator::getState(float4* pos, float4* vel, bool toDevice) const {
+    if (!m_initialized) return false;
+    if (hipStreamSynchronize(m_stream) != hipSuccess) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "getState: stream error");
+        return false;
+    }
+    const hipMemcpyKind k = toDevice ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    const size_t bytes = sizeof(float4) * m_params.numParticles;
+    bool ok = true;
+    if (pos) ok = ok && hipMemcpy(pos, m_bufs.d_pos, bytes, k) == hipSuccess;
+    if (vel) ok = ok && hipMemcpy(vel, m_bufs.d_vel, bytes, k) == hipSuccess;
+    return ok;
+}
+
+bool Simulator::hashOverflowed() const {
+    if (!m_grid.table.overflow) return false;
+    uint32_t v = 0;
+    (void)hipStreamSynchronize(m_stream);
+    (void)hipMemcpy(&v, m_grid.table.overflow, sizeof(v), hipMemcpyDeviceToHost);
+    return v != 0;
+}
+
+}  // namespace sim

@@ -1,0 +1,77 @@
+// sorted_path.hpp — launchers of the cell-reordered step used by the headless
+// pipeline.  State lives in original particle order between steps (the
+// reference's layout); inside a step the solver works on copies gathered into
+// cell-sorted order, where every neighbour row is a contiguous slot span.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pbf_params.h"
+
+namespace pbfx {
+
+enum class CellLookup : int { Dense = 0, Compact = 1, Hashed = 2 };
+
+// One slot of the open-addressing cell table: cell key -> [beg, end) of its
+// sorted slots.  (key, epoch) form the 64-bit word claimed by CAS; a slot whose
+// epoch differs from the current step's is empty, so the table is never cleared.
+struct alignas(16) HashSlot {
+    uint32_t key;
+    uint32_t epoch;
+    uint32_t beg;
+    uint32_t end;
+};
+
+struct HashTable {
+    HashSlot* slots = nullptr;
+    uint32_t mask = 0;          // table size - 1 (power of two)
+    uint32_t* epoch = nullptr;  // device counter, advanced once per step
+    uint32_t* overflow = nullptr;  // device flag, set when an insert finds no slot
+};
+
+// Neighbour-row spans: spans[r * stride + m] = [lo, hi) slots of row r of
+// occupied cell m, rows ordered dz-major then dy, (2*reach+1)^2 rows.
+struct SpanTable {
+    uint2* spans = nullptr;
+    uint32_t stride = 0;
+    int reach = 1;
+    int rows = 9;
+};
+
+// P = X + (V + g dt) dt, AABB clamp, write Xn = (P, 1) and its cell key.
+// Thread 0 also advances the hash-table epoch when `epoch` is not null.
+void launch_predict_hash(const float4* X, const float4* V, float4* Xn, uint32_t* keys, uint32_t* epoch,
+                         const sim::SimParams& p, hipStream_t s);
+
+// Insert the M occupied cells (count on device) into the table.
+void launch_hash_insert(const HashTable& t, const uint32_t* uniq, const uint32_t* off, const uint32_t* count,
+                        uint32_t maxCells, hipStream_t s);
+
+// Gather Ps[s] = Xn[idxS[s]] and build every occupied cell's row spans.
+void launch_spans_gather(CellLookup lookup, const float4* Xn, const uint32_t* idxS, float4* Ps, uint32_t N,
+                         const uint32_t* uniq, const uint32_t* off, const uint32_t* count,
+                         const uint32_t* cellStart, const uint32_t* cellEnd, const HashTable& table,
+                         const SpanTable& st, const sim::GridBounds& grid, hipStream_t s);
+
+// lambda of every sorted slot, stored in Ps[s].w.
+void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st, const sim::DeviceParams& dp,
+                          uint32_t N, hipStream_t s);
+
+// Pb[s] = Pa[s] + delta(s).  On the last iteration also clamps to the box and
+// writes Vs[s] = (Pb[s] - X[idxS[s]]) * invDt.
+void launch_delta_sorted(const float4* Pa, float4* Pb, const uint32_t* segOf, const SpanTable& st,
+                         const sim::DeviceParams& dp, uint32_t N, bool last, const float4* X, const uint32_t* idxS,
+                         float4* Vs, float invDt, hipStream_t s);
+
+// No solver iterations: clamp Ps and compute Vs (the post-solve boundary and velocity phases).
+void launch_finalize_sorted(float4* Ps, const float4* X, const uint32_t* idxS, float4* Vs,
+                            const sim::GridBounds& grid, float invDt, uint32_t N, hipStream_t s);
+
+// XSPH (Jacobi over Vs, skipped when xsph_c <= 0), boundary with restitution,
+// scatter Xn[idxS[s]] = (P, 1) and V[idxS[s]] = (v, 0).
+void launch_xsph_scatter(const float4* Ps, const float4* Vs, const uint32_t* segOf, const SpanTable& st,
+                         const sim::DeviceParams& dp, uint32_t N, const uint32_t* idxS, float4* Xn, float4* V,
+                         hipStream_t s);
+
+}  // namespace pbfx

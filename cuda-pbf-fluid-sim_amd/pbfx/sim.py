@@ -1,0 +1,99 @@
+"""Headless simulator handle (include/pbf_sim.h) with numpy host I/O."""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .params import Scene, build_sim_params, f32
+
+GRID_KINDS = {"dense": 0, "compact": 1, "hashed": 2}
+PATHS = {"reordered": 0, "reference": 1}
+
+
+class Simulator:
+    """sim::Simulator over the C ABI: initialize, seed, step, read back."""
+
+    def __init__(self, params, grid="dense", table_log2=22, path="reordered", use_graph=True):
+        self.params = params
+        self.n = int(params.numParticles)
+        L = _lib.lib()
+        h = L.pbfSimCreate(ctypes.byref(params), GRID_KINDS[grid], int(table_log2), PATHS[path], int(use_graph))
+        if not h:
+            _lib.check("pbfSimCreate")
+            raise _lib.PbfError("pbfSimCreate failed")
+        self._h = ctypes.c_void_p(h)
+        self.grid = grid
+        self.path = path
+
+    @classmethod
+    def from_scene(cls, scene: Scene, path="reordered", use_graph=True, grid=None):
+        sp = build_sim_params(scene)
+        sim = cls(sp, grid=grid or scene.grid, table_log2=scene.table_log2, path=path, use_graph=use_graph)
+        sim.seed_scene(scene)
+        return sim
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().pbfSimDestroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def seed_scene(self, scene):
+        amp = float(f32(f32(scene.jitter_frac_r) * f32(scene.radius)))
+        first = 0
+        for bi, (nx, ny, nz, o) in enumerate(scene.lattices):
+            rc = _lib.lib().pbfSimSeedLattice(self._h, first, nx, ny, nz, o[0], o[1], o[2], scene.spacing, amp,
+                                              scene.seed + bi)
+            if rc != 0:
+                _lib.check("pbfSimSeedLattice")
+                raise _lib.PbfError("seed failed")
+            first += nx * ny * nz
+
+    def set_state(self, pos, vel):
+        pos = np.ascontiguousarray(pos, np.float32)
+        vel = np.ascontiguousarray(vel, np.float32)
+        assert pos.shape == (self.n, 4) and vel.shape == (self.n, 4)
+        if _lib.lib().pbfSimSetState(self._h, pos.ctypes.data, vel.ctypes.data, self.n, 0) != 0:
+            _lib.check("pbfSimSetState")
+            raise _lib.PbfError("set_state failed")
+
+    def state(self):
+        pos = np.zeros((self.n, 4), np.float32)
+        vel = np.zeros((self.n, 4), np.float32)
+        if _lib.lib().pbfSimGetState(self._h, pos.ctypes.data, vel.ctypes.data, 0) != 0:
+            _lib.check("pbfSimGetState")
+            raise _lib.PbfError("get_state failed")
+        return pos, vel
+
+    def step(self, steps=1):
+        if _lib.lib().pbfSimStep(self._h, int(steps)) != 0:
+            _lib.check("pbfSimStep")
+            raise _lib.PbfError("step failed")
+
+    def synchronize(self):
+        if _lib.lib().pbfSimSynchronize(self._h) != 0:
+            raise _lib.PbfError("synchronize failed")
+        _lib.check("pbfSimSynchronize")
+
+    @property
+    def stream(self):
+        return _lib.lib().pbfSimStream(self._h)
+
+    def profile_launches(self):
+        """One eager step with an event after every launch: [(label, ms), ...] in launch order."""
+        ms = (ctypes.c_float * 64)()
+        names = ctypes.create_string_buffer(4096)
+        n = _lib.lib().pbfSimProfilePhases(self._h, ctypes.addressof(ms), 64, ctypes.addressof(names), 4096)
+        if n < 0:
+            _lib.check("pbfSimProfilePhases")
+            raise _lib.PbfError("profile failed")
+        labels = names.raw.split(b"\0")[:n]
+        return [(l.decode(), float(ms[i])) for i, l in enumerate(labels)]
+
+    def hash_overflow(self):
+        return bool(_lib.lib().pbfSimHashOverflow(self._h))

@@ -1,0 +1,69 @@
+// pbf_sim.h — C ABI of the headless simulator in libpbfx.so.
+//
+// A handle over sim::Simulator (reference: sim/simulator.h:47-90 —
+// initialize / step / seedBoxLattice / devicePositions / deviceVel), minus the
+// D3D12 interop.  State stays on the device between steps in the reference's
+// layout (float4 AoS, original particle order); the position ping-pong swap
+// happens after every step.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime.h>
+
+#include "pbf_params.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct PbfSim PbfSim;
+
+#define PBF_GRID_DENSE 0    /* cellStart/cellEnd over every cell (reference: dense grid) */
+#define PBF_GRID_COMPACT 1  /* unique keys + offsets, binary search (reference: "hashed" grid) */
+#define PBF_GRID_HASHED 2   /* open-addressing cell table, 2^tableLog2 slots */
+
+#define PBF_PATH_REORDERED 0  /* cell-sorted solver (default) */
+#define PBF_PATH_REFERENCE 1  /* phases over the drop-in Launch* functions */
+
+// reference: Simulator::initialize (simulator.cpp:217).  Returns NULL on error
+// (see pbfGetLastError).  `params->grid.dim` must be set.
+PbfSim* pbfSimCreate(const sim::SimParams* params, int gridKind, int tableLog2, int path, int useGraph);
+void pbfSimDestroy(PbfSim* sim);
+
+// reference: Simulator::seedBoxLattice (simulator.cpp:808-878), placed at particle offset `first`,
+// with a per-axis jitter of +-jitterAmp drawn from SplitMix64(seed, particle, axis).
+int pbfSimSeedLattice(PbfSim* sim, uint32_t first, uint32_t nx, uint32_t ny, uint32_t nz, float ox, float oy,
+                      float oz, float spacing, float jitterAmp, uint64_t seed);
+// Copy state in/out (host or device pointers, N = params->numParticles on output).
+int pbfSimSetState(PbfSim* sim, const float4* pos, const float4* vel, uint32_t N, int fromDevice);
+int pbfSimGetState(PbfSim* sim, float4* pos, float4* vel, int toDevice);
+
+// reference: Simulator::step (simulator.cpp:560-643).  Enqueues `steps` steps on the
+// simulator's stream (graph replay when enabled) without synchronising.
+int pbfSimStep(PbfSim* sim, int steps);
+// New parameters for the following steps (same grid and particle count).
+int pbfSimSetParams(PbfSim* sim, const sim::SimParams* params);
+int pbfSimSynchronize(PbfSim* sim);
+hipStream_t pbfSimStream(PbfSim* sim);
+const float4* pbfSimDevicePositions(PbfSim* sim);
+const float4* pbfSimDeviceVelocities(PbfSim* sim);
+// Sorted keys / indices of the last grid build (device pointers).
+const uint32_t* pbfSimDeviceSortedKeys(PbfSim* sim);
+const uint32_t* pbfSimDeviceSortedIndices(PbfSim* sim);
+
+// One eager step with an event around every phase.  Writes up to maxPhases
+// times (ms) and NUL-separated names into names (namesBytes long); returns the phase count.
+int pbfSimProfilePhases(PbfSim* sim, float* ms, int maxPhases, char* names, int namesBytes);
+// 1 when a hash-table insert found no free slot since creation.
+int pbfSimHashOverflow(PbfSim* sim);
+
+// ABI self-checks (host only): copy by-value structs back out, so a foreign
+// caller can verify it passes them with the same layout and calling convention.
+void pbfDebugEchoDeviceParams(sim::DeviceParams dp, sim::DeviceParams* out);
+void pbfDebugEchoGridBounds(sim::GridBounds g, sim::GridBounds* out);
+void pbfDebugEchoFloat3(float3 v, float* out3);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,386 @@
+"""Pins of the CPU oracle (no GPU).
+
+The reference ships no tests or golden vectors, so each oracle function is pinned
+by mathematics that a plausible mistake would break: closed forms, brute force
+without a grid, library routines (numpy lexsort / unique), invariants and
+symmetry.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from pbfx import params as P
+
+H = float(P.f32(P.f32(2.5) * P.f32(0.005)))
+
+
+def small_params(**kw):
+    scene = P.small_block((6, 6, 6), **kw)
+    return scene, P.build_sim_params(scene)
+
+
+# ---------------------------------------------------------------- A1 coefficients
+def test_kernel_coeffs_closed_form(oracle):
+    kc = P.KernelCoeffs()
+    oracle.make_kernel_coeffs(0.0125, kc)
+    h = 0.0125
+    assert kc.poly6 == pytest.approx(315.0 / (64.0 * math.pi * h ** 9), rel=2e-6)
+    assert kc.spiky == pytest.approx(15.0 / (math.pi * h ** 6), rel=2e-6)
+    assert kc.visc == pytest.approx(15.0 / (2.0 * math.pi * h ** 3), rel=2e-6)
+    assert kc.poly6 == pytest.approx(2.10276e17, rel=1e-5)
+    assert kc.spiky == pytest.approx(1.25165e12, rel=1e-5)
+    # the product's float32 restatement is bit-identical
+    mine = P.make_kernel_coeffs(0.0125)
+    assert bytes(mine) == bytes(kc)
+
+
+def test_device_params_from_sim_params(oracle):
+    _, sp = small_params()
+    dp = P.DeviceParams()
+    oracle.make_device_params(sp, dp)
+    assert bytes(dp) == bytes(P.make_device_params(sp))
+    assert dp.inv_dt == np.float32(1.0) / np.float32(sp.dt)
+    assert dp.pbf.compliance == pytest.approx(2e-5)
+
+
+def test_grid_dims(oracle):
+    _, sp = small_params()
+    assert oracle.compute_dims(sp.grid) == tuple(sp.grid.dim)
+    # float32 h = 2.5f * 0.005f rounds just below 0.0125, so a 1 m box needs 81 cells
+    assert tuple(sp.grid.dim) == (81, 81, 81)
+    sp2 = P.build_sim_params(P.dam_break_1m())
+    assert tuple(sp2.grid.dim) == (161, 81, 81)
+
+
+# -------------------------------------------------------- A3/A4/A13 streaming
+def test_integrate_boundary_velocity_closed_forms(oracle):
+    rng = np.random.default_rng(1)
+    X = rng.uniform(-0.1, 1.1, (257, 4)).astype(np.float32)
+    X[:, 3] = rng.uniform(0, 2, 257)
+    V = rng.normal(0, 2, (257, 4)).astype(np.float32)
+    g, dt = (0.0, -9.81, 0.5), np.float32(1 / 120)
+    Pp = oracle.integrate_pred(X, V, g, dt)
+    f = np.float32
+    for a in range(3):
+        v = (V[:, a] + f(g[a]) * dt).astype(np.float32)
+        np.testing.assert_array_equal(Pp[:, a], (X[:, a] + (v * dt).astype(np.float32)).astype(np.float32))
+    np.testing.assert_array_equal(Pp[:, 3], X[:, 3])
+
+    _, sp = small_params()
+    e = 0.4
+    Pc, Vc = oracle.boundary(Pp, V, sp.grid, e)
+    lo = np.array(list(sp.grid.mins), np.float32)
+    hi = np.array(list(sp.grid.maxs), np.float32)
+    np.testing.assert_array_equal(Pc[:, :3], np.minimum(np.maximum(Pp[:, :3], lo), hi))
+    assert np.all(Pc[:, 3] == 1.0) and np.all(Vc[:, 3] == 0.0)
+    hit = (Pp[:, :3] < lo).astype(int) + (Pp[:, :3] > hi).astype(int)
+    expect = V[:, :3] * np.where(hit > 0, np.float32(-e), np.float32(1.0))
+    np.testing.assert_array_equal(Vc[:, :3], expect)
+
+    inv_dt = np.float32(120.0)
+    Vn = oracle.velocity(V, X, Pc, inv_dt)
+    np.testing.assert_array_equal(Vn[:, :3], ((Pc[:, :3] - X[:, :3]) * inv_dt).astype(np.float32))
+    assert np.all(Vn[:, 3] == 0)
+
+
+# ------------------------------------------------------------------ A5 keys
+def test_hash_keys_brute_force(oracle):
+    _, sp = small_params()
+    g = sp.grid
+    cs = np.float32(g.cellSize)
+    rng = np.random.default_rng(2)
+    pts = rng.uniform(-0.2, 1.2, (500, 4)).astype(np.float32)
+    # particles exactly on cell faces and on the box faces
+    faces = (np.arange(20, dtype=np.float32) * cs).astype(np.float32)
+    pts[:20, 0] = faces
+    pts[20, :3] = [1.0, 1.0, 1.0]
+    pts[21, :3] = [0.0, 0.0, 0.0]
+    pts[22, :3] = [np.nan, 0.5, np.inf]
+    pts[23, :3] = [-np.inf, 1e30, -1e30]
+    keys, idx = oracle.hash_keys(pts, g)
+    dims = np.array(list(g.dim))
+    cells = []
+    with np.errstate(invalid="ignore"):
+        for a, lo in enumerate(g.mins):
+            rel = ((pts[:, a] - np.float32(lo)) / cs).astype(np.float32)
+            fl = np.floor(rel.astype(np.float64))
+            c = np.where(np.isnan(fl), 0, np.clip(fl, -2.0 ** 31, 2.0 ** 31 - 1)).astype(np.int64)
+            cells.append(np.clip(c, 0, dims[a] - 1))
+    expect = (cells[2] * dims[1] + cells[1]) * dims[0] + cells[0]
+    np.testing.assert_array_equal(keys, expect.astype(np.uint32))
+    np.testing.assert_array_equal(idx, np.arange(500, dtype=np.uint32))
+
+
+# ------------------------------------------------------------------ A6 sort
+@pytest.mark.parametrize("n", [0, 1, 2, 1000, 65537])
+def test_sort_is_lexicographic(oracle, n):
+    rng = np.random.default_rng(n)
+    keys = rng.integers(0, max(n // 3, 1), n).astype(np.uint32)
+    vals = rng.permutation(n).astype(np.uint32)
+    ko, vo = oracle.sort_pairs(keys, vals)
+    order = np.lexsort((np.arange(n), keys))  # stable: ties keep input order
+    np.testing.assert_array_equal(ko, keys[order])
+    np.testing.assert_array_equal(vo, vals[order])
+
+
+# ---------------------------------------------------------- A7/A8/A9 ranges
+def test_ranges_and_compact(oracle):
+    rng = np.random.default_rng(3)
+    C = 5000
+    keys = np.sort(rng.integers(0, C, 3000)).astype(np.uint32)
+    start, end = oracle.cell_ranges(keys, C)
+    uk, first, counts = np.unique(keys, return_index=True, return_counts=True)
+    exp_s = np.full(C, 0xFFFFFFFF, np.uint32)
+    exp_e = np.zeros(C, np.uint32)
+    exp_s[uk] = first
+    exp_e[uk] = first + counts
+    np.testing.assert_array_equal(start, exp_s)
+    np.testing.assert_array_equal(end, exp_e)
+    assert int((end.astype(np.int64) - np.where(start == 0xFFFFFFFF, 0, start)).sum()) == len(keys)
+    uniq, off, m = oracle.cell_compact(keys)
+    np.testing.assert_array_equal(uniq, uk)
+    np.testing.assert_array_equal(off[:-1], first)
+    assert off[-1] == len(keys) and m == len(uk)
+    # compact lookup equals the dense ranges for every cell
+    for c in range(0, C, 7):
+        ok, b, e = oracle.compact_cell_range(uniq, off, c)
+        if start[c] == 0xFFFFFFFF:
+            assert not ok and b == 0 and e == 0
+        else:
+            assert ok and (b, e) == (start[c], end[c])
+    # empty inputs
+    s0, e0 = oracle.cell_ranges(np.zeros(0, np.uint32), 10)
+    assert np.all(s0 == 0xFFFFFFFF) and np.all(e0 == 0)
+    u0, o0, m0 = oracle.cell_compact(np.zeros(0, np.uint32))
+    assert m0 == 0 and list(o0) == [0]
+
+
+# ------------------------------------------------------------ neighbour sums
+def _grid_state(oracle, X, sp):
+    keys, idx = oracle.hash_keys(X, sp.grid)
+    keyS, idxS = oracle.sort_pairs(keys, idx)
+    start, end = oracle.cell_ranges(keyS, sp.grid.num_cells)
+    uniq, off, _ = oracle.cell_compact(keyS)
+    return keyS, idxS, start, end, uniq, off
+
+
+def _brute_lambda(X, sp):
+    """lambda from an O(N^2) neighbour search in float64 (no grid, no cap)."""
+    kc, m, rho0 = sp.kernel, sp.particleMass, sp.restDensity
+    x = X[:, :3].astype(np.float64)
+    d = x[:, None, :] - x[None, :, :]
+    r2 = (d ** 2).sum(-1)
+    n = len(x)
+    inr = (r2 <= kc.h2) & ~np.eye(n, dtype=bool)
+    r = np.sqrt(r2)
+    W = np.where(inr, kc.poly6 * (kc.h2 - r2) ** 3, 0.0)
+    rho = m * W.sum(1)
+    coeff = np.where(inr, -3.0 * kc.spiky * (kc.h - r) ** 2 / np.maximum(r, sp.pbf.grad_r_eps), 0.0)
+    g = coeff[..., None] * d
+    gs = g.sum(1)
+    s2 = (g ** 2).sum(-1).sum(1)
+    scale = m / rho0
+    denom = scale * scale * (s2 + (gs ** 2).sum(1)) + sp.pbf.lambda_denom_eps + sp.pbf.compliance / sp.dt ** 2
+    lam = -(rho / rho0 - 1.0) / denom
+    lam = np.clip(lam, -sp.pbf.lambda_max_abs, sp.pbf.lambda_max_abs)
+    lam[(inr.sum(1) < 2)] = 0.0
+    return lam, rho, inr.sum(1)
+
+
+def test_lambda_matches_brute_force(oracle):
+    scene, sp = small_params()
+    X = P.seed_scene_reference(scene, oracle.seed_lattice)
+    # compress the block a little so neighbourhoods are non-trivial
+    X[:, :3] = (X[:, :3] * np.float32(0.9) + np.float32(0.2)).astype(np.float32)
+    keyS, idxS, start, end, uniq, off = _grid_state(oracle, X, sp)
+    dp = P.make_device_params(sp)
+    lam = oracle.lambda_dense(X, idxS, keyS, start, end, dp)
+    lam_c = oracle.lambda_compact(X, idxS, keyS, uniq, off, dp)
+    np.testing.assert_array_equal(lam, lam_c)
+    ref, rho, nn = _brute_lambda(X, sp)
+    assert nn.max() < sp.maxNeighbors  # cap inactive here
+    np.testing.assert_allclose(lam, ref, rtol=2e-4, atol=1e-7)
+
+
+def test_lambda_neighbor_cap_order(oracle):
+    """With maxNeighbors = k, only the first k in-range neighbours in z->y->x cell order and
+    ascending sorted slot contribute."""
+    scene, sp = small_params()
+    X = P.seed_scene_reference(scene, oracle.seed_lattice)
+    X[:, :3] = (X[:, :3] * np.float32(0.7) + np.float32(0.2)).astype(np.float32)
+    sp.maxNeighbors = 5
+    keyS, idxS, start, end, _, _ = _grid_state(oracle, X, sp)
+    dp = P.make_device_params(sp)
+    lam = oracle.lambda_dense(X, idxS, keyS, start, end, dp)
+    kc, dims = sp.kernel, tuple(sp.grid.dim)
+    slot_of = np.empty(len(X), np.int64)
+    slot_of[idxS] = np.arange(len(X))
+    k64 = keyS.astype(np.int64)
+    cell_xyz = np.stack([k64 % dims[0], (k64 // dims[0]) % dims[1], k64 // (dims[0] * dims[1])], 1)
+    checked_capped = 0
+    for s in range(0, len(X), 7):
+        i = idxS[s]
+        cx, cy, cz = (int(v) for v in cell_xyz[s])
+        cand = []
+        for dz in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    x, y, z = cx + dx, cy + dy, cz + dz
+                    if not (0 <= x < dims[0] and 0 <= y < dims[1] and 0 <= z < dims[2]):
+                        continue
+                    c = (z * dims[1] + y) * dims[0] + x
+                    if start[c] == 0xFFFFFFFF:
+                        continue
+                    cand.extend(idxS[start[c]:end[c]])
+        used = []
+        for j in cand:
+            if j == i:
+                continue
+            r2 = float(((X[i, :3].astype(np.float64) - X[j, :3]) ** 2).sum())
+            if r2 <= kc.h2:
+                used.append(j)
+            if len(used) == sp.maxNeighbors:
+                checked_capped += 1
+                break
+        if len(used) < 2:
+            assert lam[i] == 0.0
+            continue
+        # lambda from exactly the first k neighbours, in float64
+        d = X[i, :3].astype(np.float64) - X[used, :3].astype(np.float64)
+        r = np.sqrt((d ** 2).sum(1))
+        rho = sp.particleMass * (kc.poly6 * (kc.h2 - r ** 2) ** 3).sum()
+        g = (-3.0 * kc.spiky * (kc.h - r) ** 2 / r)[:, None] * d
+        scale = sp.particleMass / sp.restDensity
+        denom = scale ** 2 * ((g ** 2).sum() + (g.sum(0) ** 2).sum()) + sp.pbf.lambda_denom_eps \
+            + sp.pbf.compliance / sp.dt ** 2
+        ref = np.clip(-(rho / sp.restDensity - 1.0) / denom, -sp.pbf.lambda_max_abs, sp.pbf.lambda_max_abs)
+        assert lam[i] == pytest.approx(ref, rel=1e-4)
+    assert checked_capped > 0
+
+
+def test_isolated_particle_is_ballistic(oracle):
+    scene, sp = small_params()
+    X = np.array([[0.5, 0.5, 0.5, 1.0]], np.float32)
+    V = np.array([[0.3, 0.1, -0.2, 0.0]], np.float32)
+    sp.numParticles = 1
+    X1, V1 = oracle.run(X, V, sp, 1)
+    f = np.float32
+    dt = f(sp.dt)
+    g = np.array(list(sp.gravity), np.float32)
+    v = (V[0, :3] + (g * dt).astype(np.float32)).astype(np.float32)
+    p = (X[0, :3] + (v * dt).astype(np.float32)).astype(np.float32)
+    np.testing.assert_array_equal(X1[0, :3], p)
+    # velocity = (p - x) / dt, XSPH has no neighbour to smooth against
+    np.testing.assert_array_equal(V1[0, :3], ((p - X[0, :3]) * (f(1.0) / dt)).astype(np.float32))
+
+
+def test_two_particles_scorr_closed_form(oracle):
+    """Two particles within h: n = 1 < 2 so lambda = 0 and delta is the s_corr term only."""
+    _, sp = small_params()
+    sp.numParticles = 2
+    h = sp.kernel.h
+    X = np.array([[0.5, 0.5, 0.5, 1.0], [0.5 + 0.4 * h, 0.5, 0.5, 1.0]], np.float32)
+    keyS, idxS, start, end, _, _ = _grid_state(oracle, X, sp)
+    dp = P.make_device_params(sp)
+    lam = oracle.lambda_dense(X, idxS, keyS, start, end, dp)
+    assert np.all(lam == 0)
+    P2, delta = oracle.delta_apply_dense(X, lam, idxS, keyS, start, end, dp)
+    kc, pbf = sp.kernel, sp.pbf
+    r = float(np.float64(X[1, 0]) - np.float64(X[0, 0]))
+    w = kc.poly6 * (kc.h2 - r * r) ** 3
+    dq = pbf.scorr_dq_h * h
+    wq = kc.poly6 * (kc.h2 - dq * dq) ** 3
+    scorr = max(-pbf.scorr_k * (w / wq) ** pbf.scorr_n, pbf.scorr_min)
+    grad = -3.0 * kc.spiky * (h - r) ** 2 / r  # times r_ij (x component)
+    dx0 = scorr * grad * (-r) / sp.restDensity * pbf.relax_omega
+    cap = pbf.disp_clamp_max_h * h
+    dx0 = max(-cap, min(cap, dx0))
+    assert delta[0, 0] == pytest.approx(dx0, rel=1e-4)
+    assert delta[1, 0] == pytest.approx(-dx0, rel=1e-4)
+    assert delta[0, 1] == 0 and delta[0, 2] == 0
+    np.testing.assert_array_equal(P2[:, 0], (X[:, 0] + delta[:, 0]).astype(np.float32))
+
+
+def test_lattice_interior_symmetry(oracle):
+    """A perfect lattice with uniform lambda: interior gradient sums cancel, so delta ~ 0.
+    (lambda is kept at 1e-9: |grad W| ~ 2e7 at these SI parameters, so float32 rounding of the
+    lattice positions alone would otherwise saturate the 0.05 h displacement clamp.)"""
+    scene = P.small_block((9, 9, 9), jitter_frac_r=0.0)
+    sp = P.build_sim_params(scene)
+    sp.pbf.scorr_enable = 0
+    X = P.seed_scene_reference(scene, oracle.seed_lattice)
+    X[:, :3] += np.float32(0.3)
+    keyS, idxS, start, end, _, _ = _grid_state(oracle, X, sp)
+    dp = P.make_device_params(sp)
+    lam = np.full(len(X), np.float32(-1e-9))
+    _, delta = oracle.delta_apply_dense(X, lam, idxS, keyS, start, end, dp)
+    c = np.array([4, 4, 4])
+    ijk = np.stack(np.meshgrid(np.arange(9), np.arange(9), np.arange(9), indexing="ij"), -1).reshape(-1, 3)[:, ::-1]
+    interior = np.all(np.abs(ijk - c) <= 2, axis=1)
+    edge = np.abs(delta[~interior, :3]).max()
+    assert edge > 1e-7  # boundary particles do move
+    assert np.abs(delta[interior, :3]).max() < 1e-3 * edge
+
+
+def test_xsph_uniform_velocity_invariant(oracle):
+    scene, sp = small_params()
+    X = P.seed_scene_reference(scene, oracle.seed_lattice)
+    keyS, idxS, start, end, uniq, off = _grid_state(oracle, X, sp)
+    dp = P.make_device_params(sp)
+    V = np.tile(np.array([[0.25, -1.5, 0.125, 0.0]], np.float32), (len(X), 1))
+    out = oracle.xsph_dense(V, X, idxS, keyS, start, end, dp)
+    np.testing.assert_array_equal(out[:, :3], V[:, :3])
+    out_c = oracle.xsph_compact(V, X, idxS, keyS, uniq, off, dp)
+    np.testing.assert_array_equal(out_c, out)
+
+
+def test_xsph_matches_brute_force(oracle):
+    scene, sp = small_params()
+    X = P.seed_scene_reference(scene, oracle.seed_lattice)
+    X[:, :3] = (X[:, :3] * np.float32(0.85) + np.float32(0.2)).astype(np.float32)
+    rng = np.random.default_rng(5)
+    V = rng.normal(0, 1, (len(X), 4)).astype(np.float32)
+    keyS, idxS, start, end, _, _ = _grid_state(oracle, X, sp)
+    dp = P.make_device_params(sp)
+    out = oracle.xsph_dense(V, X, idxS, keyS, start, end, dp)
+    x = X[:, :3].astype(np.float64)
+    r2 = ((x[:, None] - x[None]) ** 2).sum(-1)
+    inr = (r2 <= sp.kernel.h2) & ~np.eye(len(x), dtype=bool)
+    W = np.where(inr, sp.kernel.poly6 * (sp.kernel.h2 - r2) ** 3, 0.0)
+    v = V[:, :3].astype(np.float64)
+    dv = (W[..., None] * (v[None, :, :] - v[:, None, :])).sum(1)
+    ref = v + sp.xsph_c * (sp.particleMass / sp.restDensity) * dv
+    np.testing.assert_allclose(out[:, :3], ref, rtol=1e-5, atol=1e-6)
+
+
+def test_step_dense_equals_compact(oracle):
+    scene, sp = small_params()
+    X = P.seed_scene_reference(scene, oracle.seed_lattice)
+    V = np.zeros_like(X)
+    a = oracle.run(X, V, sp, 5, compact=False)
+    b = oracle.run(X, V, sp, 5, compact=True)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_seed_lattice_generator(oracle):
+    """Lattice x fastest, clamped, plus SplitMix64 jitter (restated here with Python integers)."""
+    M = (1 << 64) - 1
+
+    def u(seed, p, a):
+        z = (seed + 0x9E3779B97F4A7C15 * (p * 3 + a + 1)) & M
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        z ^= z >> 31
+        return np.float32((z >> 40) / 16777216.0)
+
+    _, sp = small_params()
+    X = oracle.seed_lattice(3, 2, 2, (0.005, 0.005, 0.005), 0.01, sp.grid, 0.0005, 77)
+    f = np.float32
+    for i, (ix, iy, iz) in enumerate([(x, y, z) for z in range(2) for y in range(2) for x in range(3)]):
+        base = [f(f(0.005) + f(f(ix) * f(0.01))), f(f(0.005) + f(f(iy) * f(0.01))), f(f(0.005) + f(f(iz) * f(0.01)))]
+        for a in range(3):
+            j = f(f(f(2.0) * u(77, i, a) - f(1.0)) * f(0.0005))
+            assert X[i, a] == f(min(max(f(base[a] + j), f(0.0)), f(1.0)))
+        assert X[i, 3] == 1.0
