@@ -160,29 +160,89 @@ struct HashRowsDev {
     }
 };
 
+// ------------------------------------------------------------------------
+// Deferred neighbour walk.  A lane scans its candidates (one 16-byte load and a
+// distance test each) and appends the in-range slots to a small FIFO in LDS.
+// The expensive per-pair work (sqrt, IEEE divisions, the s_corr power) runs only
+// when some lane's FIFO is full or every lane has finished scanning, and then for
+// one queued pair on every lane that has one.  Without this, a wave executes the
+// per-pair path whenever any of its 64 lanes finds a neighbour (~15% of
+// candidates each), i.e. almost every iteration at low utilisation.  Each lane
+// still consumes its own pairs in scan order, so sums keep the reference order.
+constexpr int kQD = 8;  // FIFO depth per lane
+
+struct Walk {
+    uint32_t k = 0, hi = 0;
+    int r = -1;
+    bool scanning = true;
+    int qh = 0, qn = 0;
+};
+
+// Advance to the next candidate slot of lane's rows; false when the rows are exhausted.
+__device__ __forceinline__ bool next_candidate(Walk& w, uint32_t m, const SpanTable& st, uint32_t& cand) {
+    while (w.k >= w.hi) {
+        if (w.r + 1 >= st.rows) return false;
+        ++w.r;
+        const uint2 sp = st.spans[(uint64_t)w.r * st.stride + m];
+        w.k = sp.x;
+        w.hi = sp.y;
+    }
+    cand = w.k++;
+    return true;
+}
+
+template <class InRange, class Apply>
+__device__ __forceinline__ void deferred_walk(uint32_t s, uint32_t m, const SpanTable& st, uint32_t* q, int cap,
+                                              InRange&& in_range, Apply&& apply) {
+    Walk w;
+    int found = 0;
+    while (true) {
+        if (w.scanning && w.qn < kQD) {
+            uint32_t cand;
+            if (cap > 0 && found >= cap) {
+                w.scanning = false;
+            } else if (!next_candidate(w, m, st, cand)) {
+                w.scanning = false;
+            } else if (cand != s && in_range(cand)) {
+                q[((w.qh + w.qn) & (kQD - 1)) * kBS] = cand;
+                ++w.qn;
+                ++found;
+            }
+        }
+        const bool anyScanning = __any(w.scanning);
+        if (__any(w.qn == kQD) || !anyScanning) {
+            if (w.qn > 0) {
+                const uint32_t kk = q[w.qh * kBS];
+                w.qh = (w.qh + 1) & (kQD - 1);
+                --w.qn;
+                apply(kk);
+            }
+            if (!anyScanning && !__any(w.qn > 0)) break;
+        }
+    }
+}
+
 __global__ __launch_bounds__(kBS) void k_lambda_sorted(float4* Ps, const uint32_t* __restrict__ segOf, SpanTable st,
                                                        sim::DeviceParams dp, uint32_t N) {
+    __shared__ uint32_t sQ[kQD * kBS];
     const uint32_t s = slot_of_thread();
     if (s >= N) return;
     const float4 pi = Ps[s];
     const uint32_t m = segOf[s];
     const PairConsts c = make_pair_consts(dp);
-    const int cap = dp.maxNeighbors;
-    const bool hasCap = cap > 0;
     LambdaAcc a;
-    for (int r = 0; r < st.rows; ++r) {
-        const uint2 sp = st.spans[(uint64_t)r * st.stride + m];
-        for (uint32_t k = sp.x; k < sp.y; ++k) {
-            if (hasCap && a.n >= cap) goto done;
-            if (k == s) continue;
+    deferred_walk(
+        s, m, st, sQ + threadIdx.x, dp.maxNeighbors,
+        [&](uint32_t k) {
             const float4 pj = Ps[k];
             const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            const float r2 = rx * rx + ry * ry + rz * rz;
-            if (r2 > c.h2) continue;
-            lambda_pair(a, rx, ry, rz, r2, c);
-        }
-    }
-done:
+            return rx * rx + ry * ry + rz * rz <= c.h2;
+        },
+        [&](uint32_t k) {
+            const float4 pj = Ps[k];
+            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+            lambda_pair(a, rx, ry, rz, rx * rx + ry * ry + rz * rz, c);
+        });
     reinterpret_cast<float*>(Ps)[4ull * s + 3] = lambda_finish(a, dp);
 }
 
@@ -192,23 +252,25 @@ __global__ __launch_bounds__(kBS) void k_delta_sorted(const float4* __restrict__
                                                       const float4* __restrict__ X,
                                                       const uint32_t* __restrict__ idxS, float4* __restrict__ Vs,
                                                       float invDt) {
+    __shared__ uint32_t sQ[kQD * kBS];
     const uint32_t s = slot_of_thread();
     if (s >= N) return;
     const float4 pi = Pa[s];
     const uint32_t m = segOf[s];
     const PairConsts c = make_pair_consts(dp);
     float dx = 0.f, dy = 0.f, dz = 0.f;
-    for (int r = 0; r < st.rows; ++r) {
-        const uint2 sp = st.spans[(uint64_t)r * st.stride + m];
-        for (uint32_t k = sp.x; k < sp.y; ++k) {
-            if (k == s) continue;
+    deferred_walk(
+        s, m, st, sQ + threadIdx.x, 0,
+        [&](uint32_t k) {
             const float4 pj = Pa[k];
             const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            const float r2 = rx * rx + ry * ry + rz * rz;
-            if (r2 > c.h2) continue;
-            delta_pair(dx, dy, dz, rx, ry, rz, r2, pi.w, pj.w, c);
-        }
-    }
+            return rx * rx + ry * ry + rz * rz <= c.h2;
+        },
+        [&](uint32_t k) {
+            const float4 pj = Pa[k];
+            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+            delta_pair(dx, dy, dz, rx, ry, rz, rx * rx + ry * ry + rz * rz, pi.w, pj.w, c);
+        });
     const float3 d = delta_finish(dx, dy, dz, dp);
     // reference: pbf_delta_apply.cu:232-233 (P += delta)
     float px = pi.x + d.x, py = pi.y + d.y, pz = pi.z + d.z;
@@ -242,6 +304,7 @@ __global__ __launch_bounds__(kBS) void k_xsph_scatter(const float4* __restrict__
                                                       sim::DeviceParams dp, uint32_t N,
                                                       const uint32_t* __restrict__ idxS, float4* __restrict__ Xn,
                                                       float4* __restrict__ V) {
+    __shared__ uint32_t sQ[kQD * kBS];
     const uint32_t s = slot_of_thread();
     if (s >= N) return;
     const float4 pi = Ps[s];
@@ -252,23 +315,24 @@ __global__ __launch_bounds__(kBS) void k_xsph_scatter(const float4* __restrict__
         const float h2 = dp.kernel.h2, poly6 = dp.kernel.poly6;
         float sx = 0.f, sy = 0.f, sz = 0.f;
         int n = 0;
-        for (int r = 0; r < st.rows; ++r) {
-            const uint2 sp = st.spans[(uint64_t)r * st.stride + m];
-            for (uint32_t k = sp.x; k < sp.y; ++k) {
-                if (k == s) continue;
+        deferred_walk(
+            s, m, st, sQ + threadIdx.x, 0,
+            [&](uint32_t k) {
                 const float4 pj = Ps[k];
                 const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-                const float r2 = rx * rx + ry * ry + rz * rz;
-                if (r2 > h2) continue;
-                const float hr2 = h2 - r2;
+                return rx * rx + ry * ry + rz * rz <= h2;
+            },
+            [&](uint32_t k) {
+                const float4 pj = Ps[k];
+                const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+                const float hr2 = h2 - (rx * rx + ry * ry + rz * rz);
                 const float w = poly6 * hr2 * hr2 * hr2;
                 const float4 vj = Vs[k];
                 sx += (vj.x - vi.x) * w;
                 sy += (vj.y - vi.y) * w;
                 sz += (vj.z - vi.z) * w;
                 ++n;
-            }
-        }
+            });
         const float3 v = xsph_finish(vi.x, vi.y, vi.z, sx, sy, sz, n, dp);
         vx = v.x; vy = v.y; vz = v.z;
     }

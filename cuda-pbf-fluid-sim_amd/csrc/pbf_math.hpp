@@ -6,7 +6,7 @@
 //   density += ((m * poly6) * hr2) * hr2 * hr2          pbf_lambda.cu:73-74
 //   coeff    = ((-3 * spiky) * (t * t)) / max(r, eps)  pbf_lambda.cu:76-79
 //   lambda   = -(rho/rho0 - 1) / max(denom, eps)       pbf_lambda.cu:86-107
-//   s_corr   = -k * pow(W / W(dq), n), floored         pbf_delta_apply.cu:77-84
+//   s_corr   = -k * (W / W(dq))^n, floored             pbf_delta_apply.cu:77-84
 //   delta   += (lambda_i + lambda_j + s_corr) * grad   pbf_delta_apply.cu:86-89
 //   dv      += (v_j - v_i) * W                         xsph.cu:66-72
 #pragma once
@@ -15,12 +15,38 @@
 
 namespace pbfx {
 
+// s_corr's pow(W / W(dq), n).  An integral exponent in [1, 16] is evaluated as a
+// product in double by binary exponentiation (x*x is exact in double, so n = 4
+// rounds once in double and once to float: the correctly rounded power except in
+// double-rounding ties); other exponents use powf.  The CPU oracle evaluates the
+// same product in the same order, so both sides produce the same bits.  The
+// reference calls powf (CUDA: <= 4 ulp), pbf_delta_apply.cu:81.
+__device__ __forceinline__ float pow_scorr(float x, float n, int ni) {
+    if (ni > 0) {
+        double b = (double)x, r = 1.0;
+        int e = ni;
+        while (true) {
+            if (e & 1) r *= b;
+            e >>= 1;
+            if (!e) break;
+            b *= b;
+        }
+        return (float)r;
+    }
+    return powf(x, n);
+}
+
+// The integral exponent of pow_scorr (0 when n is not an integer in [1, 16]).
+__device__ __forceinline__ int scorr_int_exponent(float n) {
+    return (n >= 1.0f && n <= 16.0f && (float)(int)n == n) ? (int)n : 0;
+}
+
 // Constants hoisted out of the neighbour loops (each is the first product the
 // reference's left-to-right expression forms, so hoisting does not change bits).
 struct PairConsts {
     float h, h2, poly6, m_poly6, neg3_spiky, grad_r_eps;
     float w_q, neg_scorr_k, scorr_n, scorr_min;
-    int scorr_enable;
+    int scorr_enable, scorr_ni;
 };
 
 __device__ __forceinline__ PairConsts make_pair_consts(const sim::DeviceParams& dp) {
@@ -38,6 +64,7 @@ __device__ __forceinline__ PairConsts make_pair_consts(const sim::DeviceParams& 
     c.scorr_n = dp.pbf.scorr_n;
     c.scorr_min = dp.pbf.scorr_min;
     c.scorr_enable = dp.pbf.scorr_enable;
+    c.scorr_ni = scorr_int_exponent(dp.pbf.scorr_n);
     return c;
 }
 
@@ -93,7 +120,7 @@ __device__ __forceinline__ void delta_pair(float& dx, float& dy, float& dz, floa
         const float hr2 = c.h2 - r2;
         const float w = c.poly6 * hr2 * hr2 * hr2;
         const float ratio = (c.w_q > 0.f) ? (w / c.w_q) : 0.0f;
-        scorr = c.neg_scorr_k * powf(ratio, c.scorr_n);
+        scorr = c.neg_scorr_k * pow_scorr(ratio, c.scorr_n, c.scorr_ni);
         if (c.scorr_min < 0.f) scorr = fmaxf(scorr, c.scorr_min);
     }
     const float lij = (lam_i + lam_j + scorr);

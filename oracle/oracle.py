@@ -69,6 +69,7 @@ def _declare(L):
         "oracle_density_stats": (None, [_P, _P, _P, _P]),
         "oracle_seed_lattice": (None, [_P, _U32, _U32, _U32, _F, _F, _F, _F, _P, _F, ctypes.c_uint64]),
         "oracle_jitter_uniform": (_F, [ctypes.c_uint64, ctypes.c_uint64, _U32]),
+        "oracle_pow_scorr": (_F, [_F, _F]),
         "oracle_num_threads": (ctypes.c_int, []),
     }
     for name, (res, args) in sig.items():
@@ -265,6 +266,10 @@ def seed_lattice(nx, ny, nz, origin, spacing, grid, jitter_amp, seed):
 
 def jitter_uniform(seed, particle, axis):
     return lib().oracle_jitter_uniform(seed, particle, axis)
+
+
+def pow_scorr(x, n):
+    return lib().oracle_pow_scorr(float(x), float(n))
 
 
 def num_threads():

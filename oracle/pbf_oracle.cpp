@@ -71,6 +71,26 @@ inline void decode_key(uint32_t key, const oi3& dim, int& x, int& y, int& z) {
     z = (int)(kdx / (uint32_t)dim.y);
 }
 
+// s_corr power (reference: pbf_delta_apply.cu:81 calls powf).  An integral exponent
+// in [1, 16] is evaluated as a product in double by binary exponentiation: for
+// n = 4, x*x is exact in double and x^4 is rounded once in double and once to
+// float, i.e. the correctly rounded power except in double-rounding ties.  Other
+// exponents call powf.  Pinned against float64 powers and powf in the tests.
+inline float pow_scorr(float x, float n) {
+    if (n >= 1.0f && n <= 16.0f && (float)(int)n == n) {
+        double b = (double)x, r = 1.0;
+        int e = (int)n;
+        while (true) {
+            if (e & 1) r *= b;
+            e >>= 1;
+            if (!e) break;
+            b *= b;
+        }
+        return (float)r;
+    }
+    return std::pow(x, n);
+}
+
 // Cell range providers: dense start/end arrays or compact unique-key search.
 struct DenseRanges {
     const uint32_t* start;
@@ -191,7 +211,7 @@ of4 delta_one(uint32_t s, const of4* P, const float* lambda, const uint32_t* idx
             const float hr2 = kc.h2 - r2;
             const float w = kc.poly6 * hr2 * hr2 * hr2;
             const float ratio = (w_q > 0.f) ? (w / w_q) : 0.0f;
-            scorr = -pbf.scorr_k * std::pow(ratio, pbf.scorr_n);
+            scorr = -pbf.scorr_k * pow_scorr(ratio, pbf.scorr_n);
             if (pbf.scorr_min < 0.f) scorr = std::fmax(scorr, pbf.scorr_min);
         }
         const float lij = (lambda[i] + lambda[j] + scorr);
@@ -633,6 +653,8 @@ void oracle_seed_lattice(of4* X, uint32_t nx, uint32_t ny, uint32_t nz, float ox
         }
     }
 }
+
+float oracle_pow_scorr(float x, float n) { return pow_scorr(x, n); }
 
 int oracle_num_threads(void) {
 #ifdef _OPENMP

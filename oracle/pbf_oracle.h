@@ -115,6 +115,9 @@ void oracle_seed_lattice(of4* X, uint32_t nx, uint32_t ny, uint32_t nz, float ox
 /* The uniform variate used by the seeder: 24-bit mantissa float in [0,1). */
 float oracle_jitter_uniform(uint64_t seed, uint64_t particle, uint32_t axis);
 
+/* The s_corr power (W/W(dq))^n as the solver evaluates it. */
+float oracle_pow_scorr(float x, float n);
+
 int oracle_num_threads(void);
 
 #ifdef __cplusplus

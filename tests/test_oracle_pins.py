@@ -384,3 +384,20 @@ def test_seed_lattice_generator(oracle):
             j = f(f(f(2.0) * u(77, i, a) - f(1.0)) * f(0.0005))
             assert X[i, a] == f(min(max(f(base[a] + j), f(0.0)), f(1.0)))
         assert X[i, 3] == 1.0
+
+
+def test_scorr_power(oracle):
+    """(W/W(dq))^n: integral n is the correctly rounded power (float64 product, rounded once to
+    float32 for n <= 4) and within 1 ulp of powf; other n go through powf."""
+    rng = np.random.default_rng(9)
+    xs = np.concatenate([rng.uniform(0, 1.5, 2000), [0.0, 1.0, 1e-20, 0.5, 2.0]]).astype(np.float32)
+    for n in (1.0, 2.0, 3.0, 4.0):
+        for x in xs[:500]:
+            got = np.float32(oracle.pow_scorr(float(x), n))
+            exact = np.float32(float(x) ** int(n))  # float64 power of a float32 is exact for n <= 2;
+            # for n = 3, 4 it rounds once in float64 before the float32 rounding
+            assert got == exact or abs(int(got.view(np.int32)) - int(exact.view(np.int32))) <= 1
+            ref = np.power(np.float32(x), np.float32(n), dtype=np.float32)
+            assert abs(int(got.view(np.int32)) - int(ref.view(np.int32))) <= 1
+    assert oracle.pow_scorr(0.5, 4.0) == 0.0625 and oracle.pow_scorr(0.0, 4.0) == 0.0
+    assert oracle.pow_scorr(0.25, 2.5) == pytest.approx(0.25 ** 2.5, rel=1e-6)
