@@ -161,80 +161,123 @@ struct HashRowsDev {
 };
 
 // ------------------------------------------------------------------------
-// Deferred neighbour walk.  A lane scans its candidates (one 16-byte load and a
-// distance test each) and appends the in-range slots to a small FIFO in LDS.
-// The expensive per-pair work (sqrt, IEEE divisions, the s_corr power) runs only
-// when some lane's FIFO is full or every lane has finished scanning, and then for
-// one queued pair on every lane that has one.  Without this, a wave executes the
-// per-pair path whenever any of its 64 lanes finds a neighbour (~15% of
-// candidates each), i.e. almost every iteration at low utilisation.  Each lane
-// still consumes its own pairs in scan order, so sums keep the reference order.
-constexpr int kQD = 8;  // FIFO depth per lane
+// Neighbour walk of one wave (64 consecutive sorted slots, one particle per lane).
+//
+// Candidate stream.  For each neighbour row the wave takes the union of its
+// lanes' slot spans; when that union is not much longer than the longest lane
+// span (dense cells, many particles per cell), the wave streams it: 64
+// candidates at a time are loaded coalesced into a per-wave LDS buffer, then
+// every lane reads each candidate as an LDS broadcast and tests it against its
+// own span, itself and the distance h.  The trip count is wave-uniform, so the
+// scan does not diverge.  Rows whose union is much longer than the lane spans
+// (sparse cells) are walked per lane instead, reading candidates from global
+// memory.
+//
+// Deferred pairs.  In-range slots go to a per-lane FIFO in LDS; the expensive
+// per-pair work (sqrt, IEEE divisions, the s_corr power) runs when some lane's
+// FIFO is full, for one queued pair on every lane that has one, and at the end.
+// Otherwise a wave would execute the per-pair path whenever any of its 64 lanes
+// finds a neighbour, at low utilisation.
+//
+// Order.  Each lane still sees its candidates in reference order (rows dz-major
+// then dy, slots ascending within a row) and consumes its FIFO in order, so
+// every sum is formed in the reference's order.
+constexpr int kQD = 8;           // FIFO depth per lane
+constexpr uint32_t kStreamRatio = 3;  // stream a row when union <= ratio * longest lane span
 
-struct Walk {
-    uint32_t k = 0, hi = 0;
-    int r = -1;
-    bool scanning = true;
-    int qh = 0, qn = 0;
-};
-
-// Advance to the next candidate slot of lane's rows; false when the rows are exhausted.
-__device__ __forceinline__ bool next_candidate(Walk& w, uint32_t m, const SpanTable& st, uint32_t& cand) {
-    while (w.k >= w.hi) {
-        if (w.r + 1 >= st.rows) return false;
-        ++w.r;
-        const uint2 sp = st.spans[(uint64_t)w.r * st.stride + m];
-        w.k = sp.x;
-        w.hi = sp.y;
-    }
-    cand = w.k++;
-    return true;
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
+    return v;
 }
 
 template <class InRange, class Apply>
-__device__ __forceinline__ void deferred_walk(uint32_t s, uint32_t m, const SpanTable& st, uint32_t* q, int cap,
+__device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, uint32_t m, const SpanTable& st,
+                                              const float4* P, float4* buf, uint32_t* q, int cap,
                                               InRange&& in_range, Apply&& apply) {
-    Walk w;
-    int found = 0;
-    while (true) {
-        if (w.scanning && w.qn < kQD) {
-            uint32_t cand;
-            if (cap > 0 && found >= cap) {
-                w.scanning = false;
-            } else if (!next_candidate(w, m, st, cand)) {
-                w.scanning = false;
-            } else if (cand != s && in_range(cand)) {
-                q[((w.qh + w.qn) & (kQD - 1)) * kBS] = cand;
-                ++w.qn;
-                ++found;
-            }
+    const uint32_t lane = threadIdx.x & 63;
+    int qh = 0, qn = 0, found = 0;
+    auto push = [&](uint32_t k) {
+        q[((qh + qn) & (kQD - 1)) * kBS] = k;
+        ++qn;
+        ++found;
+    };
+    auto pop_one = [&]() {
+        if (qn > 0) {
+            const uint32_t k = q[qh * kBS];
+            qh = (qh + 1) & (kQD - 1);
+            --qn;
+            apply(k);
         }
-        const bool anyScanning = __any(w.scanning);
-        if (__any(w.qn == kQD) || !anyScanning) {
-            if (w.qn > 0) {
-                const uint32_t kk = q[w.qh * kBS];
-                w.qh = (w.qh + 1) & (kQD - 1);
-                --w.qn;
-                apply(kk);
+    };
+    for (int r = 0; r < st.rows; ++r) {
+        uint32_t lo = 0, hi = 0;
+        if (valid) {
+            const uint2 sp = st.spans[(uint64_t)r * st.stride + m];
+            lo = sp.x;
+            hi = sp.y;
+        }
+        if (cap > 0 && found >= cap) hi = lo;  // neighbour cap reached: nothing more to take
+        const bool has = lo < hi;
+        const uint32_t ulo = wave_min_u32(has ? lo : 0xFFFFFFFFu);
+        const uint32_t uhi = wave_max_u32(has ? hi : 0u);
+        if (uhi <= ulo) continue;
+        const uint32_t lmax = wave_max_u32(hi - lo);
+        if (uhi - ulo > kStreamRatio * lmax) {
+            // sparse row: per-lane walk over the lane's own span
+            uint32_t k = lo;
+            while (__any(k < hi)) {
+                if (__any(qn == kQD)) pop_one();
+                if (k < hi && qn < kQD) {
+                    const uint32_t cand = k++;
+                    if (cand != s && in_range(P[cand])) {
+                        push(cand);
+                        if (cap > 0 && found >= cap) k = hi;
+                    }
+                }
             }
-            if (!anyScanning && !__any(w.qn > 0)) break;
+        } else {
+            // dense row: stream the union through LDS
+            for (uint32_t base = ulo; base < uhi; base += 64) {
+                const uint32_t c = base + lane;
+                if (c < uhi) buf[lane] = P[c];
+                __builtin_amdgcn_wave_barrier();
+                const uint32_t n = min(64u, uhi - base);
+                for (uint32_t t = 0; t < n; ++t) {
+                    if (__any(qn == kQD)) pop_one();
+                    const uint32_t k = base + t;
+                    const float4 pj = buf[t];
+                    if (k >= lo && k < hi && k != s && in_range(pj)) {
+                        push(k);
+                        if (cap > 0 && found >= cap) hi = lo;
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
         }
     }
+    while (__any(qn > 0)) pop_one();
 }
 
 __global__ __launch_bounds__(kBS) void k_lambda_sorted(float4* Ps, const uint32_t* __restrict__ segOf, SpanTable st,
                                                        sim::DeviceParams dp, uint32_t N) {
     __shared__ uint32_t sQ[kQD * kBS];
+    __shared__ float4 sBuf[kBS];
     const uint32_t s = slot_of_thread();
-    if (s >= N) return;
-    const float4 pi = Ps[s];
-    const uint32_t m = segOf[s];
+    const bool valid = s < N;
+    const uint32_t ss = valid ? s : N - 1;
+    const float4 pi = Ps[ss];
+    const uint32_t m = segOf[ss];
     const PairConsts c = make_pair_consts(dp);
     LambdaAcc a;
-    deferred_walk(
-        s, m, st, sQ + threadIdx.x, dp.maxNeighbors,
-        [&](uint32_t k) {
-            const float4 pj = Ps[k];
+    neighbor_walk(
+        valid, s, m, st, Ps, sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x, dp.maxNeighbors,
+        [&](const float4& pj) {
             const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
             return rx * rx + ry * ry + rz * rz <= c.h2;
         },
@@ -243,7 +286,7 @@ __global__ __launch_bounds__(kBS) void k_lambda_sorted(float4* Ps, const uint32_
             const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
             lambda_pair(a, rx, ry, rz, rx * rx + ry * ry + rz * rz, c);
         });
-    reinterpret_cast<float*>(Ps)[4ull * s + 3] = lambda_finish(a, dp);
+    if (valid) reinterpret_cast<float*>(Ps)[4ull * s + 3] = lambda_finish(a, dp);
 }
 
 __global__ __launch_bounds__(kBS) void k_delta_sorted(const float4* __restrict__ Pa, float4* __restrict__ Pb,
@@ -253,16 +296,17 @@ __global__ __launch_bounds__(kBS) void k_delta_sorted(const float4* __restrict__
                                                       const uint32_t* __restrict__ idxS, float4* __restrict__ Vs,
                                                       float invDt) {
     __shared__ uint32_t sQ[kQD * kBS];
+    __shared__ float4 sBuf[kBS];
     const uint32_t s = slot_of_thread();
-    if (s >= N) return;
-    const float4 pi = Pa[s];
-    const uint32_t m = segOf[s];
+    const bool valid = s < N;
+    const uint32_t ss = valid ? s : N - 1;
+    const float4 pi = Pa[ss];
+    const uint32_t m = segOf[ss];
     const PairConsts c = make_pair_consts(dp);
     float dx = 0.f, dy = 0.f, dz = 0.f;
-    deferred_walk(
-        s, m, st, sQ + threadIdx.x, 0,
-        [&](uint32_t k) {
-            const float4 pj = Pa[k];
+    neighbor_walk(
+        valid, s, m, st, Pa, sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x, 0,
+        [&](const float4& pj) {
             const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
             return rx * rx + ry * ry + rz * rz <= c.h2;
         },
@@ -271,6 +315,7 @@ __global__ __launch_bounds__(kBS) void k_delta_sorted(const float4* __restrict__
             const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
             delta_pair(dx, dy, dz, rx, ry, rz, rx * rx + ry * ry + rz * rz, pi.w, pj.w, c);
         });
+    if (!valid) return;
     const float3 d = delta_finish(dx, dy, dz, dp);
     // reference: pbf_delta_apply.cu:232-233 (P += delta)
     float px = pi.x + d.x, py = pi.y + d.y, pz = pi.z + d.z;
@@ -305,20 +350,21 @@ __global__ __launch_bounds__(kBS) void k_xsph_scatter(const float4* __restrict__
                                                       const uint32_t* __restrict__ idxS, float4* __restrict__ Xn,
                                                       float4* __restrict__ V) {
     __shared__ uint32_t sQ[kQD * kBS];
+    __shared__ float4 sBuf[kBS];
     const uint32_t s = slot_of_thread();
-    if (s >= N) return;
-    const float4 pi = Ps[s];
-    const float4 vi = Vs[s];
+    const bool valid = s < N;
+    const uint32_t ss = valid ? s : N - 1;
+    const float4 pi = Ps[ss];
+    const float4 vi = Vs[ss];
     float vx = vi.x, vy = vi.y, vz = vi.z;
     if (dp.xsph_c > 0.f) {
-        const uint32_t m = segOf[s];
+        const uint32_t m = segOf[ss];
         const float h2 = dp.kernel.h2, poly6 = dp.kernel.poly6;
         float sx = 0.f, sy = 0.f, sz = 0.f;
         int n = 0;
-        deferred_walk(
-            s, m, st, sQ + threadIdx.x, 0,
-            [&](uint32_t k) {
-                const float4 pj = Ps[k];
+        neighbor_walk(
+            valid, s, m, st, Ps, sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x, 0,
+            [&](const float4& pj) {
                 const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
                 return rx * rx + ry * ry + rz * rz <= h2;
             },
@@ -336,6 +382,7 @@ __global__ __launch_bounds__(kBS) void k_xsph_scatter(const float4* __restrict__
         const float3 v = xsph_finish(vi.x, vi.y, vi.z, sx, sy, sz, n, dp);
         vx = v.x; vy = v.y; vz = v.z;
     }
+    if (!valid) return;
     // BoundaryOp with restitution (post_ops.cpp:25-29, boundary.cu:11-25)
     float px = pi.x, py = pi.y, pz = pi.z;
     clamp_box(px, py, pz, vx, vy, vz, dp.grid, clamp_restitution(dp.boundaryRestitution));
@@ -365,6 +412,7 @@ void launch_spans_gather(CellLookup lookup, const float4* Xn, const uint32_t* id
                          const uint32_t* uniq, const uint32_t* off, const uint32_t* count, const uint32_t* cellStart,
                          const uint32_t* cellEnd, const HashTable& table, const SpanTable& st,
                          const sim::GridBounds& grid, hipStream_t s) {
+    if (N == 0) return;
     const uint32_t b = blocks_for(N);
     switch (lookup) {
         case CellLookup::Dense:
@@ -384,6 +432,7 @@ void launch_spans_gather(CellLookup lookup, const float4* Xn, const uint32_t* id
 
 void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st, const sim::DeviceParams& dp,
                           uint32_t N, hipStream_t s) {
+    if (N == 0) return;
     k_lambda_sorted<<<blocks_for(N), kBS, 0, s>>>(Ps, segOf, st, dp, N);
     check_launch("lambda_sorted");
 }
@@ -391,12 +440,14 @@ void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st
 void launch_delta_sorted(const float4* Pa, float4* Pb, const uint32_t* segOf, const SpanTable& st,
                          const sim::DeviceParams& dp, uint32_t N, bool last, const float4* X, const uint32_t* idxS,
                          float4* Vs, float invDt, hipStream_t s) {
+    if (N == 0) return;
     k_delta_sorted<<<blocks_for(N), kBS, 0, s>>>(Pa, Pb, segOf, st, dp, N, last ? 1 : 0, X, idxS, Vs, invDt);
     check_launch("delta_sorted");
 }
 
 void launch_finalize_sorted(float4* Ps, const float4* X, const uint32_t* idxS, float4* Vs,
                             const sim::GridBounds& grid, float invDt, uint32_t N, hipStream_t s) {
+    if (N == 0) return;
     k_finalize_sorted<<<blocks_for(N), kBS, 0, s>>>(Ps, X, idxS, Vs, grid, invDt, N);
     check_launch("finalize_sorted");
 }
@@ -404,6 +455,7 @@ void launch_finalize_sorted(float4* Ps, const float4* X, const uint32_t* idxS, f
 void launch_xsph_scatter(const float4* Ps, const float4* Vs, const uint32_t* segOf, const SpanTable& st,
                          const sim::DeviceParams& dp, uint32_t N, const uint32_t* idxS, float4* Xn, float4* V,
                          hipStream_t s) {
+    if (N == 0) return;
     k_xsph_scatter<<<blocks_for(N), kBS, 0, s>>>(Ps, Vs, segOf, st, dp, N, idxS, Xn, V);
     check_launch("xsph_scatter");
 }
