@@ -95,6 +95,11 @@ extern "C" int pbfSimProfilePhases(PbfSim* sim, float* ms, int maxPhases, char* 
     return (int)out.size();
 }
 
+extern "C" int pbfSimDensityStats(PbfSim* sim, double* out5) {
+    if (!sim || !out5) return -1;
+    return sim->s.densityStats(out5) ? 0 : -1;
+}
+
 extern "C" int pbfSimHashOverflow(PbfSim* sim) { return (sim && sim->s.hashOverflowed()) ? 1 : 0; }
 
 extern "C" void pbfDebugEchoDeviceParams(sim::DeviceParams dp, sim::DeviceParams* out) {

@@ -9,6 +9,7 @@
 #include "pbf_hip.h"
 #include "radix_sort.hpp"
 #include "sim_host.h"
+#include "stats.hpp"
 
 namespace sim {
 
@@ -266,13 +267,15 @@ bool GridBuffers::allocate(uint32_t cap, uint32_t nCells, GridKind k, uint32_t t
     ok = ok && (hipMalloc(&d_sortTemp, sortTempBytes) == hipSuccess);
     if (k == GridKind::Dense) ok = ok && dalloc(d_cellStart, nCells) && dalloc(d_cellEnd, nCells);
     ok = ok && dalloc(d_cellUniqueKeys, cap) && dalloc(d_cellOffsets, (size_t)cap + 1) && dalloc(d_compactCount, 1);
+    // segment index, row spans and compaction scratch: used by the reordered step and,
+    // on both paths, by the density statistics
+    const uint32_t tiles = pbfx::compact_scratch_elems(cap);
+    spans.reach = reach;
+    spans.rows = (2 * reach + 1) * (2 * reach + 1);
+    spans.stride = cap;
+    ok = ok && dalloc(d_segOf, cap) && dalloc(d_blockCounts, tiles) && dalloc(d_blockOffsets, tiles) &&
+         dalloc(spans.spans, (size_t)spans.rows * cap);
     if (reordered) {
-        const uint32_t tiles = pbfx::compact_scratch_elems(cap);
-        spans.reach = reach;
-        spans.rows = (2 * reach + 1) * (2 * reach + 1);
-        spans.stride = cap;
-        ok = ok && dalloc(d_segOf, cap) && dalloc(d_blockCounts, tiles) && dalloc(d_blockOffsets, tiles) &&
-             dalloc(spans.spans, (size_t)spans.rows * cap);
         if (k == GridKind::Hashed) {
             tableLog2 = tLog2;
             const size_t slots = size_t(1) << tLog2;
@@ -566,6 +569,23 @@ bool Simulator::getState(float4* pos, float4* vel, bool toDevice) const {
     if (pos) ok = ok && hipMemcpy(pos, m_bufs.d_pos, bytes, k) == hipSuccess;
     if (vel) ok = ok && hipMemcpy(vel, m_bufs.d_vel, bytes, k) == hipSuccess;
     return ok;
+}
+
+bool Simulator::densityStats(double* out5) {
+    if (!m_initialized) return false;
+    const uint32_t N = m_params.numParticles;
+    const hipStream_t s = m_stream;
+    GridBuffers& g = m_grid;
+    LaunchHashKeys(g.d_cellKeys, g.d_indices, m_bufs.d_pos, m_params.grid, N, s);
+    pbfx::radix_sort_pairs(g.d_sortTemp, g.sortTempBytes, g.d_cellKeys, g.d_cellKeys_sorted, nullptr,
+                           g.d_indices_sorted, N, 0, pbfx::key_bits_for(g.numCells), s);
+    pbfx::compact_segments(g.d_cellUniqueKeys, g.d_cellOffsets, g.d_compactCount, g.d_segOf, g.d_cellKeys_sorted, N,
+                           g.d_blockCounts, g.d_blockOffsets, s);
+    pbfx::launch_spans_gather(pbfx::CellLookup::Compact, m_bufs.d_pos, g.d_indices_sorted, m_bufs.d_pos_sorted[0], N,
+                              g.d_cellUniqueKeys, g.d_cellOffsets, g.d_compactCount, nullptr, nullptr, g.table,
+                              g.spans, m_params.grid, s);
+    return pbfx::density_stats_sorted(m_bufs.d_pos_sorted[0], m_bufs.d_vel, g.d_indices_sorted, g.d_segOf, g.spans,
+                                      MakeDeviceParams(m_params), N, out5, s);
 }
 
 bool Simulator::hashOverflowed() const {

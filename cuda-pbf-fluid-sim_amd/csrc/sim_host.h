@@ -180,6 +180,8 @@ public:
     const GridBuffers& grid() const { return m_grid; }
     uint64_t frameIndex() const { return m_frameIndex; }
     bool hashOverflowed() const;
+    // Density-constraint statistics of the current state (see pbfSimDensityStats).
+    bool densityStats(double* out5);
 
 private:
     bool captureGraphs();

@@ -6,7 +6,7 @@ entry point raises.
 import ctypes
 import os
 
-from .params import DeviceParams, Float3, GridBounds, SimParams
+from .params import DeviceParams, Float3, GridBounds, KernelCoeffs, SimParams
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpbfx.so")
@@ -34,6 +34,10 @@ SIGNATURES = {
     "LaunchDeltaApplyCompact": (None, [_P, _P, _P, _P, _P, _P, _P, _P, DeviceParams, _U32, _P]),
     "LaunchXSPH": (None, [_P, _P, _P, _P, _P, _P, _P, DeviceParams, _U32, _P]),
     "LaunchXSPHCompact": (None, [_P, _P, _P, _P, _P, _P, _P, _P, DeviceParams, _U32, _P]),
+    "LaunchComputeStats": (ctypes.c_bool, [_P, _P, _P, _P, _P, GridBounds, KernelCoeffs, _F, _U32, _U32, _U32,
+                                            _P, _P, _P, _P, _P]),
+    "LaunchComputeStatsBruteforce": (ctypes.c_bool, [_P, _P, KernelCoeffs, _F, _U32, _U32, _U32, _P, _P, _P, _P,
+                                                      _P]),
     "pbfGetLastError": (_I, []),
     "pbfGetLastErrorString": (ctypes.c_char_p, []),
     "pbfSortPairsBits": (None, [_P, ctypes.POINTER(ctypes.c_size_t), _P, _P, _P, _P, _U32, _I, _I, _P]),
@@ -52,6 +56,7 @@ SIGNATURES = {
     "pbfSimDeviceSortedKeys": (_P, [_P]),
     "pbfSimDeviceSortedIndices": (_P, [_P]),
     "pbfSimProfilePhases": (_I, [_P, _P, _I, _P, _I]),
+    "pbfSimDensityStats": (_I, [_P, _P]),
     "pbfSimHashOverflow": (_I, [_P]),
     "pbfDebugEchoDeviceParams": (None, [DeviceParams, ctypes.POINTER(DeviceParams)]),
     "pbfDebugEchoGridBounds": (None, [GridBounds, ctypes.POINTER(GridBounds)]),

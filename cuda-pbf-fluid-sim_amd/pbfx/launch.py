@@ -120,3 +120,25 @@ def xsph_compact(vel_out, vel_in, pos_pred, idx_s, key_s, uniq, offsets, count, 
     _lib.lib().LaunchXSPHCompact(_p(vel_out), _p(vel_in), _p(pos_pred), _p(idx_s), _p(key_s), _p(uniq), _p(offsets),
                                  _p(count), dp, _n(pos_pred), _stream())
     _lib.check("LaunchXSPHCompact")
+
+
+def compute_stats(pos, vel, idx_s, start, end, grid, kc, mass, sample_stride=1):
+    """LaunchComputeStats: (avg neighbours, avg speed, avg sum W, avg rho); synchronises."""
+    out = [ctypes.c_double() for _ in range(4)]
+    ok = _lib.lib().LaunchComputeStats(_p(pos), _p(vel), _p(idx_s), _p(start), _p(end), grid, kc, mass, _n(pos),
+                                       int(start.shape[0]), sample_stride, *[ctypes.byref(o) for o in out],
+                                       _stream())
+    _lib.check("LaunchComputeStats")
+    assert ok
+    return tuple(o.value for o in out)
+
+
+def compute_stats_bruteforce(pos, vel, kc, mass, sample_stride=1, max_samples=None):
+    out = [ctypes.c_double() for _ in range(4)]
+    n = _n(pos)
+    ok = _lib.lib().LaunchComputeStatsBruteforce(_p(pos), _p(vel), kc, mass, n, sample_stride,
+                                                 n if max_samples is None else max_samples,
+                                                 *[ctypes.byref(o) for o in out], _stream())
+    _lib.check("LaunchComputeStatsBruteforce")
+    assert ok
+    return tuple(o.value for o in out)

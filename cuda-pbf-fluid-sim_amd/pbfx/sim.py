@@ -95,5 +95,14 @@ class Simulator:
         labels = names.raw.split(b"\0")[:n]
         return [(l.decode(), float(ms[i])) for i, l in enumerate(labels)]
 
+    def density_stats(self):
+        """Density-constraint statistics of the current state, computed on the GPU."""
+        out = np.zeros(5, np.float64)
+        if _lib.lib().pbfSimDensityStats(self._h, out.ctypes.data) != 0:
+            _lib.check("pbfSimDensityStats")
+            raise _lib.PbfError("density_stats failed")
+        return {"mean_err": out[0], "max_err": out[1], "mean_neighbors": out[2], "mean_speed": out[3],
+                "mean_pos_err": out[4]}
+
     def hash_overflow(self):
         return bool(_lib.lib().pbfSimHashOverflow(self._h))

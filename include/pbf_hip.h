@@ -90,6 +90,21 @@ void LaunchXSPHCompact(float4* vel_out, const float4* vel_in, const float4* pos_
                        const uint32_t* keysSorted, const uint32_t* uniqueKeys, const uint32_t* offsets,
                        const uint32_t* compactCount, sim::DeviceParams dp, uint32_t N, hipStream_t s);
 
+// ---- statistics (diagnostics: read back to the host and synchronise `s`) -------
+// reference: sim/stats.h:17-32 (StatsKernel, stats.cu:25-98).  Means over the
+// sampled particles (i % sampleStride == 0) of in-range neighbours (r2 < h2),
+// |v|, sum W and m * sum W.  Cross-particle sums are deterministic (double).
+bool LaunchComputeStats(const float4* pos_pred, const float4* vel, const uint32_t* indices_sorted,
+                        const uint32_t* cellStart, const uint32_t* cellEnd, sim::GridBounds grid,
+                        sim::KernelCoeffs kc, float particleMass, uint32_t N, uint32_t numCells,
+                        uint32_t sampleStride, double* outAvgNeighbors, double* outAvgSpeed, double* outAvgRhoRel,
+                        double* outAvgRho, hipStream_t s);
+// reference: sim/stats.h:34-45 (StatsBruteKernel, stats_bruteforce.cu:8-62), O(N^2).
+bool LaunchComputeStatsBruteforce(const float4* pos_pred, const float4* vel, sim::KernelCoeffs kc,
+                                  float particleMass, uint32_t N, uint32_t sampleStride, uint32_t maxISamples,
+                                  double* outAvgNeighbors, double* outAvgSpeed, double* outAvgRhoRel,
+                                  double* outAvgRho, hipStream_t s);
+
 // ---- extensions (no reference counterpart) ----------------------------------
 // Last recorded error (hipError_t value, or PBFX_E_* below), then cleared.
 #define PBFX_E_INVALID_ARGUMENT 1001

@@ -55,6 +55,11 @@ const uint32_t* pbfSimDeviceSortedIndices(PbfSim* sim);
 // One eager step with an event around every phase.  Writes up to maxPhases
 // times (ms) and NUL-separated names into names (namesBytes long); returns the phase count.
 int pbfSimProfilePhases(PbfSim* sim, float* ms, int maxPhases, char* names, int namesBytes);
+// Density-constraint statistics of the current state (grid rebuilt on the current
+// positions, lambda-pass neighbour order and cap): out5 = {mean(rho/rho0-1),
+// max(rho/rho0-1), mean neighbours, mean |v|, mean(max(rho/rho0-1, 0))}.
+// Synchronises the simulator's stream.
+int pbfSimDensityStats(PbfSim* sim, double* out5);
 // 1 when a hash-table insert found no free slot since creation.
 int pbfSimHashOverflow(PbfSim* sim);
 
