@@ -1,6 +1,7 @@
 // capi.cpp — C handle API over sim::Simulator.
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "pbf_common.hpp"
 #include "pbf_hip.h"
@@ -114,4 +115,61 @@ extern "C" void pbfDebugEchoFloat3(float3 v, float* out3) {
         out3[1] = v.y;
         out3[2] = v.z;
     }
+}
+
+static_assert(sizeof(PbfSceneDesc) == 48, "PbfSceneDesc layout");
+
+extern "C" PbfSim* pbfEnsembleCreate(const sim::SimParams* base, const PbfSceneDesc* scenes, uint32_t count,
+                                     int gridKind, int tableLog2, int useGraph) {
+    if (!base || !scenes || count == 0 || gridKind < 0 || gridKind > 2) {
+        pbfx::set_error(PBFX_E_INVALID_ARGUMENT, "pbfEnsembleCreate: bad arguments");
+        return nullptr;
+    }
+    std::vector<sim::SceneDesc> d(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        const PbfSceneDesc& c = scenes[i];
+        d[i].boxMin = make_float3(c.boxMin[0], c.boxMin[1], c.boxMin[2]);
+        d[i].boxMax = make_float3(c.boxMax[0], c.boxMax[1], c.boxMax[2]);
+        d[i].gravity = make_float3(c.gravity[0], c.gravity[1], c.gravity[2]);
+        d[i].xsph_c = c.xsph_c;
+        d[i].restitution = c.restitution;
+        d[i].numParticles = c.numParticles;
+    }
+    PbfSim* h = new (std::nothrow) PbfSim;
+    if (!h) return nullptr;
+    sim::SimOptions opt;
+    opt.grid = (sim::GridKind)gridKind;
+    opt.path = sim::PathKind::Reordered;
+    opt.tableLog2 = tableLog2 > 0 ? (uint32_t)tableLog2 : 22u;
+    opt.useGraph = useGraph != 0;
+    if (!h->s.initializeEnsemble(*base, d, opt)) {
+        delete h;
+        return nullptr;
+    }
+    return h;
+}
+
+extern "C" int pbfEnsembleSeedLattice(PbfSim* sim, uint32_t scene, uint32_t nx, uint32_t ny, uint32_t nz, float ox,
+                                      float oy, float oz, float spacing, float jitterAmp, uint64_t seed) {
+    if (!sim) return -1;
+    return sim->s.seedSceneLattice(scene, nx, ny, nz, make_float3(ox, oy, oz), spacing, jitterAmp, seed) ? 0 : -1;
+}
+
+extern "C" int pbfEnsembleGetScene(PbfSim* sim, uint32_t scene, float4* pos, float4* vel) {
+    if (!sim) return -1;
+    return sim->s.getSceneState(scene, pos, vel) ? 0 : -1;
+}
+
+extern "C" int pbfEnsembleSceneInfo(PbfSim* sim, uint32_t scene, uint32_t* first, uint32_t* count, int* dims3) {
+    if (!sim) return -1;
+    const pbfx::SceneParams* sp = sim->s.sceneParams(scene);
+    if (!sp) return -1;
+    if (first) *first = sp->particleBase;
+    if (count) *count = sp->numParticles;
+    if (dims3) {
+        dims3[0] = sp->grid.dim.x;
+        dims3[1] = sp->grid.dim.y;
+        dims3[2] = sp->grid.dim.z;
+    }
+    return 0;
 }

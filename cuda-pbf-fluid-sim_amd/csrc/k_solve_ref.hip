@@ -141,7 +141,7 @@ __global__ __launch_bounds__(kBS) void k_xsph_ref(float4* __restrict__ Vout, con
         }
         return true;
     });
-    const float3 v = xsph_finish(vi.x, vi.y, vi.z, sx, sy, sz, n, dp);
+    const float3 v = xsph_finish(vi.x, vi.y, vi.z, sx, sy, sz, n, dp, dp.xsph_c);
     Vout[i] = make_float4(v.x, v.y, v.z, 0.0f);
 }
 

@@ -30,10 +30,17 @@ __device__ __forceinline__ uint32_t slot_of_thread() {
 __global__ __launch_bounds__(kBS) void k_predict_hash(const float4* __restrict__ X, const float4* __restrict__ V,
                                                       float4* __restrict__ Xn, uint32_t* __restrict__ keys,
                                                       uint32_t* __restrict__ epoch, float3 g, float dt,
-                                                      sim::GridBounds grid, uint32_t N) {
+                                                      sim::GridBounds grid, uint32_t N, Scenes sc) {
     const uint32_t i = blockIdx.x * kBS + threadIdx.x;
     if (epoch && i == 0) epoch[0] = epoch[0] + 1u;
     if (i >= N) return;
+    uint32_t keyBase = 0;
+    if (sc.params) {
+        const SceneParams& sp = sc.params[sc.sceneOf[i]];
+        grid = sp.grid;
+        g = sp.gravity;
+        keyBase = sp.cellBase;
+    }
     const float4 x = X[i];
     const float4 v = V[i];
     // reference: integrate.cu:43-51, then boundary.cu:11-25 with e = 0 (the
@@ -45,7 +52,7 @@ __global__ __launch_bounds__(kBS) void k_predict_hash(const float4* __restrict__
     float ux = 0.f, uy = 0.f, uz = 0.f;
     clamp_box(px, py, pz, ux, uy, uz, grid, 0.0f);
     Xn[i] = make_float4(px, py, pz, 1.0f);
-    keys[i] = cell_key(px, py, pz, grid);
+    keys[i] = keyBase + cell_key(px, py, pz, grid);
 }
 
 __device__ __forceinline__ uint32_t table_home(uint32_t key, uint32_t mask) { return key & mask; }
@@ -103,9 +110,26 @@ struct HashRows {
     }
 };
 
+// Scene of a global cell key: the last scene whose key base is <= key.
+__device__ __forceinline__ uint32_t scene_of_key(const Scenes& sc, uint32_t key) {
+    uint32_t lo = 0, hi = sc.count;  // invariant: cellBase[lo] <= key < cellBase[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (sc.cellBase[mid] <= key) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
 template <class Rows>
 __device__ __forceinline__ void cell_spans(uint32_t key, uint32_t m, const Rows& rows, const SpanTable& st,
-                                           const sim::GridBounds& g) {
+                                           sim::GridBounds g, const Scenes& sc) {
+    uint32_t base0 = 0;
+    if (sc.params) {
+        const SceneParams& sp = sc.params[scene_of_key(sc, key)];
+        g = sp.grid;
+        base0 = sp.cellBase;
+        key -= base0;
+    }
     int cx, cy, cz;
     decode_key(key, g.dim, cx, cy, cz);
     const int R = st.reach;
@@ -117,7 +141,7 @@ __device__ __forceinline__ void cell_spans(uint32_t key, uint32_t m, const Rows&
             const int y = cy + dy;
             uint32_t lo = 0, hi = 0;
             if (z >= 0 && z < g.dim.z && y >= 0 && y < g.dim.y && x0 <= x1) {
-                const uint32_t base = (uint32_t)((z * g.dim.y + y) * g.dim.x);
+                const uint32_t base = base0 + (uint32_t)((z * g.dim.y + y) * g.dim.x);
                 rows.span(base + (uint32_t)x0, base + (uint32_t)x1, lo, hi);
             }
             st.spans[(uint64_t)r * st.stride + m] = make_uint2(lo, hi);
@@ -130,14 +154,16 @@ __global__ __launch_bounds__(kBS) void k_spans_gather(const float4* __restrict__
                                                       float4* __restrict__ Ps, uint32_t N,
                                                       const uint32_t* __restrict__ uniq,
                                                       const uint32_t* __restrict__ count, Rows rows, SpanTable st,
-                                                      sim::GridBounds g) {
+                                                      sim::GridBounds g, Scenes sc) {
     const uint32_t t = blockIdx.x * kBS + threadIdx.x;
     if (t < N) {
-        const float4 p = Xn[idxS[t]];
+        const uint32_t i = idxS[t];
+        const float4 p = Xn[i];
         Ps[t] = make_float4(p.x, p.y, p.z, 0.0f);
+        if (sc.sceneOfSlot) sc.sceneOfSlot[t] = sc.sceneOf[i];
     }
     const uint32_t M = *count;
-    if (t < M) cell_spans(uniq[t], t, rows, st, g);
+    if (t < M) cell_spans(uniq[t], t, rows, st, g, sc);
 }
 
 struct CompactRowsCount {
@@ -294,7 +320,7 @@ __global__ __launch_bounds__(kBS) void k_delta_sorted(const float4* __restrict__
                                                       sim::DeviceParams dp, uint32_t N, int last,
                                                       const float4* __restrict__ X,
                                                       const uint32_t* __restrict__ idxS, float4* __restrict__ Vs,
-                                                      float invDt) {
+                                                      float invDt, Scenes sc) {
     __shared__ uint32_t sQ[kQD * kBS];
     __shared__ float4 sBuf[kBS];
     const uint32_t s = slot_of_thread();
@@ -325,7 +351,7 @@ __global__ __launch_bounds__(kBS) void k_delta_sorted(const float4* __restrict__
     }
     // post-solve boundary (phases.cpp:94) and velocity (phases.cpp:102-105)
     float ux = 0.f, uy = 0.f, uz = 0.f;
-    clamp_box(px, py, pz, ux, uy, uz, dp.grid, 0.0f);
+    clamp_box(px, py, pz, ux, uy, uz, sc.params ? sc.params[sc.sceneOfSlot[s]].grid : dp.grid, 0.0f);
     Pb[s] = make_float4(px, py, pz, 1.0f);
     const float4 x = X[idxS[s]];
     Vs[s] = make_float4((px - x.x) * invDt, (py - x.y) * invDt, (pz - x.z) * invDt, 0.0f);
@@ -333,9 +359,10 @@ __global__ __launch_bounds__(kBS) void k_delta_sorted(const float4* __restrict__
 
 __global__ __launch_bounds__(kBS) void k_finalize_sorted(float4* __restrict__ Ps, const float4* __restrict__ X,
                                                          const uint32_t* __restrict__ idxS, float4* __restrict__ Vs,
-                                                         sim::GridBounds g, float invDt, uint32_t N) {
+                                                         sim::GridBounds g, float invDt, uint32_t N, Scenes sc) {
     const uint32_t s = blockIdx.x * kBS + threadIdx.x;
     if (s >= N) return;
+    if (sc.params) g = sc.params[sc.sceneOfSlot[s]].grid;
     const float4 p = Ps[s];
     float px = p.x, py = p.y, pz = p.z, ux = 0.f, uy = 0.f, uz = 0.f;
     clamp_box(px, py, pz, ux, uy, uz, g, 0.0f);
@@ -348,7 +375,7 @@ __global__ __launch_bounds__(kBS) void k_xsph_scatter(const float4* __restrict__
                                                       const uint32_t* __restrict__ segOf, SpanTable st,
                                                       sim::DeviceParams dp, uint32_t N,
                                                       const uint32_t* __restrict__ idxS, float4* __restrict__ Xn,
-                                                      float4* __restrict__ V) {
+                                                      float4* __restrict__ V, Scenes sc) {
     __shared__ uint32_t sQ[kQD * kBS];
     __shared__ float4 sBuf[kBS];
     const uint32_t s = slot_of_thread();
@@ -357,13 +384,22 @@ __global__ __launch_bounds__(kBS) void k_xsph_scatter(const float4* __restrict__
     const float4 pi = Ps[ss];
     const float4 vi = Vs[ss];
     float vx = vi.x, vy = vi.y, vz = vi.z;
-    if (dp.xsph_c > 0.f) {
+    float xc = dp.xsph_c, rest = dp.boundaryRestitution;
+    sim::GridBounds grid = dp.grid;
+    if (sc.params) {
+        const SceneParams& sp = sc.params[sc.sceneOfSlot[ss]];
+        xc = sp.xsph_c;
+        rest = sp.restitution;
+        grid = sp.grid;
+    }
+    // the walk is a wave-level operation: every lane enters it when any lane smooths
+    if (__any(xc > 0.f)) {
         const uint32_t m = segOf[ss];
         const float h2 = dp.kernel.h2, poly6 = dp.kernel.poly6;
         float sx = 0.f, sy = 0.f, sz = 0.f;
         int n = 0;
         neighbor_walk(
-            valid, s, m, st, Ps, sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x, 0,
+            valid && xc > 0.f, s, m, st, Ps, sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x, 0,
             [&](const float4& pj) {
                 const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
                 return rx * rx + ry * ry + rz * rz <= h2;
@@ -379,13 +415,15 @@ __global__ __launch_bounds__(kBS) void k_xsph_scatter(const float4* __restrict__
                 sz += (vj.z - vi.z) * w;
                 ++n;
             });
-        const float3 v = xsph_finish(vi.x, vi.y, vi.z, sx, sy, sz, n, dp);
-        vx = v.x; vy = v.y; vz = v.z;
+        if (xc > 0.f) {
+            const float3 v = xsph_finish(vi.x, vi.y, vi.z, sx, sy, sz, n, dp, xc);
+            vx = v.x; vy = v.y; vz = v.z;
+        }
     }
     if (!valid) return;
     // BoundaryOp with restitution (post_ops.cpp:25-29, boundary.cu:11-25)
     float px = pi.x, py = pi.y, pz = pi.z;
-    clamp_box(px, py, pz, vx, vy, vz, dp.grid, clamp_restitution(dp.boundaryRestitution));
+    clamp_box(px, py, pz, vx, vy, vz, grid, clamp_restitution(rest));
     const uint32_t i = idxS[s];
     Xn[i] = make_float4(px, py, pz, 1.0f);
     V[i] = make_float4(vx, vy, vz, 0.0f);
@@ -396,9 +434,9 @@ inline uint32_t blocks_for(uint32_t n) { return ceil_div(n > 0 ? n : 1, kBS); }
 }  // namespace
 
 void launch_predict_hash(const float4* X, const float4* V, float4* Xn, uint32_t* keys, uint32_t* epoch,
-                         const sim::SimParams& p, hipStream_t s) {
+                         const sim::SimParams& p, const Scenes& sc, hipStream_t s) {
     const uint32_t N = p.numParticles;
-    k_predict_hash<<<blocks_for(N), kBS, 0, s>>>(X, V, Xn, keys, epoch, p.gravity, p.dt, p.grid, N);
+    k_predict_hash<<<blocks_for(N), kBS, 0, s>>>(X, V, Xn, keys, epoch, p.gravity, p.dt, p.grid, N, sc);
     check_launch("predict_hash");
 }
 
@@ -411,20 +449,20 @@ void launch_hash_insert(const HashTable& t, const uint32_t* uniq, const uint32_t
 void launch_spans_gather(CellLookup lookup, const float4* Xn, const uint32_t* idxS, float4* Ps, uint32_t N,
                          const uint32_t* uniq, const uint32_t* off, const uint32_t* count, const uint32_t* cellStart,
                          const uint32_t* cellEnd, const HashTable& table, const SpanTable& st,
-                         const sim::GridBounds& grid, hipStream_t s) {
+                         const sim::GridBounds& grid, const Scenes& sc, hipStream_t s) {
     if (N == 0) return;
     const uint32_t b = blocks_for(N);
     switch (lookup) {
         case CellLookup::Dense:
-            k_spans_gather<<<b, kBS, 0, s>>>(Xn, idxS, Ps, N, uniq, count, DenseRows{cellStart, cellEnd}, st, grid);
+            k_spans_gather<<<b, kBS, 0, s>>>(Xn, idxS, Ps, N, uniq, count, DenseRows{cellStart, cellEnd}, st, grid, sc);
             break;
         case CellLookup::Compact:
             k_spans_gather<<<b, kBS, 0, s>>>(Xn, idxS, Ps, N, uniq, count, CompactRowsCount{uniq, off, count}, st,
-                                             grid);
+                                             grid, sc);
             break;
         case CellLookup::Hashed:
             k_spans_gather<<<b, kBS, 0, s>>>(Xn, idxS, Ps, N, uniq, count,
-                                             HashRowsDev{table.slots, table.mask, table.epoch}, st, grid);
+                                             HashRowsDev{table.slots, table.mask, table.epoch}, st, grid, sc);
             break;
     }
     check_launch("spans_gather");
@@ -439,24 +477,25 @@ void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st
 
 void launch_delta_sorted(const float4* Pa, float4* Pb, const uint32_t* segOf, const SpanTable& st,
                          const sim::DeviceParams& dp, uint32_t N, bool last, const float4* X, const uint32_t* idxS,
-                         float4* Vs, float invDt, hipStream_t s) {
+                         float4* Vs, float invDt, const Scenes& sc, hipStream_t s) {
     if (N == 0) return;
-    k_delta_sorted<<<blocks_for(N), kBS, 0, s>>>(Pa, Pb, segOf, st, dp, N, last ? 1 : 0, X, idxS, Vs, invDt);
+    k_delta_sorted<<<blocks_for(N), kBS, 0, s>>>(Pa, Pb, segOf, st, dp, N, last ? 1 : 0, X, idxS, Vs, invDt, sc);
     check_launch("delta_sorted");
 }
 
 void launch_finalize_sorted(float4* Ps, const float4* X, const uint32_t* idxS, float4* Vs,
-                            const sim::GridBounds& grid, float invDt, uint32_t N, hipStream_t s) {
+                            const sim::GridBounds& grid, float invDt, uint32_t N, const Scenes& sc,
+                            hipStream_t s) {
     if (N == 0) return;
-    k_finalize_sorted<<<blocks_for(N), kBS, 0, s>>>(Ps, X, idxS, Vs, grid, invDt, N);
+    k_finalize_sorted<<<blocks_for(N), kBS, 0, s>>>(Ps, X, idxS, Vs, grid, invDt, N, sc);
     check_launch("finalize_sorted");
 }
 
 void launch_xsph_scatter(const float4* Ps, const float4* Vs, const uint32_t* segOf, const SpanTable& st,
                          const sim::DeviceParams& dp, uint32_t N, const uint32_t* idxS, float4* Xn, float4* V,
-                         hipStream_t s) {
+                         const Scenes& sc, hipStream_t s) {
     if (N == 0) return;
-    k_xsph_scatter<<<blocks_for(N), kBS, 0, s>>>(Ps, Vs, segOf, st, dp, N, idxS, Xn, V);
+    k_xsph_scatter<<<blocks_for(N), kBS, 0, s>>>(Ps, Vs, segOf, st, dp, N, idxS, Xn, V, sc);
     check_launch("xsph_scatter");
 }
 

@@ -155,7 +155,7 @@ __device__ __forceinline__ float3 delta_finish(float dx, float dy, float dz, con
 
 // XSPH update of v_i from the summed (v_j - v_i) W (reference: xsph.cu:80-93).
 __device__ __forceinline__ float3 xsph_finish(float vx, float vy, float vz, float sx, float sy, float sz, int n,
-                                              const sim::DeviceParams& dp) {
+                                              const sim::DeviceParams& dp, float xsph_c) {
     float gate = 1.0f;
     if (dp.pbf.xsph_gate_enable) {
         const int nMin = max(0, dp.pbf.xsph_n_min);
@@ -164,7 +164,7 @@ __device__ __forceinline__ float3 xsph_finish(float vx, float vy, float vz, floa
         gate = fminf(1.f, fmaxf(0.f, t));
     }
     const float mor = (dp.restDensity > 0.f) ? (dp.particleMass / dp.restDensity) : 0.f;
-    const float cg = dp.xsph_c * gate;
+    const float cg = xsph_c * gate;
     return make_float3(vx + cg * mor * sx, vy + cg * mor * sy, vz + cg * mor * sz);
 }
 

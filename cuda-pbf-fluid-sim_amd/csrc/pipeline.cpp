@@ -143,7 +143,7 @@ void RefPost(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_t s
 // ------------------------------------------------------------ reordered path
 void SortedPredict(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_t s) {
     pbfx::launch_predict_hash(b.d_pos, b.d_vel, b.d_pos_pred, g.d_cellKeys,
-                              g.kind == GridKind::Hashed ? g.table.epoch : nullptr, p, s);
+                              g.kind == GridKind::Hashed ? g.table.epoch : nullptr, p, g.scenes, s);
     mark("predict_hash");
 }
 
@@ -166,7 +166,7 @@ void SortedGridBuild(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipSt
     if (lk != pbfx::CellLookup::Compact) mark("cell_lookup");
     pbfx::launch_spans_gather(lk, b.d_pos_pred, g.d_indices_sorted, b.d_pos_sorted[0], N, g.d_cellUniqueKeys,
                               g.d_cellOffsets, g.d_compactCount, g.d_cellStart, g.d_cellEnd, g.table, g.spans,
-                              p.grid, s);
+                              p.grid, g.scenes, s);
     mark("spans_gather");
 }
 
@@ -176,7 +176,7 @@ void SortedSolve(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream
     const float invDt = 1.0f / p.dt;
     if (p.solverIters <= 0) {
         pbfx::launch_finalize_sorted(b.d_pos_sorted[0], b.d_pos, g.d_indices_sorted, b.d_vel_sorted, p.grid, invDt,
-                                     N, s);
+                                     N, g.scenes, s);
         mark("finalize");
         return;
     }
@@ -185,7 +185,8 @@ void SortedSolve(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream
         pbfx::launch_lambda_sorted(b.d_pos_sorted[cur], g.d_segOf, g.spans, dp, N, s);
         mark("lambda");
         pbfx::launch_delta_sorted(b.d_pos_sorted[cur], b.d_pos_sorted[cur ^ 1], g.d_segOf, g.spans, dp, N,
-                                  it == p.solverIters - 1, b.d_pos, g.d_indices_sorted, b.d_vel_sorted, invDt, s);
+                                  it == p.solverIters - 1, b.d_pos, g.d_indices_sorted, b.d_vel_sorted, invDt,
+                                  g.scenes, s);
         mark(it == p.solverIters - 1 ? "delta_last" : "delta");
         cur ^= 1;
     }
@@ -194,7 +195,7 @@ void SortedSolve(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream
 void SortedPost(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_t s) {
     const int fin = p.solverIters > 0 ? (p.solverIters & 1) : 0;
     pbfx::launch_xsph_scatter(b.d_pos_sorted[fin], b.d_vel_sorted, g.d_segOf, g.spans, MakeDeviceParams(p),
-                              p.numParticles, g.d_indices_sorted, b.d_pos_pred, b.d_vel, s);
+                              p.numParticles, g.d_indices_sorted, b.d_pos_pred, b.d_vel, g.scenes, s);
     mark("xsph_scatter");
 }
 
@@ -361,6 +362,73 @@ bool Simulator::initialize(const SimParams& p, const SimOptions& opt) {
         return false;
     }
     const uint64_t cells = (uint64_t)p.grid.dim.x * (uint64_t)p.grid.dim.y * (uint64_t)p.grid.dim.z;
+    return initCommon(p, opt, cells);
+}
+
+bool Simulator::initializeEnsemble(const SimParams& base, const std::vector<SceneDesc>& scenes,
+                                   const SimOptions& opt) {
+    shutdown();
+    if (scenes.empty() || opt.path != PathKind::Reordered || !(base.grid.cellSize > 0.f)) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "initializeEnsemble: needs scenes, the reordered path and a cell size");
+        return false;
+    }
+    const float cs = base.grid.cellSize;
+    uint64_t cells = 0, parts = 0;
+    m_scenes.clear();
+    std::vector<uint32_t> cellBase;
+    for (const SceneDesc& d : scenes) {
+        pbfx::SceneParams sp{};
+        sp.grid.mins = d.boxMin;
+        sp.grid.maxs = d.boxMax;
+        sp.grid.cellSize = cs;
+        // dims as the reference derives them (console.cpp:258-265): max(1, ceil(extent / cell))
+        auto dim1 = [cs](float lo, float hi) { return std::max(1, (int)std::ceil((hi - lo) / cs)); };
+        sp.grid.dim = make_int3(dim1(d.boxMin.x, d.boxMax.x), dim1(d.boxMin.y, d.boxMax.y),
+                                dim1(d.boxMin.z, d.boxMax.z));
+        sp.gravity = d.gravity;
+        sp.xsph_c = d.xsph_c;
+        sp.restitution = d.restitution;
+        sp.cellBase = (uint32_t)cells;
+        sp.particleBase = (uint32_t)parts;
+        sp.numParticles = d.numParticles;
+        cellBase.push_back((uint32_t)cells);
+        cells += (uint64_t)sp.grid.dim.x * sp.grid.dim.y * sp.grid.dim.z;
+        parts += d.numParticles;
+        if (cells > 0xFFFFFFFFull || parts > 0xFFFFFFFFull) {
+            set_error(PBFX_E_INVALID_ARGUMENT, "initializeEnsemble: more than 2^32 cells or particles");
+            return false;
+        }
+        m_scenes.push_back(sp);
+    }
+    cellBase.push_back((uint32_t)std::min<uint64_t>(cells, 0xFFFFFFFFull));
+    SimParams p = base;
+    p.numParticles = p.maxParticles = (uint32_t)parts;
+    p.grid = m_scenes[0].grid;
+    p.gravity = m_scenes[0].gravity;
+    p.xsph_c = m_scenes[0].xsph_c;
+    p.boundaryRestitution = m_scenes[0].restitution;
+    if (!initCommon(p, opt, cells)) return false;
+    const size_t B = m_scenes.size();
+    std::vector<uint32_t> sceneOf(parts);
+    for (size_t b = 0; b < B; ++b)
+        std::fill(sceneOf.begin() + m_scenes[b].particleBase,
+                  sceneOf.begin() + m_scenes[b].particleBase + m_scenes[b].numParticles, (uint32_t)b);
+    bool ok = dalloc(d_sceneParams, B) && dalloc(d_sceneOf, parts) && dalloc(d_sceneOfSlot, parts) &&
+              dalloc(d_cellBase, B + 1);
+    ok = ok && hipMemcpy(d_sceneParams, m_scenes.data(), sizeof(pbfx::SceneParams) * B, hipMemcpyHostToDevice) ==
+                   hipSuccess;
+    ok = ok && hipMemcpy(d_sceneOf, sceneOf.data(), sizeof(uint32_t) * parts, hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(d_cellBase, cellBase.data(), sizeof(uint32_t) * (B + 1), hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) {
+        set_error(PBFX_E_CAPACITY, "initializeEnsemble: scene table upload failed");
+        shutdown();
+        return false;
+    }
+    m_grid.scenes = pbfx::Scenes{d_sceneParams, d_sceneOf, d_sceneOfSlot, d_cellBase, (uint32_t)B};
+    return true;
+}
+
+bool Simulator::initCommon(const SimParams& p, const SimOptions& opt, uint64_t cells) {
     if (cells > 0xFFFFFFFFull) {
         set_error(PBFX_E_INVALID_ARGUMENT, "Simulator::initialize: more than 2^32 cells");
         return false;
@@ -400,6 +468,15 @@ bool Simulator::initialize(const SimParams& p, const SimOptions& opt) {
     return true;
 }
 
+void Simulator::releaseScenes() {
+    dfree(d_sceneParams);
+    dfree(d_sceneOf);
+    dfree(d_sceneOfSlot);
+    dfree(d_cellBase);
+    m_scenes.clear();
+    m_grid.scenes = pbfx::Scenes{};
+}
+
 void Simulator::destroyGraphs() {
     for (auto& e : m_exec) {
         if (e) (void)hipGraphExecDestroy(e);
@@ -409,6 +486,7 @@ void Simulator::destroyGraphs() {
 
 void Simulator::shutdown() {
     destroyGraphs();
+    releaseScenes();
     m_bufs.release();
     m_grid.release();
     if (m_stream) (void)hipStreamDestroy(m_stream);
@@ -502,13 +580,37 @@ bool Simulator::profilePhases(std::vector<std::pair<std::string, float>>& out) {
 
 bool Simulator::seedBoxLattice(uint32_t first, uint32_t nx, uint32_t ny, uint32_t nz, float3 o, float spacing,
                                float amp, uint64_t seed) {
+    return fillLattice(m_params.grid, first, nx, ny, nz, o, spacing, amp, seed);
+}
+
+bool Simulator::seedSceneLattice(uint32_t scene, uint32_t nx, uint32_t ny, uint32_t nz, float3 o, float spacing,
+                                 float amp, uint64_t seed) {
+    if (scene >= m_scenes.size() || (uint64_t)nx * ny * nz > m_scenes[scene].numParticles) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "seedSceneLattice: bad scene or block larger than the scene");
+        return false;
+    }
+    return fillLattice(m_scenes[scene].grid, m_scenes[scene].particleBase, nx, ny, nz, o, spacing, amp, seed);
+}
+
+bool Simulator::getSceneState(uint32_t scene, float4* pos, float4* vel) const {
+    if (!m_initialized || scene >= m_scenes.size()) return false;
+    if (hipStreamSynchronize(m_stream) != hipSuccess) return false;
+    const pbfx::SceneParams& sp = m_scenes[scene];
+    const size_t bytes = sizeof(float4) * sp.numParticles;
+    bool ok = true;
+    if (pos) ok = ok && hipMemcpy(pos, m_bufs.d_pos + sp.particleBase, bytes, hipMemcpyDeviceToHost) == hipSuccess;
+    if (vel) ok = ok && hipMemcpy(vel, m_bufs.d_vel + sp.particleBase, bytes, hipMemcpyDeviceToHost) == hipSuccess;
+    return ok;
+}
+
+bool Simulator::fillLattice(const GridBounds& g, uint32_t first, uint32_t nx, uint32_t ny, uint32_t nz, float3 o,
+                            float spacing, float amp, uint64_t seed) {
     const uint64_t n64 = (uint64_t)nx * ny * nz;
     if (!m_initialized || first + n64 > m_bufs.capacity) {
         set_error(PBFX_E_CAPACITY, "seedBoxLattice: block exceeds capacity");
         return false;
     }
     const uint32_t n = (uint32_t)n64;
-    const GridBounds& g = m_params.grid;
     std::vector<float4> h(n);
     uint32_t idx = 0;
     for (uint32_t iz = 0; iz < nz; ++iz)
@@ -573,6 +675,10 @@ bool Simulator::getState(float4* pos, float4* vel, bool toDevice) const {
 
 bool Simulator::densityStats(double* out5) {
     if (!m_initialized) return false;
+    if (!m_scenes.empty()) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "densityStats: single-scene simulators only");
+        return false;
+    }
     const uint32_t N = m_params.numParticles;
     const hipStream_t s = m_stream;
     GridBuffers& g = m_grid;
@@ -583,7 +689,7 @@ bool Simulator::densityStats(double* out5) {
                            g.d_blockCounts, g.d_blockOffsets, s);
     pbfx::launch_spans_gather(pbfx::CellLookup::Compact, m_bufs.d_pos, g.d_indices_sorted, m_bufs.d_pos_sorted[0], N,
                               g.d_cellUniqueKeys, g.d_cellOffsets, g.d_compactCount, nullptr, nullptr, g.table,
-                              g.spans, m_params.grid, s);
+                              g.spans, m_params.grid, g.scenes, s);
     return pbfx::density_stats_sorted(m_bufs.d_pos_sorted[0], m_bufs.d_vel, g.d_indices_sorted, g.d_segOf, g.spans,
                                       MakeDeviceParams(m_params), N, out5, s);
 }

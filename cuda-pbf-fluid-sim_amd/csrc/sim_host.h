@@ -73,6 +73,7 @@ struct GridBuffers {
     int keyBits = 32;
     pbfx::SpanTable spans{};
     pbfx::HashTable table{};
+    pbfx::Scenes scenes{};  // ensemble view (empty for one scene)
     uint32_t tableLog2 = 0;
     uint32_t capacity = 0;
     GridKind kind = GridKind::Dense;
@@ -142,6 +143,16 @@ private:
 // post_ops.cpp:73-78: integrate, grid build, solve, velocity, XSPH, boundary).
 void BuildDefaultPipelines(PhasePipeline& pipeline, PathKind path);
 
+// One scene of a batched ensemble (see Simulator::initializeEnsemble).
+struct SceneDesc {
+    float3 boxMin;
+    float3 boxMax;
+    float3 gravity;
+    float xsph_c;
+    float restitution;
+    uint32_t numParticles;
+};
+
 struct SimOptions {
     GridKind grid = GridKind::Dense;
     PathKind path = PathKind::Reordered;
@@ -157,6 +168,12 @@ public:
     Simulator& operator=(const Simulator&) = delete;
 
     bool initialize(const SimParams& p, const SimOptions& opt);
+    // Batched ensemble: independent scenes packed into one particle array and one
+    // key space, advanced by the same launches (one per phase for all scenes).
+    // `base` gives the shared constants (h, rest density, mass, dt, iterations,
+    // tuning, cell size); each scene has its own box, gravity, XSPH coefficient,
+    // restitution and particle count.  Reordered path only.
+    bool initializeEnsemble(const SimParams& base, const std::vector<SceneDesc>& scenes, const SimOptions& opt);
     void shutdown();
     // One step with the given parameters (reference: simulator.cpp:560-643).
     bool step(const SimParams& p);
@@ -169,6 +186,13 @@ public:
     // jitter of +-jitterAmp from SplitMix64(seed, particle, axis).  Zero velocity.
     bool seedBoxLattice(uint32_t first, uint32_t nx, uint32_t ny, uint32_t nz, float3 origin, float spacing,
                         float jitterAmp, uint64_t seed);
+    bool seedSceneLattice(uint32_t scene, uint32_t nx, uint32_t ny, uint32_t nz, float3 origin, float spacing,
+                          float jitterAmp, uint64_t seed);
+    bool getSceneState(uint32_t scene, float4* pos, float4* vel) const;
+    uint32_t sceneCount() const { return (uint32_t)m_scenes.size(); }
+    const pbfx::SceneParams* sceneParams(uint32_t scene) const {
+        return scene < m_scenes.size() ? &m_scenes[scene] : nullptr;
+    }
     bool setState(const float4* pos, const float4* vel, uint32_t N, bool fromDevice);
     bool getState(float4* pos, float4* vel, bool toDevice) const;
 
@@ -187,6 +211,16 @@ private:
     bool captureGraphs();
     void destroyGraphs();
     void runPipeline(hipStream_t s);
+    bool initCommon(const SimParams& p, const SimOptions& opt, uint64_t cells);
+    bool fillLattice(const GridBounds& g, uint32_t first, uint32_t nx, uint32_t ny, uint32_t nz, float3 o,
+                     float spacing, float amp, uint64_t seed);
+    void releaseScenes();
+
+    std::vector<pbfx::SceneParams> m_scenes;
+    pbfx::SceneParams* d_sceneParams = nullptr;
+    uint32_t* d_sceneOf = nullptr;
+    uint32_t* d_sceneOfSlot = nullptr;
+    uint32_t* d_cellBase = nullptr;
 
     SimParams m_params{};
     SimOptions m_opt{};

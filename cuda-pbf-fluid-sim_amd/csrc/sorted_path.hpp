@@ -39,10 +39,33 @@ struct SpanTable {
     int rows = 9;
 };
 
+// Per-scene parameters of a batched ensemble.  All scenes share h, rest density,
+// mass, dt, solver settings and the PBF tuning; these fields may differ.
+struct SceneParams {
+    sim::GridBounds grid;   // box and cell grid of the scene
+    float3 gravity;
+    float xsph_c;
+    float restitution;
+    uint32_t cellBase;      // first key of the scene in the ensemble's key space
+    uint32_t particleBase;  // first particle (and first sorted slot) of the scene
+    uint32_t numParticles;
+};
+static_assert(sizeof(SceneParams) == 72, "SceneParams layout");
+
+// Device view of an ensemble.  With params == nullptr the kernels describe one
+// scene through their own parameters.
+struct Scenes {
+    const SceneParams* params = nullptr;
+    const uint32_t* sceneOf = nullptr;   // scene of each particle, original order
+    uint32_t* sceneOfSlot = nullptr;     // scene of each sorted slot (written by spans_gather)
+    const uint32_t* cellBase = nullptr;  // count + 1 ascending key bases
+    uint32_t count = 0;
+};
+
 // P = X + (V + g dt) dt, AABB clamp, write Xn = (P, 1) and its cell key.
 // Thread 0 also advances the hash-table epoch when `epoch` is not null.
 void launch_predict_hash(const float4* X, const float4* V, float4* Xn, uint32_t* keys, uint32_t* epoch,
-                         const sim::SimParams& p, hipStream_t s);
+                         const sim::SimParams& p, const Scenes& sc, hipStream_t s);
 
 // Insert the M occupied cells (count on device) into the table.
 void launch_hash_insert(const HashTable& t, const uint32_t* uniq, const uint32_t* off, const uint32_t* count,
@@ -52,7 +75,7 @@ void launch_hash_insert(const HashTable& t, const uint32_t* uniq, const uint32_t
 void launch_spans_gather(CellLookup lookup, const float4* Xn, const uint32_t* idxS, float4* Ps, uint32_t N,
                          const uint32_t* uniq, const uint32_t* off, const uint32_t* count,
                          const uint32_t* cellStart, const uint32_t* cellEnd, const HashTable& table,
-                         const SpanTable& st, const sim::GridBounds& grid, hipStream_t s);
+                         const SpanTable& st, const sim::GridBounds& grid, const Scenes& sc, hipStream_t s);
 
 // lambda of every sorted slot, stored in Ps[s].w.
 void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st, const sim::DeviceParams& dp,
@@ -62,16 +85,16 @@ void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st
 // writes Vs[s] = (Pb[s] - X[idxS[s]]) * invDt.
 void launch_delta_sorted(const float4* Pa, float4* Pb, const uint32_t* segOf, const SpanTable& st,
                          const sim::DeviceParams& dp, uint32_t N, bool last, const float4* X, const uint32_t* idxS,
-                         float4* Vs, float invDt, hipStream_t s);
+                         float4* Vs, float invDt, const Scenes& sc, hipStream_t s);
 
 // No solver iterations: clamp Ps and compute Vs (the post-solve boundary and velocity phases).
 void launch_finalize_sorted(float4* Ps, const float4* X, const uint32_t* idxS, float4* Vs,
-                            const sim::GridBounds& grid, float invDt, uint32_t N, hipStream_t s);
+                            const sim::GridBounds& grid, float invDt, uint32_t N, const Scenes& sc, hipStream_t s);
 
 // XSPH (Jacobi over Vs, skipped when xsph_c <= 0), boundary with restitution,
 // scatter Xn[idxS[s]] = (P, 1) and V[idxS[s]] = (v, 0).
 void launch_xsph_scatter(const float4* Ps, const float4* Vs, const uint32_t* segOf, const SpanTable& st,
                          const sim::DeviceParams& dp, uint32_t N, const uint32_t* idxS, float4* Xn, float4* V,
-                         hipStream_t s);
+                         const Scenes& sc, hipStream_t s);
 
 }  // namespace pbfx

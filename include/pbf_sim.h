@@ -63,6 +63,30 @@ int pbfSimDensityStats(PbfSim* sim, double* out5);
 // 1 when a hash-table insert found no free slot since creation.
 int pbfSimHashOverflow(PbfSim* sim);
 
+// ---- batched ensembles ----------------------------------------------------
+// Independent scenes packed into one particle array and one cell-key space and
+// advanced by the same launches (one per phase for all scenes).  `base` holds the
+// shared constants (h, rest density, mass, dt, solver iterations, PBF tuning,
+// neighbour cap, cell size in base->grid.cellSize); each scene has its own box,
+// gravity, XSPH coefficient, restitution and particle count.  Scene s occupies
+// particles [first_s, first_s + numParticles_s) in scene order.  Every scene
+// evolves bit for bit as it would alone.  No reference counterpart (new form).
+typedef struct {
+    float boxMin[3];
+    float boxMax[3];
+    float gravity[3];
+    float xsph_c;
+    float restitution;
+    uint32_t numParticles;
+} PbfSceneDesc;
+PbfSim* pbfEnsembleCreate(const sim::SimParams* base, const PbfSceneDesc* scenes, uint32_t count, int gridKind,
+                          int tableLog2, int useGraph);
+int pbfEnsembleSeedLattice(PbfSim* sim, uint32_t scene, uint32_t nx, uint32_t ny, uint32_t nz, float ox, float oy,
+                           float oz, float spacing, float jitterAmp, uint64_t seed);
+int pbfEnsembleGetScene(PbfSim* sim, uint32_t scene, float4* pos, float4* vel);
+// first particle, particle count and grid dims (3 ints) of a scene
+int pbfEnsembleSceneInfo(PbfSim* sim, uint32_t scene, uint32_t* first, uint32_t* count, int* dims3);
+
 // ABI self-checks (host only): copy by-value structs back out, so a foreign
 // caller can verify it passes them with the same layout and calling convention.
 void pbfDebugEchoDeviceParams(sim::DeviceParams dp, sim::DeviceParams* out);
