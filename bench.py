@@ -44,6 +44,53 @@ def sort_bytes(key_bits):
     return 16.0 * passes + 4.0
 
 
+def bench_batched(count, warmup, steps):
+    """The batched ensemble (one launch per phase for all scenes, graph replay) against the
+    scene-by-scene plain path: the same scenes as separate simulators, each replaying its own
+    graph on its own stream, all enqueued before one synchronisation (and, as a second
+    figure, synchronised scene by scene)."""
+    from pbfx.ensemble import Ensemble, ensemble_scenes, solo
+    scenes = ensemble_scenes(count, seed=2024)
+    total = sum(sc.num_particles for sc in scenes)
+    ens = Ensemble(scenes, grid="hashed")
+    ens.step(warmup)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ens.step(steps)
+    ens.synchronize()
+    t_ens = time.perf_counter() - t0
+    ens.close()
+    print(f"[bench] ensemble {count} scenes / {total} particles: {steps} steps in {t_ens:.4f} s",
+          file=sys.stderr, flush=True)
+    sims = [solo(sc, grid="hashed") for sc in scenes]
+    for sim in sims:
+        sim.step(warmup)
+    for sim in sims:
+        sim.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for sim in sims:
+            sim.step(1)
+    for sim in sims:
+        sim.synchronize()
+    t_streams = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for sim in sims:
+            sim.step(1)
+            sim.synchronize()
+    t_serial = time.perf_counter() - t0
+    for sim in sims:
+        sim.close()
+    print(f"[bench] scene-by-scene: {t_streams:.4f} s (streams), {t_serial:.4f} s (serial)", file=sys.stderr,
+          flush=True)
+    return {"scenes": count, "particles": total, "steps": steps, "after_warmup_steps": warmup,
+            "particle_steps_per_s": round(total * steps / t_ens, 1),
+            "scene_by_scene_particle_steps_per_s": round(total * steps / t_streams, 1),
+            "speedup_vs_scene_by_scene": round(t_streams / t_ens, 3),
+            "speedup_vs_serial_scenes": round(t_serial / t_ens, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,6 +100,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--grid", default="hashed", choices=["hashed", "compact", "dense"])
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--batch-warmup", type=int, default=10)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
@@ -126,6 +177,9 @@ def main():
         "hash_overflow": overflow,
         "finite": bool((X == X).all() and (V == V).all()),
     }
+
+    if not a.no_batched:
+        out["batched"] = bench_batched(a.batch_scenes, a.batch_warmup, a.batch_steps)
 
     if not a.no_cpu_baseline:
         from oracle import oracle as O

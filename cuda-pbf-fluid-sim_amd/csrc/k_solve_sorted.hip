@@ -222,12 +222,24 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
     return v;
 }
 
-template <class InRange, class Apply>
+// Candidate mask: bit c of particle s is set when candidate c of its walk (the
+// concatenated neighbour-row spans, in walk order) was within the skin radius
+// h + 2 * I * maxDisp when the first delta pass scanned the predicted positions.
+// Every later pass of the step moves a particle by at most I * maxDisp (each
+// delta is clamped to maxDisp; the box clamp is a projection and cannot increase
+// distances), so a pair outside the skin cannot come within h later in the step.
+// Later passes visit only the set bits, in walk order, and still test r2 <= h2.
+// The mask covers the first 32 * kMaskWords candidates; candidates past that are
+// tested directly.
+constexpr uint32_t kMaskWords = kCandMaskWords;
+
+template <bool kBuild, class Dist2, class Apply>
 __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, uint32_t m, const SpanTable& st,
-                                              const float4* P, float4* buf, uint32_t* q, int cap,
-                                              InRange&& in_range, Apply&& apply) {
+                                              const CandMask& mk, const float4* P, float4* buf, uint32_t* q,
+                                              int cap, float h2, Dist2&& dist2, Apply&& apply) {
     const uint32_t lane = threadIdx.x & 63;
     int qh = 0, qn = 0, found = 0;
+    uint32_t cidx = 0, word = 0;  // mask building: own-candidate index and pending word
     auto push = [&](uint32_t k) {
         q[((qh + qn) & (kQD - 1)) * kBS] = k;
         ++qn;
@@ -239,6 +251,22 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, uint32_t m
             qh = (qh + 1) & (kQD - 1);
             --qn;
             apply(k);
+        }
+    };
+    // one own candidate: record its skin bit, queue it when in range
+    auto visit = [&](uint32_t k, float r2, uint32_t& hi, uint32_t lo) {
+        if (kBuild) {
+            if (r2 <= mk.rs2) word |= 1u << (cidx & 31);
+            ++cidx;
+            if ((cidx & 31) == 0) {
+                const uint32_t w = (cidx >> 5) - 1;
+                if (w < kMaskWords) mk.words[(uint64_t)w * mk.stride + s] = word;
+                word = 0;
+            }
+        }
+        if (k != s && r2 <= h2) {
+            push(k);
+            if (cap > 0 && found >= cap) hi = lo;
         }
     };
     for (int r = 0; r < st.rows; ++r) {
@@ -261,10 +289,7 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, uint32_t m
                 if (__any(qn == kQD)) pop_one();
                 if (k < hi && qn < kQD) {
                     const uint32_t cand = k++;
-                    if (cand != s && in_range(P[cand])) {
-                        push(cand);
-                        if (cap > 0 && found >= cap) k = hi;
-                    }
+                    visit(cand, dist2(P[cand]), hi, lo);
                 }
             }
         } else {
@@ -277,21 +302,96 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, uint32_t m
                 for (uint32_t t = 0; t < n; ++t) {
                     if (__any(qn == kQD)) pop_one();
                     const uint32_t k = base + t;
-                    const float4 pj = buf[t];
-                    if (k >= lo && k < hi && k != s && in_range(pj)) {
-                        push(k);
-                        if (cap > 0 && found >= cap) hi = lo;
-                    }
+                    if (k >= lo && k < hi) visit(k, dist2(buf[t]), hi, lo);
                 }
                 __builtin_amdgcn_wave_barrier();
+            }
+        }
+    }
+    if (kBuild && valid && (cidx & 31) != 0) {
+        const uint32_t w = cidx >> 5;
+        if (w < kMaskWords) mk.words[(uint64_t)w * mk.stride + s] = word;
+    }
+    while (__any(qn > 0)) pop_one();
+}
+
+// Walk of the set mask bits (then the unmasked tail) in walk order, per lane.
+template <class Dist2, class Apply>
+__device__ __forceinline__ void mask_walk(bool valid, uint32_t s, uint32_t m, const SpanTable& st,
+                                          const CandMask& mk, const float4* P, uint32_t* q, int cap, float h2,
+                                          Dist2&& dist2, Apply&& apply) {
+    int qh = 0, qn = 0, found = 0;
+    auto pop_one = [&]() {
+        if (qn > 0) {
+            const uint32_t k = q[qh * kBS];
+            qh = (qh + 1) & (kQD - 1);
+            --qn;
+            apply(k);
+        }
+    };
+    uint32_t total = 0;
+    if (valid) {
+        for (int r = 0; r < st.rows; ++r) {
+            const uint2 sp = st.spans[(uint64_t)r * st.stride + m];
+            total += sp.y - sp.x;
+        }
+    }
+    // row cursor: candidate indices [rowStart, rowStart + rowLen) are slots rowLo + (ci - rowStart)
+    int r = -1;
+    uint32_t rowStart = 0, rowLen = 0, rowLo = 0;
+    const uint32_t nw = min((total + 31u) >> 5, kMaskWords);
+    uint32_t w = 0, wbase = 0, word = 0, tail = kMaskWords * 32u;
+    bool inWords = true;
+    bool active = valid && total > 0;
+    while (__any(active)) {
+        if (__any(qn == kQD)) pop_one();
+        if (active && qn < kQD) {
+            bool have = false;
+            uint32_t ci = 0;
+            if (inWords) {
+                if (word == 0) {
+                    if (w < nw) {
+                        word = mk.words[(uint64_t)w * mk.stride + s];
+                        wbase = w * 32u;
+                        ++w;
+                    } else {
+                        inWords = false;
+                    }
+                } else {
+                    ci = wbase + (uint32_t)__builtin_ctz(word);
+                    word &= word - 1u;
+                    have = true;
+                }
+            } else if (tail < total) {
+                ci = tail++;
+                have = true;
+            } else {
+                active = false;
+            }
+            if (have) {
+                while (ci >= rowStart + rowLen) {
+                    rowStart += rowLen;
+                    ++r;
+                    const uint2 sp = st.spans[(uint64_t)r * st.stride + m];
+                    rowLo = sp.x;
+                    rowLen = sp.y - sp.x;
+                }
+                const uint32_t k = rowLo + (ci - rowStart);
+                if (k != s && dist2(P[k]) <= h2) {
+                    q[((qh + qn) & (kQD - 1)) * kBS] = k;
+                    ++qn;
+                    ++found;
+                    if (cap > 0 && found >= cap) active = false;
+                }
             }
         }
     }
     while (__any(qn > 0)) pop_one();
 }
 
+// Lambda of a sorted slot: the full walk (first iteration) or the mask walk.
 __global__ __launch_bounds__(kBS) void k_lambda_sorted(float4* Ps, const uint32_t* __restrict__ segOf, SpanTable st,
-                                                       sim::DeviceParams dp, uint32_t N) {
+                                                       CandMask mk, int useMask, sim::DeviceParams dp, uint32_t N) {
     __shared__ uint32_t sQ[kQD * kBS];
     __shared__ float4 sBuf[kBS];
     const uint32_t s = slot_of_thread();
@@ -301,24 +401,29 @@ __global__ __launch_bounds__(kBS) void k_lambda_sorted(float4* Ps, const uint32_
     const uint32_t m = segOf[ss];
     const PairConsts c = make_pair_consts(dp);
     LambdaAcc a;
-    neighbor_walk(
-        valid, s, m, st, Ps, sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x, dp.maxNeighbors,
-        [&](const float4& pj) {
-            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            return rx * rx + ry * ry + rz * rz <= c.h2;
-        },
-        [&](uint32_t k) {
-            const float4 pj = Ps[k];
-            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            lambda_pair(a, rx, ry, rz, rx * rx + ry * ry + rz * rz, c);
-        });
+    auto dist2 = [&](const float4& pj) {
+        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+        return rx * rx + ry * ry + rz * rz;
+    };
+    auto apply = [&](uint32_t k) {
+        const float4 pj = Ps[k];
+        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+        lambda_pair(a, rx, ry, rz, rx * rx + ry * ry + rz * rz, c);
+    };
+    if (useMask)
+        mask_walk(valid, s, m, st, mk, Ps, sQ + threadIdx.x, dp.maxNeighbors, c.h2, dist2, apply);
+    else
+        neighbor_walk<false>(valid, s, m, st, mk, Ps, sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x,
+                             dp.maxNeighbors, c.h2, dist2, apply);
     if (valid) reinterpret_cast<float*>(Ps)[4ull * s + 3] = lambda_finish(a, dp);
 }
 
+// Delta of a sorted slot.  maskMode: 0 full walk, 1 full walk that builds the
+// candidate mask, 2 mask walk.
 __global__ __launch_bounds__(kBS) void k_delta_sorted(const float4* __restrict__ Pa, float4* __restrict__ Pb,
                                                       const uint32_t* __restrict__ segOf, SpanTable st,
-                                                      sim::DeviceParams dp, uint32_t N, int last,
-                                                      const float4* __restrict__ X,
+                                                      CandMask mk, int maskMode, sim::DeviceParams dp, uint32_t N,
+                                                      int last, const float4* __restrict__ X,
                                                       const uint32_t* __restrict__ idxS, float4* __restrict__ Vs,
                                                       float invDt, Scenes sc) {
     __shared__ uint32_t sQ[kQD * kBS];
@@ -330,17 +435,22 @@ __global__ __launch_bounds__(kBS) void k_delta_sorted(const float4* __restrict__
     const uint32_t m = segOf[ss];
     const PairConsts c = make_pair_consts(dp);
     float dx = 0.f, dy = 0.f, dz = 0.f;
-    neighbor_walk(
-        valid, s, m, st, Pa, sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x, 0,
-        [&](const float4& pj) {
-            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            return rx * rx + ry * ry + rz * rz <= c.h2;
-        },
-        [&](uint32_t k) {
-            const float4 pj = Pa[k];
-            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            delta_pair(dx, dy, dz, rx, ry, rz, rx * rx + ry * ry + rz * rz, pi.w, pj.w, c);
-        });
+    auto dist2 = [&](const float4& pj) {
+        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+        return rx * rx + ry * ry + rz * rz;
+    };
+    auto apply = [&](uint32_t k) {
+        const float4 pj = Pa[k];
+        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+        delta_pair(dx, dy, dz, rx, ry, rz, rx * rx + ry * ry + rz * rz, pi.w, pj.w, c);
+    };
+    float4* buf = sBuf + (threadIdx.x & ~63u);
+    if (maskMode == 2)
+        mask_walk(valid, s, m, st, mk, Pa, sQ + threadIdx.x, 0, c.h2, dist2, apply);
+    else if (maskMode == 1)
+        neighbor_walk<true>(valid, s, m, st, mk, Pa, buf, sQ + threadIdx.x, 0, c.h2, dist2, apply);
+    else
+        neighbor_walk<false>(valid, s, m, st, mk, Pa, buf, sQ + threadIdx.x, 0, c.h2, dist2, apply);
     if (!valid) return;
     const float3 d = delta_finish(dx, dy, dz, dp);
     // reference: pbf_delta_apply.cu:232-233 (P += delta)
@@ -373,7 +483,7 @@ __global__ __launch_bounds__(kBS) void k_finalize_sorted(float4* __restrict__ Ps
 
 __global__ __launch_bounds__(kBS) void k_xsph_scatter(const float4* __restrict__ Ps, const float4* __restrict__ Vs,
                                                       const uint32_t* __restrict__ segOf, SpanTable st,
-                                                      sim::DeviceParams dp, uint32_t N,
+                                                      CandMask mk, int useMask, sim::DeviceParams dp, uint32_t N,
                                                       const uint32_t* __restrict__ idxS, float4* __restrict__ Xn,
                                                       float4* __restrict__ V, Scenes sc) {
     __shared__ uint32_t sQ[kQD * kBS];
@@ -398,23 +508,26 @@ __global__ __launch_bounds__(kBS) void k_xsph_scatter(const float4* __restrict__
         const float h2 = dp.kernel.h2, poly6 = dp.kernel.poly6;
         float sx = 0.f, sy = 0.f, sz = 0.f;
         int n = 0;
-        neighbor_walk(
-            valid && xc > 0.f, s, m, st, Ps, sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x, 0,
-            [&](const float4& pj) {
-                const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-                return rx * rx + ry * ry + rz * rz <= h2;
-            },
-            [&](uint32_t k) {
-                const float4 pj = Ps[k];
-                const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-                const float hr2 = h2 - (rx * rx + ry * ry + rz * rz);
-                const float w = poly6 * hr2 * hr2 * hr2;
-                const float4 vj = Vs[k];
-                sx += (vj.x - vi.x) * w;
-                sy += (vj.y - vi.y) * w;
-                sz += (vj.z - vi.z) * w;
-                ++n;
-            });
+        auto dist2 = [&](const float4& pj) {
+            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+            return rx * rx + ry * ry + rz * rz;
+        };
+        auto apply = [&](uint32_t k) {
+            const float4 pj = Ps[k];
+            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+            const float hr2 = h2 - (rx * rx + ry * ry + rz * rz);
+            const float w = poly6 * hr2 * hr2 * hr2;
+            const float4 vj = Vs[k];
+            sx += (vj.x - vi.x) * w;
+            sy += (vj.y - vi.y) * w;
+            sz += (vj.z - vi.z) * w;
+            ++n;
+        };
+        if (useMask)
+            mask_walk(valid && xc > 0.f, s, m, st, mk, Ps, sQ + threadIdx.x, 0, h2, dist2, apply);
+        else
+            neighbor_walk<false>(valid && xc > 0.f, s, m, st, mk, Ps, sBuf + (threadIdx.x & ~63u),
+                                 sQ + threadIdx.x, 0, h2, dist2, apply);
         if (xc > 0.f) {
             const float3 v = xsph_finish(vi.x, vi.y, vi.z, sx, sy, sz, n, dp, xc);
             vx = v.x; vy = v.y; vz = v.z;
@@ -432,6 +545,15 @@ __global__ __launch_bounds__(kBS) void k_xsph_scatter(const float4* __restrict__
 inline uint32_t blocks_for(uint32_t n) { return ceil_div(n > 0 ? n : 1, kBS); }
 
 }  // namespace
+
+float mask_skin_radius2(const sim::SimParams& p) {
+    if (!p.pbf.enable_disp_clamp || p.solverIters < 1 || !(p.pbf.disp_clamp_max_h >= 0.f)) return 0.f;
+    const double h = p.kernel.h;
+    const double rs = h * (1.0 + 2.0 * p.solverIters * (double)p.pbf.disp_clamp_max_h);
+    if (rs > 2.0 * h) return 0.f;
+    const double r = rs * (1.0 + 1e-3);  // margin for float rounding of positions and r^2
+    return (float)(r * r);
+}
 
 void launch_predict_hash(const float4* X, const float4* V, float4* Xn, uint32_t* keys, uint32_t* epoch,
                          const sim::SimParams& p, const Scenes& sc, hipStream_t s) {
@@ -468,18 +590,20 @@ void launch_spans_gather(CellLookup lookup, const float4* Xn, const uint32_t* id
     check_launch("spans_gather");
 }
 
-void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st, const sim::DeviceParams& dp,
-                          uint32_t N, hipStream_t s) {
+void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st, const CandMask& mk,
+                          bool useMask, const sim::DeviceParams& dp, uint32_t N, hipStream_t s) {
     if (N == 0) return;
-    k_lambda_sorted<<<blocks_for(N), kBS, 0, s>>>(Ps, segOf, st, dp, N);
+    k_lambda_sorted<<<blocks_for(N), kBS, 0, s>>>(Ps, segOf, st, mk, useMask ? 1 : 0, dp, N);
     check_launch("lambda_sorted");
 }
 
 void launch_delta_sorted(const float4* Pa, float4* Pb, const uint32_t* segOf, const SpanTable& st,
-                         const sim::DeviceParams& dp, uint32_t N, bool last, const float4* X, const uint32_t* idxS,
-                         float4* Vs, float invDt, const Scenes& sc, hipStream_t s) {
+                         const CandMask& mk, int maskMode, const sim::DeviceParams& dp, uint32_t N, bool last,
+                         const float4* X, const uint32_t* idxS, float4* Vs, float invDt, const Scenes& sc,
+                         hipStream_t s) {
     if (N == 0) return;
-    k_delta_sorted<<<blocks_for(N), kBS, 0, s>>>(Pa, Pb, segOf, st, dp, N, last ? 1 : 0, X, idxS, Vs, invDt, sc);
+    k_delta_sorted<<<blocks_for(N), kBS, 0, s>>>(Pa, Pb, segOf, st, mk, maskMode, dp, N, last ? 1 : 0, X, idxS, Vs,
+                                                 invDt, sc);
     check_launch("delta_sorted");
 }
 
@@ -492,10 +616,10 @@ void launch_finalize_sorted(float4* Ps, const float4* X, const uint32_t* idxS, f
 }
 
 void launch_xsph_scatter(const float4* Ps, const float4* Vs, const uint32_t* segOf, const SpanTable& st,
-                         const sim::DeviceParams& dp, uint32_t N, const uint32_t* idxS, float4* Xn, float4* V,
-                         const Scenes& sc, hipStream_t s) {
+                         const CandMask& mk, bool useMask, const sim::DeviceParams& dp, uint32_t N,
+                         const uint32_t* idxS, float4* Xn, float4* V, const Scenes& sc, hipStream_t s) {
     if (N == 0) return;
-    k_xsph_scatter<<<blocks_for(N), kBS, 0, s>>>(Ps, Vs, segOf, st, dp, N, idxS, Xn, V, sc);
+    k_xsph_scatter<<<blocks_for(N), kBS, 0, s>>>(Ps, Vs, segOf, st, mk, useMask ? 1 : 0, dp, N, idxS, Xn, V, sc);
     check_launch("xsph_scatter");
 }
 

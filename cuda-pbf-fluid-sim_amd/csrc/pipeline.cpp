@@ -180,12 +180,17 @@ void SortedSolve(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream
         mark("finalize");
         return;
     }
+    // the first delta pass records the candidate mask the later passes walk
+    pbfx::CandMask mk = g.mask;
+    mk.rs2 = pbfx::mask_skin_radius2(p);
+    const bool useMask = g.mask.words != nullptr && mk.rs2 > 0.f;
     int cur = 0;
     for (int it = 0; it < p.solverIters; ++it) {
-        pbfx::launch_lambda_sorted(b.d_pos_sorted[cur], g.d_segOf, g.spans, dp, N, s);
+        pbfx::launch_lambda_sorted(b.d_pos_sorted[cur], g.d_segOf, g.spans, mk, useMask && it > 0, dp, N, s);
         mark("lambda");
-        pbfx::launch_delta_sorted(b.d_pos_sorted[cur], b.d_pos_sorted[cur ^ 1], g.d_segOf, g.spans, dp, N,
-                                  it == p.solverIters - 1, b.d_pos, g.d_indices_sorted, b.d_vel_sorted, invDt,
+        const int maskMode = !useMask ? 0 : (it == 0 ? 1 : 2);
+        pbfx::launch_delta_sorted(b.d_pos_sorted[cur], b.d_pos_sorted[cur ^ 1], g.d_segOf, g.spans, mk, maskMode,
+                                  dp, N, it == p.solverIters - 1, b.d_pos, g.d_indices_sorted, b.d_vel_sorted, invDt,
                                   g.scenes, s);
         mark(it == p.solverIters - 1 ? "delta_last" : "delta");
         cur ^= 1;
@@ -194,8 +199,12 @@ void SortedSolve(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream
 
 void SortedPost(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_t s) {
     const int fin = p.solverIters > 0 ? (p.solverIters & 1) : 0;
-    pbfx::launch_xsph_scatter(b.d_pos_sorted[fin], b.d_vel_sorted, g.d_segOf, g.spans, MakeDeviceParams(p),
-                              p.numParticles, g.d_indices_sorted, b.d_pos_pred, b.d_vel, g.scenes, s);
+    pbfx::CandMask mk = g.mask;
+    mk.rs2 = pbfx::mask_skin_radius2(p);
+    const bool useMask = g.mask.words != nullptr && mk.rs2 > 0.f && p.solverIters > 0;
+    pbfx::launch_xsph_scatter(b.d_pos_sorted[fin], b.d_vel_sorted, g.d_segOf, g.spans, mk, useMask,
+                              MakeDeviceParams(p), p.numParticles, g.d_indices_sorted, b.d_pos_pred, b.d_vel,
+                              g.scenes, s);
     mark("xsph_scatter");
 }
 
@@ -277,6 +286,8 @@ bool GridBuffers::allocate(uint32_t cap, uint32_t nCells, GridKind k, uint32_t t
     ok = ok && dalloc(d_segOf, cap) && dalloc(d_blockCounts, tiles) && dalloc(d_blockOffsets, tiles) &&
          dalloc(spans.spans, (size_t)spans.rows * cap);
     if (reordered) {
+        mask.stride = cap;
+        ok = ok && dalloc(mask.words, (size_t)pbfx::kCandMaskWords * cap);
         if (k == GridKind::Hashed) {
             tableLog2 = tLog2;
             const size_t slots = size_t(1) << tLog2;
@@ -305,6 +316,7 @@ void GridBuffers::release() {
     d_sortTemp = nullptr;
     sortTempBytes = 0;
     dfree(spans.spans);
+    dfree(mask.words);
     dfree(table.slots); dfree(table.epoch); dfree(table.overflow);
     capacity = 0;
 }

@@ -72,6 +72,7 @@ struct GridBuffers {
     size_t sortTempBytes = 0;
     int keyBits = 32;
     pbfx::SpanTable spans{};
+    pbfx::CandMask mask{};  // per-step candidate mask (reordered path)
     pbfx::HashTable table{};
     pbfx::Scenes scenes{};  // ensemble view (empty for one scene)
     uint32_t tableLog2 = 0;

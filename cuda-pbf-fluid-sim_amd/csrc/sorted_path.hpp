@@ -39,6 +39,22 @@ struct SpanTable {
     int rows = 9;
 };
 
+// Candidate mask words per particle: the mask covers the first 32 * kCandMaskWords
+// candidates of a particle's walk; later candidates are tested directly.
+constexpr uint32_t kCandMaskWords = 64;
+
+// Per-step candidate mask (see k_solve_sorted.hip): words[w * stride + s] holds
+// bits 32w..32w+31 of slot s's walk; rs2 is the squared skin radius.
+struct CandMask {
+    uint32_t* words = nullptr;
+    uint32_t stride = 0;
+    float rs2 = 0.f;
+};
+
+// Squared skin radius for `iters` clamped iterations, or 0 when no mask applies
+// (no displacement clamp, or a skin wider than 2h).
+float mask_skin_radius2(const sim::SimParams& p);
+
 // Per-scene parameters of a batched ensemble.  All scenes share h, rest density,
 // mass, dt, solver settings and the PBF tuning; these fields may differ.
 struct SceneParams {
@@ -77,15 +93,17 @@ void launch_spans_gather(CellLookup lookup, const float4* Xn, const uint32_t* id
                          const uint32_t* cellStart, const uint32_t* cellEnd, const HashTable& table,
                          const SpanTable& st, const sim::GridBounds& grid, const Scenes& sc, hipStream_t s);
 
-// lambda of every sorted slot, stored in Ps[s].w.
-void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st, const sim::DeviceParams& dp,
-                          uint32_t N, hipStream_t s);
+// lambda of every sorted slot, stored in Ps[s].w (full walk, or the candidate mask's walk).
+void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st, const CandMask& mk,
+                          bool useMask, const sim::DeviceParams& dp, uint32_t N, hipStream_t s);
 
 // Pb[s] = Pa[s] + delta(s).  On the last iteration also clamps to the box and
-// writes Vs[s] = (Pb[s] - X[idxS[s]]) * invDt.
+// writes Vs[s] = (Pb[s] - X[idxS[s]]) * invDt.  maskMode 0: full walk; 1: full
+// walk that also builds the candidate mask; 2: walk of the mask.
 void launch_delta_sorted(const float4* Pa, float4* Pb, const uint32_t* segOf, const SpanTable& st,
-                         const sim::DeviceParams& dp, uint32_t N, bool last, const float4* X, const uint32_t* idxS,
-                         float4* Vs, float invDt, const Scenes& sc, hipStream_t s);
+                         const CandMask& mk, int maskMode, const sim::DeviceParams& dp, uint32_t N, bool last,
+                         const float4* X, const uint32_t* idxS, float4* Vs, float invDt, const Scenes& sc,
+                         hipStream_t s);
 
 // No solver iterations: clamp Ps and compute Vs (the post-solve boundary and velocity phases).
 void launch_finalize_sorted(float4* Ps, const float4* X, const uint32_t* idxS, float4* Vs,
@@ -94,7 +112,7 @@ void launch_finalize_sorted(float4* Ps, const float4* X, const uint32_t* idxS, f
 // XSPH (Jacobi over Vs, skipped when xsph_c <= 0), boundary with restitution,
 // scatter Xn[idxS[s]] = (P, 1) and V[idxS[s]] = (v, 0).
 void launch_xsph_scatter(const float4* Ps, const float4* Vs, const uint32_t* segOf, const SpanTable& st,
-                         const sim::DeviceParams& dp, uint32_t N, const uint32_t* idxS, float4* Xn, float4* V,
-                         const Scenes& sc, hipStream_t s);
+                         const CandMask& mk, bool useMask, const sim::DeviceParams& dp, uint32_t N,
+                         const uint32_t* idxS, float4* Xn, float4* V, const Scenes& sc, hipStream_t s);
 
 }  // namespace pbfx
