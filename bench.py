@@ -91,6 +91,56 @@ def bench_batched(count, warmup, steps):
             "speedup_vs_serial_scenes": round(t_serial / t_ens, 3)}
 
 
+def bench_adjoint(steps=60, checkpoint=10):
+    """The differentiable rollout: 65,536 particles (64x32x32 lattice), 60 steps, checkpoint every
+    10.  Loss: squared L2 distance between the 32^3 Gaussian occupancy grid (sigma = h) of X_T and
+    that of a target rollout from a seeded perturbed initial velocity.  Times the checkpointed
+    forward, then the reverse pass split into re-recording the segments and the reverse kernels."""
+    import torch
+    from pbfx.diff import Rollout, differentiable_scene, initial_state, occupancy_grid, seeded_velocity
+    scene = differentiable_scene()
+    sp = P.build_sim_params(scene)
+    n = sp.numParticles
+    h = float(sp.kernel.h)
+    X0, _ = initial_state(scene)
+    x0 = torch.from_numpy(X0).cuda()
+    roll = Rollout(sp, steps, checkpoint=checkpoint)
+    try:
+        vt = torch.from_numpy(seeded_velocity(n, 77, 0.5)).cuda()
+        XT, _ = roll.forward(x0, vt, sp.xsph_c)
+        target = occupancy_grid(XT[:, :3].double(), scene.box_min, scene.box_max, cells=32, sigma=h)
+        v0 = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
+        roll.forward(x0, v0, sp.xsph_c)  # warm-up of the same rollout
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        XT, _ = roll.forward(x0, v0, sp.xsph_c)
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+        xT = XT[:, :3].double().requires_grad_(True)
+        loss = ((occupancy_grid(xT, scene.box_min, scene.box_max, cells=32, sigma=h) - target) ** 2).sum()
+        loss.backward()
+        roll.timed = True
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, gV0, gc = roll.backward(xT.grad.float())
+        torch.cuda.synchronize()
+        t_bwd = time.perf_counter() - t0
+        tm = dict(roll.timing)
+        gnorm = float(gV0[:, :3].double().norm())
+    finally:
+        roll.close()
+    print(f"[bench] adjoint rollout {n} particles x {steps} steps: forward {t_fwd:.4f} s, backward {t_bwd:.4f} s",
+          file=sys.stderr, flush=True)
+    return {"particles": n, "steps": steps, "checkpoint": checkpoint, "loss": float(loss),
+            "forward_ms_per_step": round(t_fwd * 1e3 / steps, 4),
+            "record_ms_per_step": round(tm["record_s"] * 1e3 / steps, 4),
+            "reverse_ms_per_step": round(tm["reverse_s"] * 1e3 / steps, 4),
+            "backward_over_forward": round(t_bwd / t_fwd, 3),
+            "reverse_kernels_over_forward": round(tm["reverse_s"] / t_fwd, 3),
+            "grad_v0_norm": gnorm, "grad_c": gc,
+            "note": "forward = eager checkpointed rollout; backward = re-record every segment + reverse kernels"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +154,7 @@ def main():
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--batch-warmup", type=int, default=10)
+    ap.add_argument("--no-adjoint", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
@@ -180,6 +231,9 @@ def main():
 
     if not a.no_batched:
         out["batched"] = bench_batched(a.batch_scenes, a.batch_warmup, a.batch_steps)
+
+    if not a.no_adjoint:
+        out["adjoint"] = bench_adjoint()
 
     if not a.no_cpu_baseline:
         from oracle import oracle as O

@@ -1,0 +1,61 @@
+// adjoint.hpp — reverse-mode kernels of the reordered PBF step.
+//
+// A recorded step (StepTape) freezes what the forward pass decided: the sort
+// permutation, the occupied-cell index and neighbour-row spans, the clamp bits of
+// the predict and post-solve box clamps, the sorted positions of every solver
+// iteration (lambda in w) and the sorted velocities before XSPH.  The reverse pass
+// differentiates the step on that frozen structure: the neighbour sets, the
+// neighbour cap, the n < 2 gate, the lambda and displacement clamps and the box
+// clamps keep the branch the forward pass took.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pbf_params.h"
+#include "sorted_path.hpp"
+
+namespace pbfx {
+
+constexpr int kMaxTapeIters = 8;
+
+struct StepTape {
+    float4* X = nullptr;            // positions at the start of the step (particle order)
+    float4* V = nullptr;            // velocities at the start of the step (particle order)
+    uint32_t* idxS = nullptr;       // sorted slot -> particle
+    uint32_t* segOf = nullptr;      // sorted slot -> occupied cell
+    uint2* spans = nullptr;         // neighbour-row spans of the step (rows x stride)
+    float4* P[kMaxTapeIters + 1] = {};  // P[it]: sorted positions of iteration it (w = lambda); P[I]: final
+    float4* Vs = nullptr;           // sorted velocities after the velocity phase
+    uint32_t* predClamp = nullptr;  // particle order: bit a set when the predict clamp moved axis a
+    uint32_t* finalClamp = nullptr; // sorted: bit a set when the post-solve clamp moved axis a
+};
+
+// Scratch of the reverse pass (sorted order unless noted).
+struct AdjointScratch {
+    float4* gPf = nullptr;    // dL/d(final position)
+    float4* gVo = nullptr;    // dL/d(velocity after XSPH)
+    float4* gVn = nullptr;    // dL/d(velocity before XSPH)
+    float4* gP = nullptr;     // dL/d(positions of the current iteration)
+    float4* a = nullptr;      // delta adjoint vectors
+    float* gLam = nullptr;    // dL/d(lambda)
+    float4* coefA = nullptr;  // lambda adjoint: (sum of spiky gradients, beta)
+    float4* coefB = nullptr;  // lambda adjoint: (epsilon, cap cut-off row, cut-off slot, -)
+    float* kappa = nullptr;   // XSPH coefficient per slot (c * gate * m / rho0)
+    double* partial = nullptr;  // per-block partial sums of dL/dc
+    double* gc = nullptr;       // dL/dc of the step
+    uint32_t blocks = 0;
+};
+
+// Reverse of one recorded step.  gXin/gVin: dL/dX_{t+1}, dL/dV_{t+1} (particle
+// order).  gXout/gVout: dL/dX_t, dL/dV_t (particle order; may not alias the
+// inputs).  Adds dL/dc of the step into *gcAccum (device double).
+void adjoint_step(const StepTape& t, const SpanTable& st, const sim::SimParams& p, const float4* gXin,
+                  const float4* gVin, float4* gXout, float4* gVout, double* gcAccum, AdjointScratch& w,
+                  hipStream_t s);
+
+// Clamp bits (bit a: axis a was moved by the box clamp) are written by the
+// forward kernels when recording: launch_predict_hash(..., clampBits) in particle
+// order and launch_delta_sorted(..., clampBits) on the last iteration, sorted.
+
+}  // namespace pbfx

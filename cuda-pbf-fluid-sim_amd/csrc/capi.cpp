@@ -56,11 +56,7 @@ extern "C" int pbfSimStep(PbfSim* sim, int steps) {
 
 extern "C" int pbfSimSetParams(PbfSim* sim, const sim::SimParams* params) {
     if (!sim || !params) return -1;
-    // step(p) applies the parameters and advances once; here only apply them.
-    sim::SimParams p = *params;
-    if (std::memcmp(&p, &sim->s.params(), sizeof(p)) == 0) return 0;
-    if (!sim->s.step(p)) return -1;
-    return 0;
+    return sim->s.setParams(*params) ? 0 : -1;
 }
 
 extern "C" int pbfSimSynchronize(PbfSim* sim) {
@@ -172,4 +168,20 @@ extern "C" int pbfEnsembleSceneInfo(PbfSim* sim, uint32_t scene, uint32_t* first
         dims3[2] = sp->grid.dim.z;
     }
     return 0;
+}
+
+extern "C" int pbfSimAdjointInit(PbfSim* sim, int tapeSlots) {
+    if (!sim) return -1;
+    return sim->s.adjointInit(tapeSlots) ? 0 : -1;
+}
+
+extern "C" int pbfSimStepRecord(PbfSim* sim, int slot) {
+    if (!sim) return -1;
+    return sim->s.stepRecord(slot) ? 0 : -1;
+}
+
+extern "C" int pbfSimBackwardStep(PbfSim* sim, int slot, const float4* gXin, const float4* gVin, float4* gXout,
+                                  float4* gVout, double* gcAccum) {
+    if (!sim || !gXin || !gVin || !gXout || !gVout || !gcAccum) return -1;
+    return sim->s.backwardStep(slot, gXin, gVin, gXout, gVout, gcAccum) ? 0 : -1;
 }

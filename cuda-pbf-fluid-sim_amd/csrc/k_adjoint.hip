@@ -1,0 +1,407 @@
+// k_adjoint.hip — hand-written reverse pass of one recorded PBF step.
+//
+// Forward step (sorted order, per particle i; reference: phases.cpp:30-106,
+// post_ops.cpp:25-71):
+//   P0 = clamp(X + (V + g dt) dt)
+//   for it: lambda_i = -C_i / den_i,  C_i = rho_i/rho0 - 1,  rho_i = m sum_j W6(r_ij),
+//           den_i = (m/rho0)^2 (sum_j |g_ij|^2 + |sum_j g_ij|^2) + eps (+ compliance/dt^2)
+//           u_i = sum_j (lambda_i + lambda_j + scorr_ij) g_ij,   D_i = clampDisp(omega/rho0 * u_i)
+//           P_{it+1} = P_it + D
+//   Pf = clamp(P_I);  Vn = (Pf - X) / dt
+//   Vout_i = Vn_i + kappa_i sum_j (Vn_j - Vn_i) W6(Pf_i - Pf_j),  kappa_i = c gate_i m / rho0
+// with g_ij = f(r) r_ij the spiky gradient, f(r) = -3 spiky (h - r)^2 / max(r, eps).
+//
+// Reverse, in the order below; each kernel gathers over its own neighbours, so
+// every particle's gradient is written by one lane (no atomics, deterministic):
+//   gather   dL/dPf, dL/dVout to sorted order
+//   xsph     dL/dVn_k = gVo_k (1 - kappa_k sum_j W_kj) + sum_j kappa_j W_kj gVo_j
+//            dL/dPf_k += sum_j ((kappa_k gVo_k - kappa_j gVo_j) . (Vn_j - Vn_k)) dW6_kj
+//            dL/dc    += gVo_k . (gate_k m/rho0 sum_j (Vn_j - Vn_k) W_kj)
+//   velocity dL/dP_I = finalMask (gPf + gVn / dt);   dL/dX = -gVn / dt
+//   per iteration, last first:
+//     delta_pre  a_i = (omega/rho0) J_i^T gP_i   (J: Jacobian of the displacement clamp)
+//     delta      dL/dlambda_k = sum_j g_kj . (a_k - a_j)
+//                dL/dP_k += sum_j T_kj (a_k - a_j),  T_kj = c_kj H_kj + g_kj (x) dscorr_kj
+//                (H the Hessian of the spiky kernel; T is even in r_kj, hence symmetric)
+//     lambda_pre beta_i = gLam_i (-1/den_i) (m/rho0),  eps_i = gLam_i (C_i/den_i^2) (m/rho0)^2
+//                (zero where the forward gated, clamped or floored lambda)
+//     lambda     dL/dP_k += sum_{j in Ncap(k)} [beta_k dW6_kj + 2 eps_k H_kj (g_kj + G_k)]
+//                         + sum_{j : k in Ncap(j)} [beta_j dW6_kj + 2 eps_j H_kj (g_kj - G_j)]
+//                (Ncap: the first maxNeighbors in-range neighbours in walk order)
+//   predict  dL/dX += predMask gP0,  dL/dV = dt predMask gP0
+#include "adjoint.hpp"
+#include "pbf_common.hpp"
+#include "pbf_hip.h"
+#include "pbf_math.hpp"
+
+namespace pbfx {
+namespace {
+
+constexpr int kBS = 256;
+
+__device__ __forceinline__ float3 f3(const float4& v) { return make_float3(v.x, v.y, v.z); }
+__device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 sub3(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 add3(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 mul3(float3 a, float s) { return make_float3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float4 mk4(float3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+
+__device__ __forceinline__ float3 mask3(float3 g, uint32_t bits) {
+    return make_float3((bits & 1u) ? 0.f : g.x, (bits & 2u) ? 0.f : g.y, (bits & 4u) ? 0.f : g.z);
+}
+
+// Spiky gradient factor and its Hessian applied to a vector: g = f r,
+// H v = f v + (df/dr / |r|) (r . v) r.
+struct Spiky {
+    float f, dfr;  // f(r) and (df/dr)/|r|
+};
+__device__ __forceinline__ Spiky spiky_of(float r2, const PairConsts& c) {
+    const float r = sqrtf(r2);
+    const float t = c.h - r;
+    Spiky s;
+    if (r > c.grad_r_eps) {
+        s.f = c.neg3_spiky * (t * t) / r;
+        // f = -3 spiky t^2 / r  ->  df/dr = 3 spiky (2 t / r + t^2 / r^2)
+        s.dfr = (-c.neg3_spiky) * (2.0f * t / r + (t * t) / (r * r)) / r;
+    } else {
+        s.f = c.neg3_spiky * (t * t) / c.grad_r_eps;
+        // r below the clamp: f depends on r only through t^2
+        s.dfr = (r > 0.f) ? ((-c.neg3_spiky) * 2.0f * t / c.grad_r_eps) / r : 0.f;
+    }
+    return s;
+}
+__device__ __forceinline__ float3 hess_apply(const Spiky& s, float3 r, float3 v) {
+    return add3(mul3(v, s.f), mul3(r, s.dfr * dot3(r, v)));
+}
+
+// Walk every neighbour slot k != s in walk order; fn(row, k) returns false to stop.
+template <class Fn>
+__device__ __forceinline__ void walk_all(uint32_t s, uint32_t m, const SpanTable& st, Fn&& fn) {
+    for (int r = 0; r < st.rows; ++r) {
+        const uint2 sp = st.spans[(uint64_t)r * st.stride + m];
+        for (uint32_t k = sp.x; k < sp.y; ++k) {
+            if (k == s) continue;
+            if (!fn(r, k)) return;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBS) void k_adj_gather(const uint32_t* __restrict__ idxS, const float4* __restrict__ gX,
+                                                    const float4* __restrict__ gV, float4* __restrict__ gPf,
+                                                    float4* __restrict__ gVo, uint32_t N) {
+    const uint32_t s = blockIdx.x * kBS + threadIdx.x;
+    if (s >= N) return;
+    const uint32_t i = idxS[s];
+    gPf[s] = gX[i];
+    gVo[s] = gV[i];
+}
+
+// XSPH coefficient per slot: c * gate(n) * m / rho0 (reference: xsph.cu:80-93).
+__global__ __launch_bounds__(kBS) void k_adj_kappa(const float4* __restrict__ Pf, const uint32_t* __restrict__ segOf,
+                                                   SpanTable st, sim::DeviceParams dp, uint32_t N,
+                                                   float* __restrict__ kappa) {
+    const uint32_t s = blockIdx.x * kBS + threadIdx.x;
+    if (s >= N) return;
+    float gate = 1.0f;
+    if (dp.pbf.xsph_gate_enable) {
+        const float3 pi = f3(Pf[s]);
+        int n = 0;
+        walk_all(s, segOf[s], st, [&](int, uint32_t k) {
+            const float3 r = sub3(pi, f3(Pf[k]));
+            if (dot3(r, r) <= dp.kernel.h2) ++n;
+            return true;
+        });
+        const int nMin = max(0, dp.pbf.xsph_n_min);
+        const int nMax = max(nMin + 1, dp.pbf.xsph_n_max);
+        gate = fminf(1.f, fmaxf(0.f, (float(n) - float(nMin)) / float(nMax - nMin)));
+    }
+    const float mor = (dp.restDensity > 0.f) ? (dp.particleMass / dp.restDensity) : 0.f;
+    kappa[s] = dp.xsph_c * gate * mor;
+}
+
+__global__ __launch_bounds__(kBS) void k_adj_xsph(const float4* __restrict__ Pf, const float4* __restrict__ Vn,
+                                                  const uint32_t* __restrict__ segOf, SpanTable st,
+                                                  sim::DeviceParams dp, uint32_t N, const float* __restrict__ kappa,
+                                                  const float4* __restrict__ gVo, float4* __restrict__ gVn,
+                                                  float4* __restrict__ gPf, double* __restrict__ partial) {
+    __shared__ double sh[kBS];
+    const uint32_t s = blockIdx.x * kBS + threadIdx.x;
+    double gcl = 0.0;
+    if (s < N) {
+        const float h2 = dp.kernel.h2, poly6 = dp.kernel.poly6;
+        const float3 pk = f3(Pf[s]), vk = f3(Vn[s]), gk = f3(gVo[s]);
+        const float kk = kappa[s];
+        float sumW = 0.f;
+        float3 accV = make_float3(0.f, 0.f, 0.f), accP = accV, dv = accV;
+        walk_all(s, segOf[s], st, [&](int, uint32_t j) {
+            const float3 r = sub3(pk, f3(Pf[j]));
+            const float r2 = dot3(r, r);
+            if (r2 > h2) return true;
+            const float hr2 = h2 - r2;
+            const float W = poly6 * hr2 * hr2 * hr2;
+            const float3 dW = mul3(r, -6.0f * poly6 * hr2 * hr2);
+            const float3 vj = f3(Vn[j]), gj = f3(gVo[j]);
+            const float kj = kappa[j];
+            sumW += W;
+            accV = add3(accV, mul3(gj, kj * W));
+            const float3 dvj = sub3(vj, vk);
+            dv = add3(dv, mul3(dvj, W));
+            accP = add3(accP, mul3(dW, dot3(sub3(mul3(gk, kk), mul3(gj, kj)), dvj)));
+            return true;
+        });
+        gVn[s] = mk4(add3(mul3(gk, 1.0f - kk * sumW), accV), 0.f);
+        gPf[s] = mk4(add3(f3(gPf[s]), accP), 0.f);
+        // d Vout_k / dc = kappa_k / c * sum_j (Vn_j - Vn_k) W_kj
+        const double dkdc = dp.xsph_c > 0.f ? (double)kk / (double)dp.xsph_c : 0.0;
+        gcl = dkdc * (double)dot3(gk, dv);
+    }
+    sh[threadIdx.x] = gcl;
+    __syncthreads();
+    for (int w = kBS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+__global__ void k_adj_sum(const double* __restrict__ partial, uint32_t n, double* __restrict__ accum) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double t = 0.0;
+    for (uint32_t b = 0; b < n; ++b) t += partial[b];
+    *accum += t;
+}
+
+__global__ __launch_bounds__(kBS) void k_adj_velocity(const uint32_t* __restrict__ idxS,
+                                                      const uint32_t* __restrict__ finalClamp,
+                                                      const float4* __restrict__ gPf, const float4* __restrict__ gVn,
+                                                      float invDt, uint32_t N, float4* __restrict__ gP,
+                                                      float4* __restrict__ gX) {
+    const uint32_t s = blockIdx.x * kBS + threadIdx.x;
+    if (s >= N) return;
+    const float3 gv = f3(gVn[s]);
+    const float3 g = add3(f3(gPf[s]), mul3(gv, invDt));
+    gP[s] = mk4(mask3(g, finalClamp[s]), 0.f);
+    gX[idxS[s]] = mk4(mul3(gv, -invDt), 0.f);
+}
+
+// a_i = s J^T gP_i with s = omega / rho0 and J the displacement-clamp Jacobian at s u_i.
+__global__ __launch_bounds__(kBS) void k_adj_delta_pre(const float4* __restrict__ P, const uint32_t* __restrict__ segOf,
+                                                       SpanTable st, sim::DeviceParams dp, uint32_t N,
+                                                       const float4* __restrict__ gP, float4* __restrict__ A) {
+    const uint32_t s = blockIdx.x * kBS + threadIdx.x;
+    if (s >= N) return;
+    const PairConsts c = make_pair_consts(dp);
+    const float4 pi = P[s];
+    float dx = 0.f, dy = 0.f, dz = 0.f;
+    walk_all(s, segOf[s], st, [&](int, uint32_t k) {
+        const float4 pj = P[k];
+        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        if (r2 <= c.h2) delta_pair(dx, dy, dz, rx, ry, rz, r2, pi.w, pj.w, c);
+        return true;
+    });
+    const float invRest = (dp.restDensity > 0.f) ? (1.0f / dp.restDensity) : 0.f;
+    const float sc = dp.pbf.enable_relax ? invRest * dp.pbf.relax_omega : invRest;
+    float3 v = make_float3(invRest * dx, invRest * dy, invRest * dz);
+    if (dp.pbf.enable_relax) v = mul3(v, dp.pbf.relax_omega);
+    float3 g = f3(gP[s]);
+    if (dp.pbf.enable_disp_clamp) {
+        const float maxDisp = dp.pbf.disp_clamp_max_h * dp.kernel.h;
+        const float len2 = dot3(v, v);
+        if (len2 > maxDisp * maxDisp) {
+            const float len = sqrtf(len2);
+            const float3 vh = mul3(v, 1.0f / len);
+            // J = maxDisp / |v| (I - vh vh^T), symmetric
+            g = mul3(sub3(g, mul3(vh, dot3(vh, g))), maxDisp / len);
+        }
+    }
+    A[s] = mk4(mul3(g, sc), 0.f);
+}
+
+__global__ __launch_bounds__(kBS) void k_adj_delta(const float4* __restrict__ P, const uint32_t* __restrict__ segOf,
+                                                   SpanTable st, sim::DeviceParams dp, uint32_t N,
+                                                   const float4* __restrict__ A, float* __restrict__ gLam,
+                                                   float4* __restrict__ gP) {
+    const uint32_t s = blockIdx.x * kBS + threadIdx.x;
+    if (s >= N) return;
+    const PairConsts c = make_pair_consts(dp);
+    const float4 pk4 = P[s];
+    const float3 pk = f3(pk4), ak = f3(A[s]);
+    float gl = 0.f;
+    float3 acc = make_float3(0.f, 0.f, 0.f);
+    walk_all(s, segOf[s], st, [&](int, uint32_t j) {
+        const float4 pj4 = P[j];
+        const float3 r = sub3(pk, f3(pj4));
+        const float r2 = dot3(r, r);
+        if (r2 > c.h2) return true;
+        const Spiky sp = spiky_of(r2, c);
+        const float3 g = mul3(r, sp.f);
+        const float3 da = sub3(ak, f3(A[j]));
+        gl += dot3(g, da);
+        // c_kj and its gradient (s_corr, reference: pbf_delta_apply.cu:77-84)
+        float scorr = 0.f, dsr = 0.f;  // dscorr = dsr * r
+        if (c.scorr_enable) {
+            const float hr2 = c.h2 - r2;
+            const float W = c.poly6 * hr2 * hr2 * hr2;
+            const float ratio = (c.w_q > 0.f) ? (W / c.w_q) : 0.0f;
+            const float raw = c.neg_scorr_k * pow_scorr(ratio, c.scorr_n, c.scorr_ni);
+            scorr = raw;
+            if (c.scorr_min < 0.f && raw < c.scorr_min) {
+                scorr = c.scorr_min;
+            } else if (c.w_q > 0.f && ratio > 0.f) {
+                // d/dr of -k (W/Wq)^n: -k n (W/Wq)^(n-1) / Wq * dW6, dW6 = -6 poly6 hr2^2 r
+                const float pn1 = powf(ratio, c.scorr_n - 1.0f);
+                dsr = c.neg_scorr_k * c.scorr_n * pn1 / c.w_q * (-6.0f * c.poly6 * hr2 * hr2);
+            }
+        }
+        const float ckj = pk4.w + pj4.w + scorr;
+        acc = add3(acc, add3(mul3(hess_apply(sp, r, da), ckj), mul3(g, dsr * dot3(r, da))));
+        return true;
+    });
+    gLam[s] = gl;
+    gP[s] = mk4(add3(f3(gP[s]), acc), 0.f);
+}
+
+// Forward lambda sums with the cap, and the adjoint coefficients of the slot.
+__global__ __launch_bounds__(kBS) void k_adj_lambda_pre(const float4* __restrict__ P,
+                                                        const uint32_t* __restrict__ segOf, SpanTable st,
+                                                        sim::DeviceParams dp, uint32_t N,
+                                                        const float* __restrict__ gLam, float4* __restrict__ coefA,
+                                                        float4* __restrict__ coefB) {
+    const uint32_t s = blockIdx.x * kBS + threadIdx.x;
+    if (s >= N) return;
+    const PairConsts c = make_pair_consts(dp);
+    const float4 pi = P[s];
+    const int cap = dp.maxNeighbors;
+    LambdaAcc a;
+    int cutRow = 0x7fffffff;
+    uint32_t cutSlot = 0xFFFFFFFFu;
+    walk_all(s, segOf[s], st, [&](int row, uint32_t k) {
+        const float4 pj = P[k];
+        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+        const float r2 = rx * rx + ry * ry + rz * rz;
+        if (r2 > c.h2) return true;
+        lambda_pair(a, rx, ry, rz, r2, c);
+        if (cap > 0 && a.n >= cap) {
+            cutRow = row;
+            cutSlot = k;
+            return false;
+        }
+        return true;
+    });
+    // lambda_finish with the branch bookkeeping (reference: pbf_lambda.cu:86-107)
+    const float gradSum2 = a.gx * a.gx + a.gy * a.gy + a.gz * a.gz;
+    const float scale = (dp.restDensity > 0.f) ? (dp.particleMass / dp.restDensity) : 0.f;
+    float den = (scale * scale) * (a.sumGrad2 + gradSum2) + dp.pbf.lambda_denom_eps;
+    if (dp.pbf.compliance > 0.f && dp.dt > 0.f) den += dp.pbf.compliance / (dp.dt * dp.dt);
+    const bool denFloored = !(den > dp.pbf.lambda_denom_eps);
+    den = fmaxf(den, dp.pbf.lambda_denom_eps);
+    float beta = 0.f, eps = 0.f;
+    const bool gated = a.n < 2 || (a.sumGrad2 + gradSum2) < 1e-12f;
+    if (!gated) {
+        const float C = (dp.restDensity > 0.f) ? (a.density / dp.restDensity - 1.f) : 0.f;
+        const float lam = -C / den;
+        const bool clamped = dp.pbf.enable_lambda_clamp && fabsf(lam) > dp.pbf.lambda_max_abs;
+        if (isfinite(lam) && !clamped) {
+            const float gl = gLam[s];
+            beta = gl * (-1.0f / den) * scale;
+            eps = denFloored ? 0.f : gl * (C / (den * den)) * scale * scale;
+        }
+    }
+    coefA[s] = make_float4(a.gx, a.gy, a.gz, beta);
+    coefB[s] = make_float4(eps, __int_as_float(cutRow), __uint_as_float(cutSlot), 0.f);
+}
+
+__global__ __launch_bounds__(kBS) void k_adj_lambda(const float4* __restrict__ P, const uint32_t* __restrict__ segOf,
+                                                    SpanTable st, sim::DeviceParams dp, uint32_t N,
+                                                    const float4* __restrict__ coefA,
+                                                    const float4* __restrict__ coefB, float4* __restrict__ gP) {
+    const uint32_t s = blockIdx.x * kBS + threadIdx.x;
+    if (s >= N) return;
+    const PairConsts c = make_pair_consts(dp);
+    const float3 pk = f3(P[s]);
+    const float4 cak = coefA[s], cbk = coefB[s];
+    const float3 Gk = f3(cak);
+    const float bk = cak.w, ek = cbk.x;
+    const int cutRowK = __float_as_int(cbk.y);
+    const uint32_t cutSlotK = __float_as_uint(cbk.z);
+    const float poly6 = dp.kernel.poly6;
+    float3 acc = make_float3(0.f, 0.f, 0.f);
+    walk_all(s, segOf[s], st, [&](int row, uint32_t j) {
+        const float3 r = sub3(pk, f3(P[j]));
+        const float r2 = dot3(r, r);
+        if (r2 > c.h2) return true;
+        const float hr2 = c.h2 - r2;
+        const float3 dW6 = mul3(r, -6.0f * poly6 * hr2 * hr2);
+        const Spiky sp = spiky_of(r2, c);
+        const float3 g = mul3(r, sp.f);
+        // j in Ncap(k): position (row, j) not past k's cut-off
+        const bool jInK = row < cutRowK || (row == cutRowK && j <= cutSlotK);
+        if (jInK && (bk != 0.f || ek != 0.f))
+            acc = add3(acc, add3(mul3(dW6, bk), mul3(hess_apply(sp, r, add3(g, Gk)), 2.0f * ek)));
+        // k in Ncap(j): k sits in j's walk at the mirrored row
+        const float4 caj = coefA[j], cbj = coefB[j];
+        if (caj.w != 0.f || cbj.x != 0.f) {
+            const int rowJ = st.rows - 1 - row;
+            const int cutRowJ = __float_as_int(cbj.y);
+            const uint32_t cutSlotJ = __float_as_uint(cbj.z);
+            if (rowJ < cutRowJ || (rowJ == cutRowJ && s <= cutSlotJ))
+                acc = add3(acc, add3(mul3(dW6, caj.w), mul3(hess_apply(sp, r, sub3(g, f3(caj))), 2.0f * cbj.x)));
+        }
+        return true;
+    });
+    gP[s] = mk4(add3(f3(gP[s]), acc), 0.f);
+}
+
+__global__ __launch_bounds__(kBS) void k_adj_predict(const uint32_t* __restrict__ idxS,
+                                                     const uint32_t* __restrict__ predClamp,
+                                                     const float4* __restrict__ gP, float dt, uint32_t N,
+                                                     float4* __restrict__ gX, float4* __restrict__ gV) {
+    const uint32_t s = blockIdx.x * kBS + threadIdx.x;
+    if (s >= N) return;
+    const uint32_t i = idxS[s];
+    const float3 g = mask3(f3(gP[s]), predClamp[i]);
+    gX[i] = mk4(add3(f3(gX[i]), g), 0.f);
+    gV[i] = mk4(mul3(g, dt), 0.f);
+}
+
+__global__ __launch_bounds__(kBS) void k_copy_vs(float4* __restrict__ dst, const float4* __restrict__ src,
+                                                 uint32_t N) {
+    const uint32_t s = blockIdx.x * kBS + threadIdx.x;
+    if (s < N) dst[s] = src[s];
+}
+
+}  // namespace
+
+void adjoint_step(const StepTape& t, const SpanTable& stIn, const sim::SimParams& p, const float4* gXin,
+                  const float4* gVin, float4* gXout, float4* gVout, double* gcAccum, AdjointScratch& w,
+                  hipStream_t s) {
+    const uint32_t N = p.numParticles;
+    if (N == 0) return;
+    const uint32_t B = ceil_div(N, kBS);
+    const sim::DeviceParams dp = sim::MakeDeviceParams(p);
+    SpanTable st = stIn;
+    st.spans = t.spans;
+    const int I = p.solverIters;
+    const float4* Pf = t.P[I > 0 ? I : 0];
+    k_adj_gather<<<B, kBS, 0, s>>>(t.idxS, gXin, gVin, w.gPf, w.gVo, N);
+    if (p.xsph_c > 0.f) {
+        k_adj_kappa<<<B, kBS, 0, s>>>(Pf, t.segOf, st, dp, N, w.kappa);
+        k_adj_xsph<<<B, kBS, 0, s>>>(Pf, t.Vs, t.segOf, st, dp, N, w.kappa, w.gVo, w.gVn, w.gPf, w.partial);
+        k_adj_sum<<<1, 1, 0, s>>>(w.partial, B, gcAccum);
+    } else {
+        k_copy_vs<<<B, kBS, 0, s>>>(w.gVn, w.gVo, N);
+    }
+    k_adj_velocity<<<B, kBS, 0, s>>>(t.idxS, t.finalClamp, w.gPf, w.gVn, 1.0f / p.dt, N, w.gP, gXout);
+    for (int it = I - 1; it >= 0; --it) {
+        const float4* Pit = t.P[it];
+        k_adj_delta_pre<<<B, kBS, 0, s>>>(Pit, t.segOf, st, dp, N, w.gP, w.a);
+        k_adj_delta<<<B, kBS, 0, s>>>(Pit, t.segOf, st, dp, N, w.a, w.gLam, w.gP);
+        k_adj_lambda_pre<<<B, kBS, 0, s>>>(Pit, t.segOf, st, dp, N, w.gLam, w.coefA, w.coefB);
+        k_adj_lambda<<<B, kBS, 0, s>>>(Pit, t.segOf, st, dp, N, w.coefA, w.coefB, w.gP);
+    }
+    k_adj_predict<<<B, kBS, 0, s>>>(t.idxS, t.predClamp, w.gP, p.dt, N, gXout, gVout);
+    check_launch("adjoint_step");
+}
+
+}  // namespace pbfx

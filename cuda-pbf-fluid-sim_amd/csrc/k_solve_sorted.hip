@@ -30,7 +30,8 @@ __device__ __forceinline__ uint32_t slot_of_thread() {
 __global__ __launch_bounds__(kBS) void k_predict_hash(const float4* __restrict__ X, const float4* __restrict__ V,
                                                       float4* __restrict__ Xn, uint32_t* __restrict__ keys,
                                                       uint32_t* __restrict__ epoch, float3 g, float dt,
-                                                      sim::GridBounds grid, uint32_t N, Scenes sc) {
+                                                      sim::GridBounds grid, uint32_t N, Scenes sc,
+                                                      uint32_t* __restrict__ clampBits) {
     const uint32_t i = blockIdx.x * kBS + threadIdx.x;
     if (epoch && i == 0) epoch[0] = epoch[0] + 1u;
     if (i >= N) return;
@@ -49,6 +50,7 @@ __global__ __launch_bounds__(kBS) void k_predict_hash(const float4* __restrict__
     const float vy = v.y + g.y * dt;
     const float vz = v.z + g.z * dt;
     float px = x.x + vx * dt, py = x.y + vy * dt, pz = x.z + vz * dt;
+    if (clampBits) clampBits[i] = clamp_bits(px, py, pz, grid);
     float ux = 0.f, uy = 0.f, uz = 0.f;
     clamp_box(px, py, pz, ux, uy, uz, grid, 0.0f);
     Xn[i] = make_float4(px, py, pz, 1.0f);
@@ -425,7 +427,7 @@ __global__ __launch_bounds__(kBS) void k_delta_sorted(const float4* __restrict__
                                                       CandMask mk, int maskMode, sim::DeviceParams dp, uint32_t N,
                                                       int last, const float4* __restrict__ X,
                                                       const uint32_t* __restrict__ idxS, float4* __restrict__ Vs,
-                                                      float invDt, Scenes sc) {
+                                                      float invDt, Scenes sc, uint32_t* __restrict__ clampBits) {
     __shared__ uint32_t sQ[kQD * kBS];
     __shared__ float4 sBuf[kBS];
     const uint32_t s = slot_of_thread();
@@ -461,7 +463,9 @@ __global__ __launch_bounds__(kBS) void k_delta_sorted(const float4* __restrict__
     }
     // post-solve boundary (phases.cpp:94) and velocity (phases.cpp:102-105)
     float ux = 0.f, uy = 0.f, uz = 0.f;
-    clamp_box(px, py, pz, ux, uy, uz, sc.params ? sc.params[sc.sceneOfSlot[s]].grid : dp.grid, 0.0f);
+    const sim::GridBounds& gb = sc.params ? sc.params[sc.sceneOfSlot[s]].grid : dp.grid;
+    if (clampBits) clampBits[s] = clamp_bits(px, py, pz, gb);
+    clamp_box(px, py, pz, ux, uy, uz, gb, 0.0f);
     Pb[s] = make_float4(px, py, pz, 1.0f);
     const float4 x = X[idxS[s]];
     Vs[s] = make_float4((px - x.x) * invDt, (py - x.y) * invDt, (pz - x.z) * invDt, 0.0f);
@@ -556,9 +560,10 @@ float mask_skin_radius2(const sim::SimParams& p) {
 }
 
 void launch_predict_hash(const float4* X, const float4* V, float4* Xn, uint32_t* keys, uint32_t* epoch,
-                         const sim::SimParams& p, const Scenes& sc, hipStream_t s) {
+                         const sim::SimParams& p, const Scenes& sc, hipStream_t s, uint32_t* clampBits) {
     const uint32_t N = p.numParticles;
-    k_predict_hash<<<blocks_for(N), kBS, 0, s>>>(X, V, Xn, keys, epoch, p.gravity, p.dt, p.grid, N, sc);
+    k_predict_hash<<<blocks_for(N), kBS, 0, s>>>(X, V, Xn, keys, epoch, p.gravity, p.dt, p.grid, N, sc,
+                                                 clampBits);
     check_launch("predict_hash");
 }
 
@@ -600,10 +605,10 @@ void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st
 void launch_delta_sorted(const float4* Pa, float4* Pb, const uint32_t* segOf, const SpanTable& st,
                          const CandMask& mk, int maskMode, const sim::DeviceParams& dp, uint32_t N, bool last,
                          const float4* X, const uint32_t* idxS, float4* Vs, float invDt, const Scenes& sc,
-                         hipStream_t s) {
+                         hipStream_t s, uint32_t* clampBits) {
     if (N == 0) return;
     k_delta_sorted<<<blocks_for(N), kBS, 0, s>>>(Pa, Pb, segOf, st, mk, maskMode, dp, N, last ? 1 : 0, X, idxS, Vs,
-                                                 invDt, sc);
+                                                 invDt, sc, clampBits);
     check_launch("delta_sorted");
 }
 

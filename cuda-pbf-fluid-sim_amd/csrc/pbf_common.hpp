@@ -69,6 +69,12 @@ __device__ __forceinline__ void clamp_box(float& px, float& py, float& pz, float
     if (pz > g.maxs.z) { pz = g.maxs.z; vz *= -e; }
 }
 
+// Axes the box clamp will move: bit a set when p_a < min_a or p_a > max_a.
+__device__ __forceinline__ uint32_t clamp_bits(float px, float py, float pz, const sim::GridBounds& g) {
+    return ((px < g.mins.x || px > g.maxs.x) ? 1u : 0u) | ((py < g.mins.y || py > g.maxs.y) ? 2u : 0u) |
+           ((pz < g.mins.z || pz > g.maxs.z) ? 4u : 0u);
+}
+
 __device__ __forceinline__ float clamp_restitution(float r) { return fminf(fmaxf(r, 0.0f), 1.0f); }
 
 // Blocks of a 1-D launch are dealt round-robin to the 8 XCDs.  This remaps the

@@ -143,7 +143,8 @@ void RefPost(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_t s
 // ------------------------------------------------------------ reordered path
 void SortedPredict(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_t s) {
     pbfx::launch_predict_hash(b.d_pos, b.d_vel, b.d_pos_pred, g.d_cellKeys,
-                              g.kind == GridKind::Hashed ? g.table.epoch : nullptr, p, g.scenes, s);
+                              g.kind == GridKind::Hashed ? g.table.epoch : nullptr, p, g.scenes, s,
+                              b.rec.predClamp);
     mark("predict_hash");
 }
 
@@ -164,7 +165,8 @@ void SortedGridBuild(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipSt
         lk = pbfx::CellLookup::Hashed;
     }
     if (lk != pbfx::CellLookup::Compact) mark("cell_lookup");
-    pbfx::launch_spans_gather(lk, b.d_pos_pred, g.d_indices_sorted, b.d_pos_sorted[0], N, g.d_cellUniqueKeys,
+    pbfx::launch_spans_gather(lk, b.d_pos_pred, g.d_indices_sorted, b.rec.P[0] ? b.rec.P[0] : b.d_pos_sorted[0], N,
+                              g.d_cellUniqueKeys,
                               g.d_cellOffsets, g.d_compactCount, g.d_cellStart, g.d_cellEnd, g.table, g.spans,
                               p.grid, g.scenes, s);
     mark("spans_gather");
@@ -184,25 +186,32 @@ void SortedSolve(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream
     pbfx::CandMask mk = g.mask;
     mk.rs2 = pbfx::mask_skin_radius2(p);
     const bool useMask = g.mask.words != nullptr && mk.rs2 > 0.f;
+    const bool rec = b.rec.P[0] != nullptr;
     int cur = 0;
     for (int it = 0; it < p.solverIters; ++it) {
-        pbfx::launch_lambda_sorted(b.d_pos_sorted[cur], g.d_segOf, g.spans, mk, useMask && it > 0, dp, N, s);
+        float4* Pa = rec ? b.rec.P[it] : b.d_pos_sorted[cur];
+        float4* Pb = rec ? b.rec.P[it + 1] : b.d_pos_sorted[cur ^ 1];
+        const bool last = it == p.solverIters - 1;
+        pbfx::launch_lambda_sorted(Pa, g.d_segOf, g.spans, mk, useMask && it > 0, dp, N, s);
         mark("lambda");
         const int maskMode = !useMask ? 0 : (it == 0 ? 1 : 2);
-        pbfx::launch_delta_sorted(b.d_pos_sorted[cur], b.d_pos_sorted[cur ^ 1], g.d_segOf, g.spans, mk, maskMode,
-                                  dp, N, it == p.solverIters - 1, b.d_pos, g.d_indices_sorted, b.d_vel_sorted, invDt,
-                                  g.scenes, s);
-        mark(it == p.solverIters - 1 ? "delta_last" : "delta");
+        pbfx::launch_delta_sorted(Pa, Pb, g.d_segOf, g.spans, mk, maskMode, dp, N, last, b.d_pos, g.d_indices_sorted,
+                                  rec ? b.rec.Vs : b.d_vel_sorted, invDt, g.scenes, s,
+                                  last ? b.rec.finalClamp : nullptr);
+        mark(last ? "delta_last" : "delta");
         cur ^= 1;
     }
 }
 
 void SortedPost(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_t s) {
     const int fin = p.solverIters > 0 ? (p.solverIters & 1) : 0;
+    // XSPH's per-pair work is light, so its scan is cheaper as the coalesced LDS
+    // stream than as the per-lane walk of the candidate mask (measured)
     pbfx::CandMask mk = g.mask;
-    mk.rs2 = pbfx::mask_skin_radius2(p);
-    const bool useMask = g.mask.words != nullptr && mk.rs2 > 0.f && p.solverIters > 0;
-    pbfx::launch_xsph_scatter(b.d_pos_sorted[fin], b.d_vel_sorted, g.d_segOf, g.spans, mk, useMask,
+    const bool useMask = false;
+    const bool rec = b.rec.P[0] != nullptr;
+    pbfx::launch_xsph_scatter(rec ? b.rec.P[p.solverIters] : b.d_pos_sorted[fin], rec ? b.rec.Vs : b.d_vel_sorted,
+                              g.d_segOf, g.spans, mk, useMask,
                               MakeDeviceParams(p), p.numParticles, g.d_indices_sorted, b.d_pos_pred, b.d_vel,
                               g.scenes, s);
     mark("xsph_scatter");
@@ -498,6 +507,7 @@ void Simulator::destroyGraphs() {
 
 void Simulator::shutdown() {
     destroyGraphs();
+    releaseAdjoint();
     releaseScenes();
     m_bufs.release();
     m_grid.release();
@@ -529,20 +539,22 @@ bool Simulator::captureGraphs() {
     return true;
 }
 
-bool Simulator::step(const SimParams& p) {
+bool Simulator::setParams(const SimParams& p) {
     if (!m_initialized) return false;
-    if (std::memcmp(&p, &m_params, sizeof(SimParams)) != 0) {
-        if (p.numParticles > m_bufs.capacity || p.grid.dim.x != m_params.grid.dim.x ||
-            p.grid.dim.y != m_params.grid.dim.y || p.grid.dim.z != m_params.grid.dim.z ||
-            p.grid.cellSize != m_params.grid.cellSize || p.kernel.h != m_params.kernel.h) {
-            set_error(PBFX_E_INVALID_ARGUMENT, "Simulator::step: structural change needs initialize()");
-            return false;
-        }
-        m_params = p;
-        m_graphDirty = true;
+    if (std::memcmp(&p, &m_params, sizeof(SimParams)) == 0) return true;
+    if (p.numParticles != m_params.numParticles || p.grid.dim.x != m_params.grid.dim.x ||
+        p.grid.dim.y != m_params.grid.dim.y || p.grid.dim.z != m_params.grid.dim.z ||
+        p.grid.cellSize != m_params.grid.cellSize || p.kernel.h != m_params.kernel.h ||
+        p.solverIters != m_params.solverIters) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "Simulator::setParams: structural change needs initialize()");
+        return false;
     }
-    return stepMany(1);
+    m_params = p;
+    m_graphDirty = true;
+    return true;
 }
+
+bool Simulator::step(const SimParams& p) { return setParams(p) && stepMany(1); }
 
 bool Simulator::stepMany(int steps) {
     if (!m_initialized) return false;
@@ -704,6 +716,87 @@ bool Simulator::densityStats(double* out5) {
                               g.spans, m_params.grid, g.scenes, s);
     return pbfx::density_stats_sorted(m_bufs.d_pos_sorted[0], m_bufs.d_vel, g.d_indices_sorted, g.d_segOf, g.spans,
                                       MakeDeviceParams(m_params), N, out5, s);
+}
+
+void Simulator::releaseAdjoint() {
+    for (auto& t : m_tapes) {
+        dfree(t.X); dfree(t.V); dfree(t.idxS); dfree(t.segOf); dfree(t.spans); dfree(t.Vs);
+        dfree(t.predClamp); dfree(t.finalClamp);
+        for (auto& q : t.P) dfree(q);
+    }
+    m_tapes.clear();
+    dfree(m_adj.gPf); dfree(m_adj.gVo); dfree(m_adj.gVn); dfree(m_adj.gP); dfree(m_adj.a); dfree(m_adj.gLam);
+    dfree(m_adj.coefA); dfree(m_adj.coefB); dfree(m_adj.kappa); dfree(m_adj.partial); dfree(m_adj.gc);
+    m_adj = pbfx::AdjointScratch{};
+}
+
+bool Simulator::adjointInit(int slots) {
+    releaseAdjoint();
+    if (!m_initialized || slots < 1 || m_opt.path != PathKind::Reordered || !m_scenes.empty() ||
+        m_params.solverIters < 1 || m_params.solverIters > pbfx::kMaxTapeIters) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "adjointInit: needs a single-scene reordered simulator, 1..8 iterations");
+        return false;
+    }
+    const uint32_t cap = m_bufs.capacity;
+    const size_t spanWords = (size_t)m_grid.spans.rows * m_grid.spans.stride;
+    m_tapes.resize(slots);
+    bool ok = true;
+    for (auto& t : m_tapes) {
+        ok = ok && dalloc(t.X, cap) && dalloc(t.V, cap) && dalloc(t.idxS, cap) && dalloc(t.segOf, cap) &&
+             dalloc(t.spans, spanWords) && dalloc(t.Vs, cap) && dalloc(t.predClamp, cap) && dalloc(t.finalClamp, cap);
+        for (int it = 0; ok && it <= m_params.solverIters; ++it) ok = dalloc(t.P[it], cap);
+    }
+    m_adj.blocks = pbfx::ceil_div(cap, 256);
+    ok = ok && dalloc(m_adj.gPf, cap) && dalloc(m_adj.gVo, cap) && dalloc(m_adj.gVn, cap) && dalloc(m_adj.gP, cap) &&
+         dalloc(m_adj.a, cap) && dalloc(m_adj.gLam, cap) && dalloc(m_adj.coefA, cap) && dalloc(m_adj.coefB, cap) &&
+         dalloc(m_adj.kappa, cap) && dalloc(m_adj.partial, m_adj.blocks) && dalloc(m_adj.gc, 1);
+    if (!ok) {
+        releaseAdjoint();
+        return false;
+    }
+    return true;
+}
+
+bool Simulator::stepRecord(int slot) {
+    if (!m_initialized || slot < 0 || slot >= (int)m_tapes.size()) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "stepRecord: no such tape slot (adjointInit first)");
+        return false;
+    }
+    pbfx::StepTape& t = m_tapes[slot];
+    const uint32_t N = m_params.numParticles;
+    const size_t b4 = sizeof(float4) * N;
+    bool ok = hipMemcpyAsync(t.X, m_bufs.d_pos, b4, hipMemcpyDeviceToDevice, m_stream) == hipSuccess &&
+              hipMemcpyAsync(t.V, m_bufs.d_vel, b4, hipMemcpyDeviceToDevice, m_stream) == hipSuccess;
+    m_bufs.rec = t;
+    runPipeline(m_stream);
+    m_bufs.rec = pbfx::StepTape{};
+    const size_t b1 = sizeof(uint32_t) * N;
+    ok = ok && hipMemcpyAsync(t.idxS, m_grid.d_indices_sorted, b1, hipMemcpyDeviceToDevice, m_stream) == hipSuccess &&
+         hipMemcpyAsync(t.segOf, m_grid.d_segOf, b1, hipMemcpyDeviceToDevice, m_stream) == hipSuccess &&
+         hipMemcpyAsync(t.spans, m_grid.spans.spans, sizeof(uint2) * m_grid.spans.rows * m_grid.spans.stride,
+                        hipMemcpyDeviceToDevice, m_stream) == hipSuccess;
+    if (!ok) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "stepRecord: tape copy failed");
+        return false;
+    }
+    m_bufs.swapPositionPingPong();
+    m_parity ^= 1;
+    ++m_frameIndex;
+    return true;
+}
+
+bool Simulator::backwardStep(int slot, const float4* gXin, const float4* gVin, float4* gXout, float4* gVout,
+                             double* gcAccum) {
+    if (!m_initialized || slot < 0 || slot >= (int)m_tapes.size()) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "backwardStep: no such tape slot");
+        return false;
+    }
+    if (gXin == gXout || gVin == gVout) {
+        set_error(PBFX_E_ALIAS, "backwardStep: outputs alias inputs");
+        return false;
+    }
+    pbfx::adjoint_step(m_tapes[slot], m_grid.spans, m_params, gXin, gVin, gXout, gVout, gcAccum, m_adj, m_stream);
+    return true;
 }
 
 bool Simulator::hashOverflowed() const {

@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "pbf_params.h"
+#include "adjoint.hpp"
 #include "sorted_path.hpp"
 
 namespace sim {
@@ -46,6 +47,10 @@ struct DeviceBuffers {
     float4* d_pos_next = nullptr;
     float4* d_pos_sorted[2] = {nullptr, nullptr};  // cell-sorted working copies
     float4* d_vel_sorted = nullptr;
+    // Recording a step for the adjoint: when rec.P[0] is set, iteration it works in
+    // rec.P[it] (so every iteration's positions and lambdas survive), velocities in
+    // rec.Vs, and the clamps write their bits.
+    pbfx::StepTape rec{};
     uint32_t capacity = 0;
 
     bool allocate(uint32_t cap);
@@ -178,6 +183,8 @@ public:
     void shutdown();
     // One step with the given parameters (reference: simulator.cpp:560-643).
     bool step(const SimParams& p);
+    // New parameters for the following steps (same particle count and grid).
+    bool setParams(const SimParams& p);
     bool stepMany(int steps);
     // Per-phase device times of one eager step, in milliseconds.
     bool profilePhases(std::vector<std::pair<std::string, float>>& out);
@@ -208,6 +215,16 @@ public:
     // Density-constraint statistics of the current state (see pbfSimDensityStats).
     bool densityStats(double* out5);
 
+    // ---- differentiable rollout (adjoint of the reordered step) ----
+    // Allocates `slots` step tapes and the reverse-pass scratch.
+    bool adjointInit(int slots);
+    // One eager forward step recorded into tape `slot`.
+    bool stepRecord(int slot);
+    // Reverse of the step in tape `slot` (device pointers, particle order); adds
+    // dL/dc into gcAccum (device double).
+    bool backwardStep(int slot, const float4* gXin, const float4* gVin, float4* gXout, float4* gVout,
+                      double* gcAccum);
+
 private:
     bool captureGraphs();
     void destroyGraphs();
@@ -216,6 +233,10 @@ private:
     bool fillLattice(const GridBounds& g, uint32_t first, uint32_t nx, uint32_t ny, uint32_t nz, float3 o,
                      float spacing, float amp, uint64_t seed);
     void releaseScenes();
+
+    std::vector<pbfx::StepTape> m_tapes;
+    pbfx::AdjointScratch m_adj{};
+    void releaseAdjoint();
 
     std::vector<pbfx::SceneParams> m_scenes;
     pbfx::SceneParams* d_sceneParams = nullptr;

@@ -80,8 +80,9 @@ struct Scenes {
 
 // P = X + (V + g dt) dt, AABB clamp, write Xn = (P, 1) and its cell key.
 // Thread 0 also advances the hash-table epoch when `epoch` is not null.
+// clampBits (optional, particle order): axes moved by the clamp, for the adjoint.
 void launch_predict_hash(const float4* X, const float4* V, float4* Xn, uint32_t* keys, uint32_t* epoch,
-                         const sim::SimParams& p, const Scenes& sc, hipStream_t s);
+                         const sim::SimParams& p, const Scenes& sc, hipStream_t s, uint32_t* clampBits = nullptr);
 
 // Insert the M occupied cells (count on device) into the table.
 void launch_hash_insert(const HashTable& t, const uint32_t* uniq, const uint32_t* off, const uint32_t* count,
@@ -103,7 +104,7 @@ void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st
 void launch_delta_sorted(const float4* Pa, float4* Pb, const uint32_t* segOf, const SpanTable& st,
                          const CandMask& mk, int maskMode, const sim::DeviceParams& dp, uint32_t N, bool last,
                          const float4* X, const uint32_t* idxS, float4* Vs, float invDt, const Scenes& sc,
-                         hipStream_t s);
+                         hipStream_t s, uint32_t* clampBits = nullptr);
 
 // No solver iterations: clamp Ps and compute Vs (the post-solve boundary and velocity phases).
 void launch_finalize_sorted(float4* Ps, const float4* X, const uint32_t* idxS, float4* Vs,

@@ -1,0 +1,202 @@
+"""Differentiable PBF rollouts: the hand-written adjoint kernels behind torch.autograd.
+
+``Rollout`` runs T steps on the GPU and keeps (X, V) checkpoints every k steps.
+The reverse pass walks the segments from the last one back. For each segment it
+restores the checkpoint, re-runs the segment's forward steps while recording them
+(sort permutation, spans, clamp bits, every iteration's positions and lambdas), and
+applies the reverse kernel of each recorded step, last first.
+``PBFRollout`` exposes X_T as a differentiable function of the initial velocity
+and of the XSPH coefficient c. Gradients reach x0 as well.
+
+``occupancy_grid`` is the loss's Gaussian splat: a G^3 grid over the box, each
+particle contributing exp(-|x_c - p|^2 / (2 sigma^2)) to the cells within 3 sigma.
+It is written in torch and differentiated by torch.
+"""
+import ctypes
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from .params import build_sim_params, f32
+from .sim import Simulator
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _as4(t):
+    """(N, 3) or (N, 4) float tensor -> contiguous (N, 4) float32 on the same device."""
+    if t.shape[1] == 4:
+        return t.float().contiguous()
+    out = torch.zeros((t.shape[0], 4), dtype=torch.float32, device=t.device)
+    out[:, :3] = t
+    return out
+
+
+class Rollout:
+    """T-step forward on the GPU, checkpointed every `checkpoint` steps, with a reverse pass."""
+
+    def __init__(self, params, steps, checkpoint=10, grid="dense", table_log2=None):
+        self.params = params
+        self.steps = int(steps)
+        self.ck = max(1, int(checkpoint))
+        tl = table_log2 or max(16, int(math.ceil(math.log2(max(params.numParticles, 1)))) + 1)
+        self.sim = Simulator(params, grid=grid, table_log2=tl, use_graph=False)
+        if _lib.lib().pbfSimAdjointInit(self.sim._h, self.ck) != 0:
+            _lib.check("pbfSimAdjointInit")
+            raise _lib.PbfError("adjoint init failed")
+        self.n = int(params.numParticles)
+        self._ckpts = []
+        self.timed = False    # when set, backward() synchronises between its phases and fills `timing`
+        self.timing = {}
+
+    def close(self):
+        self.sim.close()
+
+    def _clock(self):
+        if not self.timed:
+            return 0.0
+        self.sim.synchronize()
+        return time.perf_counter()
+
+    def _set_c(self, c):
+        if float(f32(c)) != self.params.xsph_c:
+            self.params.xsph_c = float(c)
+            if _lib.lib().pbfSimSetParams(self.sim._h, ctypes.byref(self.params)) != 0:
+                _lib.check("pbfSimSetParams")
+                raise _lib.PbfError("set params failed")
+
+    def _set_state(self, X, V):
+        L = _lib.lib()
+        torch.cuda.synchronize()
+        if L.pbfSimSetState(self.sim._h, _p(X), _p(V), self.n, 1) != 0:
+            _lib.check("pbfSimSetState")
+            raise _lib.PbfError("set state failed")
+
+    def _get_state(self):
+        X = torch.empty((self.n, 4), dtype=torch.float32, device="cuda")
+        V = torch.empty_like(X)
+        if _lib.lib().pbfSimGetState(self.sim._h, _p(X), _p(V), 1) != 0:
+            _lib.check("pbfSimGetState")
+            raise _lib.PbfError("get state failed")
+        return X, V
+
+    def forward(self, x0, v0, c):
+        """Run the rollout; returns (X_T, V_T) as (N, 4) float32 CUDA tensors."""
+        self._set_c(c)
+        X, V = _as4(x0), _as4(v0)
+        self._set_state(X, V)
+        self._ckpts = []
+        L = _lib.lib()
+        done = 0
+        while done < self.steps:
+            self._ckpts.append(self._get_state())
+            k = min(self.ck, self.steps - done)
+            if L.pbfSimStep(self.sim._h, k) != 0:
+                _lib.check("pbfSimStep")
+                raise _lib.PbfError("step failed")
+            done += k
+        return self._get_state()
+
+    def backward(self, gX, gV=None):
+        """dL/dX_0, dL/dV_0 (N, 4 float32) and dL/dc (float) from dL/dX_T and dL/dV_T."""
+        L = _lib.lib()
+        gX = _as4(gX).clone()
+        gV = torch.zeros_like(gX) if gV is None else _as4(gV).clone()
+        gX[:, 3] = 0
+        gV[:, 3] = 0
+        oX, oV = torch.empty_like(gX), torch.empty_like(gV)
+        gc = torch.zeros(1, dtype=torch.float64, device="cuda")
+        starts = list(range(0, self.steps, self.ck))
+        t_rec = t_rev = 0.0
+        for seg in reversed(range(len(starts))):
+            k = min(self.ck, self.steps - starts[seg])
+            self._set_state(*self._ckpts[seg])
+            t0 = self._clock()
+            for t in range(k):
+                if L.pbfSimStepRecord(self.sim._h, t) != 0:
+                    _lib.check("pbfSimStepRecord")
+                    raise _lib.PbfError("record failed")
+            t1 = self._clock()
+            for t in reversed(range(k)):
+                if L.pbfSimBackwardStep(self.sim._h, t, _p(gX), _p(gV), _p(oX), _p(oV), _p(gc)) != 0:
+                    _lib.check("pbfSimBackwardStep")
+                    raise _lib.PbfError("backward failed")
+                gX, oX = oX, gX
+                gV, oV = oV, gV
+            t2 = self._clock()
+            t_rec += t1 - t0
+            t_rev += t2 - t1
+        if self.timed:
+            self.timing = {"record_s": t_rec, "reverse_s": t_rev}
+        self.sim.synchronize()
+        _lib.check("Rollout.backward")
+        return gX, gV, float(gc.item())
+
+
+class PBFRollout(torch.autograd.Function):
+    """X_T = rollout(x0, v0, c) on the GPU; backward through the adjoint kernels."""
+
+    @staticmethod
+    def forward(ctx, v0, c, x0, roll):
+        XT, VT = roll.forward(x0, v0, float(c))
+        ctx.roll = roll
+        return XT[:, :3].clone()
+
+    @staticmethod
+    def backward(ctx, gXT):
+        gX0, gV0, gc = ctx.roll.backward(gXT.contiguous())
+        return gV0[:, :3].clone(), torch.tensor(gc, dtype=torch.float32, device=gXT.device), gX0[:, :3].clone(), None
+
+
+def occupancy_grid(pos, box_min, box_max, cells=32, sigma=0.0125, reach=3.0):
+    """Gaussian splat of (N, 3) positions on a cells^3 grid spanning the box (cell centres)."""
+    dev, dt = pos.device, pos.dtype
+    lo = torch.tensor(box_min, dtype=dt, device=dev)
+    hi = torch.tensor(box_max, dtype=dt, device=dev)
+    size = (hi - lo) / cells
+    R = int(math.ceil(reach * sigma / float(size.min())))
+    base = torch.floor((pos - lo) / size).long()
+    offs = torch.stack(torch.meshgrid(*[torch.arange(-R, R + 1, device=dev)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    idx = base[:, None, :] + offs[None, :, :]
+    valid = ((idx >= 0) & (idx < cells)).all(-1)
+    centres = lo + (idx.to(dt) + 0.5) * size
+    d2 = ((centres - pos[:, None, :]) ** 2).sum(-1)
+    w = torch.exp(-d2 / (2.0 * sigma * sigma)) * valid.to(dt)
+    flat = (idx[..., 2].clamp(0, cells - 1) * cells + idx[..., 1].clamp(0, cells - 1)) * cells + \
+        idx[..., 0].clamp(0, cells - 1)
+    grid = torch.zeros(cells ** 3, dtype=dt, device=dev)
+    grid.index_add_(0, flat.reshape(-1), w.reshape(-1))
+    return grid.reshape(cells, cells, cells)
+
+
+def differentiable_scene(nx=64, ny=32, nz=32, seed=7, box=(1.0, 1.0, 1.0)):
+    """The differentiable rollout configuration: an nx*ny*nz lattice (spacing 2r, r = 0.005 m,
+    seeded jitter) in the corner of the box, dt = 1/120, 2 iterations."""
+    from .params import Scene
+    r = 0.005
+    return Scene(name="diff_rollout", lattices=[(nx, ny, nz, (r, r, r))], box_min=(0.0, 0.0, 0.0),
+                 box_max=box, radius=r, grid="dense", seed=seed, steps=60)
+
+
+def initial_state(scene):
+    """Seeded lattice positions (the simulator's own seeding) as a host array."""
+    sim = Simulator.from_scene(scene, use_graph=False)
+    X, V = sim.state()
+    sim.close()
+    return X, V
+
+
+def seeded_velocity(n, seed, scale):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    v = np.zeros((n, 4), np.float32)
+    v[:, :3] = rng.normal(0.0, scale, (n, 3)).astype(np.float32)
+    return v
+
+
+__all__ = ["Rollout", "PBFRollout", "occupancy_grid", "differentiable_scene", "initial_state", "seeded_velocity",
+           "build_sim_params"]

@@ -1,0 +1,94 @@
+"""An independent dense float64 PBF step in torch, used as the reference gradient.
+
+Same semantics as the reference step (predict and clamp, grid candidates from the
+predicted positions, I iterations of lambda / delta, clamp, velocity, XSPH):
+- the candidate pairs are the 27-cell neighbourhood of the grid built on the
+  predicted positions;
+- the in-range test r^2 <= h^2 is redone at each stage.
+It uses all in-range pairs, so it is only valid while no particle reaches the
+neighbour cap (asserted). It is written with dense N x N tensors, so it is meant
+for small N. torch.autograd differentiates it; its own gradients are checked
+against float64 central differences in tests.
+"""
+import torch
+
+
+def pbf_step(X, V, sp, c, box_min, box_max):
+    dt_ = X.dtype
+    h = float(sp.kernel.h)
+    h2 = h * h
+    poly6, spiky = float(sp.kernel.poly6), float(sp.kernel.spiky)
+    m, rho0, dt = float(sp.particleMass), float(sp.restDensity), float(sp.dt)
+    pbf = sp.pbf
+    g = torch.tensor(list(sp.gravity), dtype=dt_)
+    lo = torch.tensor(box_min, dtype=dt_)
+    hi = torch.tensor(box_max, dtype=dt_)
+    n = X.shape[0]
+    eye = torch.eye(n, dtype=torch.bool)
+
+    P = X + (V + g * dt) * dt
+    P = torch.minimum(torch.maximum(P, lo), hi)
+    cs = float(sp.grid.cellSize)
+    dims = torch.tensor(list(sp.grid.dim))
+    cell = torch.minimum(torch.maximum(torch.floor((P.detach() - lo) / cs).long(), torch.zeros(3, dtype=torch.long)),
+                         dims - 1)
+    cand = ((cell[:, None, :] - cell[None, :, :]).abs() <= 1).all(-1) & ~eye
+
+    def pairs(P):
+        d = P[:, None, :] - P[None, :, :]
+        r2 = (d * d).sum(-1)
+        nb = cand & (r2 <= h2)
+        r = torch.sqrt(torch.where(nb, r2, torch.ones_like(r2)))
+        return d, r2, r, nb
+
+    dq = pbf.scorr_dq_h * h
+    wq = poly6 * (h2 - dq * dq) ** 3
+    maxDisp = pbf.disp_clamp_max_h * h
+    for _ in range(sp.solverIters):
+        d, r2, r, nb = pairs(P)
+        nbf = nb.to(dt_)
+        assert int(nb.sum(1).max()) < sp.maxNeighbors, "neighbour cap reached: reference not valid"
+        W = poly6 * torch.clamp(h2 - r2, min=0.0) ** 3 * nbf
+        rho = m * W.sum(1)
+        C = rho / rho0 - 1.0
+        rs = torch.clamp(r, min=pbf.grad_r_eps)
+        coeff = (-3.0 * spiky * (h - r) ** 2 / rs) * nbf
+        gr = coeff[..., None] * d
+        S1 = (gr * gr).sum(-1).sum(1)
+        G = gr.sum(1)
+        scale = m / rho0
+        den = scale * scale * (S1 + (G * G).sum(1)) + pbf.lambda_denom_eps
+        if pbf.compliance > 0 and dt > 0:
+            den = den + pbf.compliance / (dt * dt)
+        den = torch.clamp(den, min=pbf.lambda_denom_eps)
+        lam = -C / den
+        gate = (nb.sum(1) >= 2) & ((S1 + (G * G).sum(1)) >= 1e-12)
+        lam = torch.where(gate, lam, torch.zeros_like(lam))
+        if pbf.enable_lambda_clamp:
+            lam = torch.clamp(lam, -pbf.lambda_max_abs, pbf.lambda_max_abs)
+        scorr = torch.zeros_like(W)
+        if pbf.scorr_enable:
+            scorr = -pbf.scorr_k * (W / wq) ** pbf.scorr_n
+            if pbf.scorr_min < 0:
+                scorr = torch.clamp(scorr, min=pbf.scorr_min)
+        lij = (lam[:, None] + lam[None, :] + scorr) * nbf
+        D = (lij[..., None] * gr).sum(1) / rho0
+        if pbf.enable_relax:
+            D = D * pbf.relax_omega
+        if pbf.enable_disp_clamp:
+            ln = torch.sqrt((D * D).sum(1, keepdim=True))
+            D = torch.where(ln > maxDisp, D * (maxDisp / ln), D)
+        P = P + D
+    P = torch.minimum(torch.maximum(P, lo), hi)
+    Vn = (P - X) / dt
+    d, r2, r, nb = pairs(P)
+    W = poly6 * torch.clamp(h2 - r2, min=0.0) ** 3 * nb.to(dt_)
+    dv = (W[..., None] * (Vn[None, :, :] - Vn[:, None, :])).sum(1)
+    Vout = Vn + c * (m / rho0) * dv
+    return P, Vout
+
+
+def rollout(X, V, sp, c, steps, box_min, box_max):
+    for _ in range(steps):
+        X, V = pbf_step(X, V, sp, c, box_min, box_max)
+    return X, V
