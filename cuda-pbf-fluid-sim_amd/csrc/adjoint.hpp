@@ -29,6 +29,11 @@ struct StepTape {
     float4* Vs = nullptr;           // sorted velocities after the velocity phase
     uint32_t* predClamp = nullptr;  // particle order: bit a set when the predict clamp moved axis a
     uint32_t* finalClamp = nullptr; // sorted: bit a set when the post-solve clamp moved axis a
+    // written by the forward kernels while recording (sorted order, per iteration):
+    float4* lamA[kMaxTapeIters] = {};   // (sum of spiky gradients, b0) of the lambda pass
+    uint2* lamB[kMaxTapeIters] = {};    // (bits of e0, slot that reached the neighbour cap or ~0)
+    float4* dispU[kMaxTapeIters] = {};  // scaled correction before the displacement clamp
+    float* xsphGm = nullptr;            // XSPH gate * m / rho0
 };
 
 // Scratch of the reverse pass (sorted order unless noted).
@@ -38,12 +43,9 @@ struct AdjointScratch {
     float4* gVn = nullptr;    // dL/d(velocity before XSPH)
     float4* gP = nullptr;     // dL/d(positions of the current iteration)
     float4* a = nullptr;      // delta adjoint vectors
-    float* gLam = nullptr;    // dL/d(lambda)
-    float4* coefA = nullptr;  // lambda adjoint: (sum of spiky gradients, beta)
-    float4* coefB = nullptr;  // lambda adjoint: (epsilon, cap cut-off row, cut-off slot, -)
-    float* kappa = nullptr;   // XSPH coefficient per slot (c * gate * m / rho0)
+    float4* lamQ = nullptr;   // lambda adjoint: (sum of spiky gradients, beta = gLam b0)
+    uint2* lamE = nullptr;    // lambda adjoint: (bits of eps = gLam e0, cap cut-off slot)
     double* partial = nullptr;  // per-block partial sums of dL/dc
-    double* gc = nullptr;       // dL/dc of the step
     uint32_t blocks = 0;
 };
 

@@ -12,23 +12,32 @@
 // with g_ij = f(r) r_ij the spiky gradient, f(r) = -3 spiky (h - r)^2 / max(r, eps).
 //
 // Reverse, in the order below; each kernel gathers over its own neighbours, so
-// every particle's gradient is written by one lane (no atomics, deterministic):
+// every particle's gradient is written by one lane (no atomics, deterministic).
+// The forward pass, while recording, also kept what the reverse pass would
+// otherwise walk the neighbours again for: the lambda coefficients (sum of spiky
+// gradients, b0, e0, the slot where the neighbour cap cut the walk), the scaled
+// correction before the displacement clamp, and the XSPH gate.
 //   gather   dL/dPf, dL/dVout to sorted order
 //   xsph     dL/dVn_k = gVo_k (1 - kappa_k sum_j W_kj) + sum_j kappa_j W_kj gVo_j
 //            dL/dPf_k += sum_j ((kappa_k gVo_k - kappa_j gVo_j) . (Vn_j - Vn_k)) dW6_kj
 //            dL/dc    += gVo_k . (gate_k m/rho0 sum_j (Vn_j - Vn_k) W_kj)
 //   velocity dL/dP_I = finalMask (gPf + gVn / dt);   dL/dX = -gVn / dt
+//            and a of the last iteration (below)
 //   per iteration, last first:
-//     delta_pre  a_i = (omega/rho0) J_i^T gP_i   (J: Jacobian of the displacement clamp)
-//     delta      dL/dlambda_k = sum_j g_kj . (a_k - a_j)
-//                dL/dP_k += sum_j T_kj (a_k - a_j),  T_kj = c_kj H_kj + g_kj (x) dscorr_kj
-//                (H the Hessian of the spiky kernel; T is even in r_kj, hence symmetric)
-//     lambda_pre beta_i = gLam_i (-1/den_i) (m/rho0),  eps_i = gLam_i (C_i/den_i^2) (m/rho0)^2
-//                (zero where the forward gated, clamped or floored lambda)
-//     lambda     dL/dP_k += sum_{j in Ncap(k)} [beta_k dW6_kj + 2 eps_k H_kj (g_kj + G_k)]
-//                         + sum_{j : k in Ncap(j)} [beta_j dW6_kj + 2 eps_j H_kj (g_kj - G_j)]
-//                (Ncap: the first maxNeighbors in-range neighbours in walk order)
+//     delta   dL/dlambda_k = sum_j g_kj . (a_k - a_j)
+//             dL/dP_k += sum_j T_kj (a_k - a_j),  T_kj = c_kj H_kj + g_kj (x) dscorr_kj
+//             (H the Hessian of the spiky kernel; T is even in r_kj, hence symmetric)
+//             then beta_k = gLam_k b0_k, eps_k = gLam_k e0_k
+//     lambda  dL/dP_k += sum_{j in Ncap(k)} [beta_k dW6_kj + 2 eps_k H_kj (g_kj + G_k)]
+//                      + sum_{j : k in Ncap(j)} [beta_j dW6_kj + 2 eps_j H_kj (g_kj - G_j)]
+//             (Ncap: the first maxNeighbors in-range neighbours in walk order), then
+//             a_i = (omega/rho0) J_i^T gP_i of the iteration before (J: Jacobian of the
+//             displacement clamp at the recorded scaled correction)
 //   predict  dL/dX += predMask gP0,  dL/dV = dt predMask gP0
+// Walk order is ascending slot order: rows go dz-major then dy, every row is one
+// x-line of cells, and slots are sorted by the row-major cell key, so the lines
+// (and the slots) of later rows come later.  "j in Ncap(k)" is therefore
+// j <= cut(k), with cut(k) the slot at which k's walk reached the cap (~0 if never).
 #include "adjoint.hpp"
 #include "pbf_common.hpp"
 #include "pbf_hip.h"
@@ -96,41 +105,37 @@ __global__ __launch_bounds__(kBS) void k_adj_gather(const uint32_t* __restrict__
     gVo[s] = gV[i];
 }
 
-// XSPH coefficient per slot: c * gate(n) * m / rho0 (reference: xsph.cu:80-93).
-__global__ __launch_bounds__(kBS) void k_adj_kappa(const float4* __restrict__ Pf, const uint32_t* __restrict__ segOf,
-                                                   SpanTable st, sim::DeviceParams dp, uint32_t N,
-                                                   float* __restrict__ kappa) {
-    const uint32_t s = blockIdx.x * kBS + threadIdx.x;
-    if (s >= N) return;
-    float gate = 1.0f;
-    if (dp.pbf.xsph_gate_enable) {
-        const float3 pi = f3(Pf[s]);
-        int n = 0;
-        walk_all(s, segOf[s], st, [&](int, uint32_t k) {
-            const float3 r = sub3(pi, f3(Pf[k]));
-            if (dot3(r, r) <= dp.kernel.h2) ++n;
-            return true;
-        });
-        const int nMin = max(0, dp.pbf.xsph_n_min);
-        const int nMax = max(nMin + 1, dp.pbf.xsph_n_max);
-        gate = fminf(1.f, fmaxf(0.f, (float(n) - float(nMin)) / float(nMax - nMin)));
+// a = s J^T g for the recorded scaled correction e (s = omega / rho0, or 1 / rho0
+// without relaxation).  J = maxDisp / |e| (I - e^ e^T) where the clamp was active.
+__device__ __forceinline__ float3 disp_adjoint(float3 g, const float4& e4, const sim::DeviceParams& dp) {
+    const float invRest = (dp.restDensity > 0.f) ? (1.0f / dp.restDensity) : 0.f;
+    const float sc = dp.pbf.enable_relax ? invRest * dp.pbf.relax_omega : invRest;
+    if (dp.pbf.enable_disp_clamp) {
+        const float3 e = f3(e4);
+        const float maxDisp = dp.pbf.disp_clamp_max_h * dp.kernel.h;
+        const float len2 = dot3(e, e);
+        if (len2 > maxDisp * maxDisp) {
+            const float len = sqrtf(len2);
+            const float3 eh = mul3(e, 1.0f / len);
+            g = mul3(sub3(g, mul3(eh, dot3(eh, g))), maxDisp / (len + 1e-20f));
+        }
     }
-    const float mor = (dp.restDensity > 0.f) ? (dp.particleMass / dp.restDensity) : 0.f;
-    kappa[s] = dp.xsph_c * gate * mor;
+    return mul3(g, sc);
 }
 
 __global__ __launch_bounds__(kBS) void k_adj_xsph(const float4* __restrict__ Pf, const float4* __restrict__ Vn,
                                                   const uint32_t* __restrict__ segOf, SpanTable st,
-                                                  sim::DeviceParams dp, uint32_t N, const float* __restrict__ kappa,
+                                                  sim::DeviceParams dp, uint32_t N, const float* __restrict__ gm,
                                                   const float4* __restrict__ gVo, float4* __restrict__ gVn,
                                                   float4* __restrict__ gPf, double* __restrict__ partial) {
     __shared__ double sh[kBS];
     const uint32_t s = blockIdx.x * kBS + threadIdx.x;
     double gcl = 0.0;
     if (s < N) {
-        const float h2 = dp.kernel.h2, poly6 = dp.kernel.poly6;
+        const float h2 = dp.kernel.h2, poly6 = dp.kernel.poly6, c = dp.xsph_c;
         const float3 pk = f3(Pf[s]), vk = f3(Vn[s]), gk = f3(gVo[s]);
-        const float kk = kappa[s];
+        const float gmk = gm[s];
+        const float kk = c * gmk;
         float sumW = 0.f;
         float3 accV = make_float3(0.f, 0.f, 0.f), accP = accV, dv = accV;
         walk_all(s, segOf[s], st, [&](int, uint32_t j) {
@@ -139,21 +144,22 @@ __global__ __launch_bounds__(kBS) void k_adj_xsph(const float4* __restrict__ Pf,
             if (r2 > h2) return true;
             const float hr2 = h2 - r2;
             const float W = poly6 * hr2 * hr2 * hr2;
-            const float3 dW = mul3(r, -6.0f * poly6 * hr2 * hr2);
-            const float3 vj = f3(Vn[j]), gj = f3(gVo[j]);
-            const float kj = kappa[j];
-            sumW += W;
-            accV = add3(accV, mul3(gj, kj * W));
+            const float3 vj = f3(Vn[j]);
             const float3 dvj = sub3(vj, vk);
             dv = add3(dv, mul3(dvj, W));
-            accP = add3(accP, mul3(dW, dot3(sub3(mul3(gk, kk), mul3(gj, kj)), dvj)));
+            if (c != 0.f) {
+                const float3 dW = mul3(r, -6.0f * poly6 * hr2 * hr2);
+                const float3 gj = f3(gVo[j]);
+                const float kj = c * gm[j];
+                sumW += W;
+                accV = add3(accV, mul3(gj, kj * W));
+                accP = add3(accP, mul3(dW, dot3(sub3(mul3(gk, kk), mul3(gj, kj)), dvj)));
+            }
             return true;
         });
         gVn[s] = mk4(add3(mul3(gk, 1.0f - kk * sumW), accV), 0.f);
         gPf[s] = mk4(add3(f3(gPf[s]), accP), 0.f);
-        // d Vout_k / dc = kappa_k / c * sum_j (Vn_j - Vn_k) W_kj
-        const double dkdc = dp.xsph_c > 0.f ? (double)kk / (double)dp.xsph_c : 0.0;
-        gcl = dkdc * (double)dot3(gk, dv);
+        gcl = (double)gmk * (double)dot3(gk, dv);
     }
     sh[threadIdx.x] = gcl;
     __syncthreads();
@@ -174,54 +180,23 @@ __global__ void k_adj_sum(const double* __restrict__ partial, uint32_t n, double
 __global__ __launch_bounds__(kBS) void k_adj_velocity(const uint32_t* __restrict__ idxS,
                                                       const uint32_t* __restrict__ finalClamp,
                                                       const float4* __restrict__ gPf, const float4* __restrict__ gVn,
-                                                      float invDt, uint32_t N, float4* __restrict__ gP,
+                                                      const float4* __restrict__ U, sim::DeviceParams dp, float invDt,
+                                                      uint32_t N, float4* __restrict__ gP, float4* __restrict__ A,
                                                       float4* __restrict__ gX) {
     const uint32_t s = blockIdx.x * kBS + threadIdx.x;
     if (s >= N) return;
     const float3 gv = f3(gVn[s]);
-    const float3 g = add3(f3(gPf[s]), mul3(gv, invDt));
-    gP[s] = mk4(mask3(g, finalClamp[s]), 0.f);
+    const float3 g = mask3(add3(f3(gPf[s]), mul3(gv, invDt)), finalClamp[s]);
+    gP[s] = mk4(g, 0.f);
+    A[s] = mk4(disp_adjoint(g, U[s], dp), 0.f);
     gX[idxS[s]] = mk4(mul3(gv, -invDt), 0.f);
-}
-
-// a_i = s J^T gP_i with s = omega / rho0 and J the displacement-clamp Jacobian at s u_i.
-__global__ __launch_bounds__(kBS) void k_adj_delta_pre(const float4* __restrict__ P, const uint32_t* __restrict__ segOf,
-                                                       SpanTable st, sim::DeviceParams dp, uint32_t N,
-                                                       const float4* __restrict__ gP, float4* __restrict__ A) {
-    const uint32_t s = blockIdx.x * kBS + threadIdx.x;
-    if (s >= N) return;
-    const PairConsts c = make_pair_consts(dp);
-    const float4 pi = P[s];
-    float dx = 0.f, dy = 0.f, dz = 0.f;
-    walk_all(s, segOf[s], st, [&](int, uint32_t k) {
-        const float4 pj = P[k];
-        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        if (r2 <= c.h2) delta_pair(dx, dy, dz, rx, ry, rz, r2, pi.w, pj.w, c);
-        return true;
-    });
-    const float invRest = (dp.restDensity > 0.f) ? (1.0f / dp.restDensity) : 0.f;
-    const float sc = dp.pbf.enable_relax ? invRest * dp.pbf.relax_omega : invRest;
-    float3 v = make_float3(invRest * dx, invRest * dy, invRest * dz);
-    if (dp.pbf.enable_relax) v = mul3(v, dp.pbf.relax_omega);
-    float3 g = f3(gP[s]);
-    if (dp.pbf.enable_disp_clamp) {
-        const float maxDisp = dp.pbf.disp_clamp_max_h * dp.kernel.h;
-        const float len2 = dot3(v, v);
-        if (len2 > maxDisp * maxDisp) {
-            const float len = sqrtf(len2);
-            const float3 vh = mul3(v, 1.0f / len);
-            // J = maxDisp / |v| (I - vh vh^T), symmetric
-            g = mul3(sub3(g, mul3(vh, dot3(vh, g))), maxDisp / len);
-        }
-    }
-    A[s] = mk4(mul3(g, sc), 0.f);
 }
 
 __global__ __launch_bounds__(kBS) void k_adj_delta(const float4* __restrict__ P, const uint32_t* __restrict__ segOf,
                                                    SpanTable st, sim::DeviceParams dp, uint32_t N,
-                                                   const float4* __restrict__ A, float* __restrict__ gLam,
-                                                   float4* __restrict__ gP) {
+                                                   const float4* __restrict__ A, const float4* __restrict__ lamA,
+                                                   const uint2* __restrict__ lamB, float4* __restrict__ gP,
+                                                   float4* __restrict__ lamQ, uint2* __restrict__ lamE) {
     const uint32_t s = blockIdx.x * kBS + threadIdx.x;
     if (s >= N) return;
     const PairConsts c = make_pair_consts(dp);
@@ -258,99 +233,53 @@ __global__ __launch_bounds__(kBS) void k_adj_delta(const float4* __restrict__ P,
         acc = add3(acc, add3(mul3(hess_apply(sp, r, da), ckj), mul3(g, dsr * dot3(r, da))));
         return true;
     });
-    gLam[s] = gl;
     gP[s] = mk4(add3(f3(gP[s]), acc), 0.f);
+    const float4 la = lamA[s];
+    const uint2 lb = lamB[s];
+    lamQ[s] = make_float4(la.x, la.y, la.z, gl * la.w);
+    lamE[s] = make_uint2(__float_as_uint(gl * __uint_as_float(lb.x)), lb.y);
 }
 
-// Forward lambda sums with the cap, and the adjoint coefficients of the slot.
-__global__ __launch_bounds__(kBS) void k_adj_lambda_pre(const float4* __restrict__ P,
-                                                        const uint32_t* __restrict__ segOf, SpanTable st,
-                                                        sim::DeviceParams dp, uint32_t N,
-                                                        const float* __restrict__ gLam, float4* __restrict__ coefA,
-                                                        float4* __restrict__ coefB) {
-    const uint32_t s = blockIdx.x * kBS + threadIdx.x;
-    if (s >= N) return;
-    const PairConsts c = make_pair_consts(dp);
-    const float4 pi = P[s];
-    const int cap = dp.maxNeighbors;
-    LambdaAcc a;
-    int cutRow = 0x7fffffff;
-    uint32_t cutSlot = 0xFFFFFFFFu;
-    walk_all(s, segOf[s], st, [&](int row, uint32_t k) {
-        const float4 pj = P[k];
-        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        if (r2 > c.h2) return true;
-        lambda_pair(a, rx, ry, rz, r2, c);
-        if (cap > 0 && a.n >= cap) {
-            cutRow = row;
-            cutSlot = k;
-            return false;
-        }
-        return true;
-    });
-    // lambda_finish with the branch bookkeeping (reference: pbf_lambda.cu:86-107)
-    const float gradSum2 = a.gx * a.gx + a.gy * a.gy + a.gz * a.gz;
-    const float scale = (dp.restDensity > 0.f) ? (dp.particleMass / dp.restDensity) : 0.f;
-    float den = (scale * scale) * (a.sumGrad2 + gradSum2) + dp.pbf.lambda_denom_eps;
-    if (dp.pbf.compliance > 0.f && dp.dt > 0.f) den += dp.pbf.compliance / (dp.dt * dp.dt);
-    const bool denFloored = !(den > dp.pbf.lambda_denom_eps);
-    den = fmaxf(den, dp.pbf.lambda_denom_eps);
-    float beta = 0.f, eps = 0.f;
-    const bool gated = a.n < 2 || (a.sumGrad2 + gradSum2) < 1e-12f;
-    if (!gated) {
-        const float C = (dp.restDensity > 0.f) ? (a.density / dp.restDensity - 1.f) : 0.f;
-        const float lam = -C / den;
-        const bool clamped = dp.pbf.enable_lambda_clamp && fabsf(lam) > dp.pbf.lambda_max_abs;
-        if (isfinite(lam) && !clamped) {
-            const float gl = gLam[s];
-            beta = gl * (-1.0f / den) * scale;
-            eps = denFloored ? 0.f : gl * (C / (den * den)) * scale * scale;
-        }
-    }
-    coefA[s] = make_float4(a.gx, a.gy, a.gz, beta);
-    coefB[s] = make_float4(eps, __int_as_float(cutRow), __uint_as_float(cutSlot), 0.f);
-}
-
+// Lambda terms of dL/dP; then a of the iteration before (U = its recorded scaled
+// correction), or nothing after the first iteration (U == nullptr).
 __global__ __launch_bounds__(kBS) void k_adj_lambda(const float4* __restrict__ P, const uint32_t* __restrict__ segOf,
                                                     SpanTable st, sim::DeviceParams dp, uint32_t N,
-                                                    const float4* __restrict__ coefA,
-                                                    const float4* __restrict__ coefB, float4* __restrict__ gP) {
+                                                    const float4* __restrict__ lamQ, const uint2* __restrict__ lamE,
+                                                    const float4* __restrict__ U, float4* __restrict__ gP,
+                                                    float4* __restrict__ A) {
     const uint32_t s = blockIdx.x * kBS + threadIdx.x;
     if (s >= N) return;
     const PairConsts c = make_pair_consts(dp);
     const float3 pk = f3(P[s]);
-    const float4 cak = coefA[s], cbk = coefB[s];
-    const float3 Gk = f3(cak);
-    const float bk = cak.w, ek = cbk.x;
-    const int cutRowK = __float_as_int(cbk.y);
-    const uint32_t cutSlotK = __float_as_uint(cbk.z);
+    const float4 qk = lamQ[s];
+    const uint2 ek = lamE[s];
+    const float3 Gk = f3(qk);
+    const float bk = qk.w, epk = __uint_as_float(ek.x);
+    const uint32_t cutK = ek.y;
     const float poly6 = dp.kernel.poly6;
+    const bool own = bk != 0.f || epk != 0.f;
     float3 acc = make_float3(0.f, 0.f, 0.f);
-    walk_all(s, segOf[s], st, [&](int row, uint32_t j) {
+    walk_all(s, segOf[s], st, [&](int, uint32_t j) {
         const float3 r = sub3(pk, f3(P[j]));
         const float r2 = dot3(r, r);
         if (r2 > c.h2) return true;
+        const float4 qj = lamQ[j];
+        const uint2 ej = lamE[j];
+        const float epj = __uint_as_float(ej.x);
+        const bool inK = own && j <= cutK;                       // j in Ncap(k)
+        const bool inJ = (qj.w != 0.f || epj != 0.f) && s <= ej.y;  // k in Ncap(j)
+        if (!inK && !inJ) return true;
         const float hr2 = c.h2 - r2;
         const float3 dW6 = mul3(r, -6.0f * poly6 * hr2 * hr2);
         const Spiky sp = spiky_of(r2, c);
         const float3 g = mul3(r, sp.f);
-        // j in Ncap(k): position (row, j) not past k's cut-off
-        const bool jInK = row < cutRowK || (row == cutRowK && j <= cutSlotK);
-        if (jInK && (bk != 0.f || ek != 0.f))
-            acc = add3(acc, add3(mul3(dW6, bk), mul3(hess_apply(sp, r, add3(g, Gk)), 2.0f * ek)));
-        // k in Ncap(j): k sits in j's walk at the mirrored row
-        const float4 caj = coefA[j], cbj = coefB[j];
-        if (caj.w != 0.f || cbj.x != 0.f) {
-            const int rowJ = st.rows - 1 - row;
-            const int cutRowJ = __float_as_int(cbj.y);
-            const uint32_t cutSlotJ = __float_as_uint(cbj.z);
-            if (rowJ < cutRowJ || (rowJ == cutRowJ && s <= cutSlotJ))
-                acc = add3(acc, add3(mul3(dW6, caj.w), mul3(hess_apply(sp, r, sub3(g, f3(caj))), 2.0f * cbj.x)));
-        }
+        if (inK) acc = add3(acc, add3(mul3(dW6, bk), mul3(hess_apply(sp, r, add3(g, Gk)), 2.0f * epk)));
+        if (inJ) acc = add3(acc, add3(mul3(dW6, qj.w), mul3(hess_apply(sp, r, sub3(g, f3(qj))), 2.0f * epj)));
         return true;
     });
-    gP[s] = mk4(add3(f3(gP[s]), acc), 0.f);
+    const float3 g = add3(f3(gP[s]), acc);
+    gP[s] = mk4(g, 0.f);
+    if (U) A[s] = mk4(disp_adjoint(g, U[s], dp), 0.f);
 }
 
 __global__ __launch_bounds__(kBS) void k_adj_predict(const uint32_t* __restrict__ idxS,
@@ -365,12 +294,6 @@ __global__ __launch_bounds__(kBS) void k_adj_predict(const uint32_t* __restrict_
     gV[i] = mk4(mul3(g, dt), 0.f);
 }
 
-__global__ __launch_bounds__(kBS) void k_copy_vs(float4* __restrict__ dst, const float4* __restrict__ src,
-                                                 uint32_t N) {
-    const uint32_t s = blockIdx.x * kBS + threadIdx.x;
-    if (s < N) dst[s] = src[s];
-}
-
 }  // namespace
 
 void adjoint_step(const StepTape& t, const SpanTable& stIn, const sim::SimParams& p, const float4* gXin,
@@ -378,27 +301,30 @@ void adjoint_step(const StepTape& t, const SpanTable& stIn, const sim::SimParams
                   hipStream_t s) {
     const uint32_t N = p.numParticles;
     if (N == 0) return;
+    const int I = p.solverIters;
+    if (I < 1 || I > kMaxTapeIters) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "adjoint_step: 1..8 solver iterations");
+        return;
+    }
     const uint32_t B = ceil_div(N, kBS);
+    if (B > w.blocks) {
+        set_error(PBFX_E_CAPACITY, "adjoint_step: scratch smaller than the particle count");
+        return;
+    }
     const sim::DeviceParams dp = sim::MakeDeviceParams(p);
     SpanTable st = stIn;
     st.spans = t.spans;
-    const int I = p.solverIters;
-    const float4* Pf = t.P[I > 0 ? I : 0];
+    const float4* Pf = t.P[I];
     k_adj_gather<<<B, kBS, 0, s>>>(t.idxS, gXin, gVin, w.gPf, w.gVo, N);
-    if (p.xsph_c > 0.f) {
-        k_adj_kappa<<<B, kBS, 0, s>>>(Pf, t.segOf, st, dp, N, w.kappa);
-        k_adj_xsph<<<B, kBS, 0, s>>>(Pf, t.Vs, t.segOf, st, dp, N, w.kappa, w.gVo, w.gVn, w.gPf, w.partial);
-        k_adj_sum<<<1, 1, 0, s>>>(w.partial, B, gcAccum);
-    } else {
-        k_copy_vs<<<B, kBS, 0, s>>>(w.gVn, w.gVo, N);
-    }
-    k_adj_velocity<<<B, kBS, 0, s>>>(t.idxS, t.finalClamp, w.gPf, w.gVn, 1.0f / p.dt, N, w.gP, gXout);
+    k_adj_xsph<<<B, kBS, 0, s>>>(Pf, t.Vs, t.segOf, st, dp, N, t.xsphGm, w.gVo, w.gVn, w.gPf, w.partial);
+    k_adj_sum<<<1, 1, 0, s>>>(w.partial, B, gcAccum);
+    k_adj_velocity<<<B, kBS, 0, s>>>(t.idxS, t.finalClamp, w.gPf, w.gVn, t.dispU[I - 1], dp, 1.0f / p.dt, N, w.gP,
+                                     w.a, gXout);
     for (int it = I - 1; it >= 0; --it) {
         const float4* Pit = t.P[it];
-        k_adj_delta_pre<<<B, kBS, 0, s>>>(Pit, t.segOf, st, dp, N, w.gP, w.a);
-        k_adj_delta<<<B, kBS, 0, s>>>(Pit, t.segOf, st, dp, N, w.a, w.gLam, w.gP);
-        k_adj_lambda_pre<<<B, kBS, 0, s>>>(Pit, t.segOf, st, dp, N, w.gLam, w.coefA, w.coefB);
-        k_adj_lambda<<<B, kBS, 0, s>>>(Pit, t.segOf, st, dp, N, w.coefA, w.coefB, w.gP);
+        k_adj_delta<<<B, kBS, 0, s>>>(Pit, t.segOf, st, dp, N, w.a, t.lamA[it], t.lamB[it], w.gP, w.lamQ, w.lamE);
+        k_adj_lambda<<<B, kBS, 0, s>>>(Pit, t.segOf, st, dp, N, w.lamQ, w.lamE, it > 0 ? t.dispU[it - 1] : nullptr,
+                                       w.gP, w.a);
     }
     k_adj_predict<<<B, kBS, 0, s>>>(t.idxS, t.predClamp, w.gP, p.dt, N, gXout, gVout);
     check_launch("adjoint_step");

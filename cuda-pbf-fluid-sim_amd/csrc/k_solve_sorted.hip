@@ -393,7 +393,8 @@ __device__ __forceinline__ void mask_walk(bool valid, uint32_t s, uint32_t m, co
 
 // Lambda of a sorted slot: the full walk (first iteration) or the mask walk.
 __global__ __launch_bounds__(kBS) void k_lambda_sorted(float4* Ps, const uint32_t* __restrict__ segOf, SpanTable st,
-                                                       CandMask mk, int useMask, sim::DeviceParams dp, uint32_t N) {
+                                                       CandMask mk, int useMask, sim::DeviceParams dp, uint32_t N,
+                                                       float4* __restrict__ recA, uint2* __restrict__ recB) {
     __shared__ uint32_t sQ[kQD * kBS];
     __shared__ float4 sBuf[kBS];
     const uint32_t s = slot_of_thread();
@@ -403,6 +404,8 @@ __global__ __launch_bounds__(kBS) void k_lambda_sorted(float4* Ps, const uint32_
     const uint32_t m = segOf[ss];
     const PairConsts c = make_pair_consts(dp);
     LambdaAcc a;
+    uint32_t cut = 0xFFFFFFFFu;  // recording: slot of the neighbour that reached the cap
+    const int cap = dp.maxNeighbors;
     auto dist2 = [&](const float4& pj) {
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         return rx * rx + ry * ry + rz * rz;
@@ -411,13 +414,21 @@ __global__ __launch_bounds__(kBS) void k_lambda_sorted(float4* Ps, const uint32_
         const float4 pj = Ps[k];
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         lambda_pair(a, rx, ry, rz, rx * rx + ry * ry + rz * rz, c);
+        if (recB && a.n == cap) cut = k;
     };
     if (useMask)
         mask_walk(valid, s, m, st, mk, Ps, sQ + threadIdx.x, dp.maxNeighbors, c.h2, dist2, apply);
     else
         neighbor_walk<false>(valid, s, m, st, mk, Ps, sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x,
                              dp.maxNeighbors, c.h2, dist2, apply);
-    if (valid) reinterpret_cast<float*>(Ps)[4ull * s + 3] = lambda_finish(a, dp);
+    if (!valid) return;
+    reinterpret_cast<float*>(Ps)[4ull * s + 3] = lambda_finish(a, dp);
+    if (recA) {
+        float b0, e0;
+        lambda_adjoint_coefs(a, dp, b0, e0);
+        recA[s] = make_float4(a.gx, a.gy, a.gz, b0);
+        recB[s] = make_uint2(__float_as_uint(e0), cut);
+    }
 }
 
 // Delta of a sorted slot.  maskMode: 0 full walk, 1 full walk that builds the
@@ -427,7 +438,8 @@ __global__ __launch_bounds__(kBS) void k_delta_sorted(const float4* __restrict__
                                                       CandMask mk, int maskMode, sim::DeviceParams dp, uint32_t N,
                                                       int last, const float4* __restrict__ X,
                                                       const uint32_t* __restrict__ idxS, float4* __restrict__ Vs,
-                                                      float invDt, Scenes sc, uint32_t* __restrict__ clampBits) {
+                                                      float invDt, Scenes sc, uint32_t* __restrict__ clampBits,
+                                                      float4* __restrict__ recU) {
     __shared__ uint32_t sQ[kQD * kBS];
     __shared__ float4 sBuf[kBS];
     const uint32_t s = slot_of_thread();
@@ -454,7 +466,9 @@ __global__ __launch_bounds__(kBS) void k_delta_sorted(const float4* __restrict__
     else
         neighbor_walk<false>(valid, s, m, st, mk, Pa, buf, sQ + threadIdx.x, 0, c.h2, dist2, apply);
     if (!valid) return;
-    const float3 d = delta_finish(dx, dy, dz, dp);
+    const float3 e = delta_scaled(dx, dy, dz, dp);
+    if (recU) recU[s] = make_float4(e.x, e.y, e.z, 0.f);
+    const float3 d = delta_clamp(e, dp);
     // reference: pbf_delta_apply.cu:232-233 (P += delta)
     float px = pi.x + d.x, py = pi.y + d.y, pz = pi.z + d.z;
     if (!last) {
@@ -489,7 +503,7 @@ __global__ __launch_bounds__(kBS) void k_xsph_scatter(const float4* __restrict__
                                                       const uint32_t* __restrict__ segOf, SpanTable st,
                                                       CandMask mk, int useMask, sim::DeviceParams dp, uint32_t N,
                                                       const uint32_t* __restrict__ idxS, float4* __restrict__ Xn,
-                                                      float4* __restrict__ V, Scenes sc) {
+                                                      float4* __restrict__ V, Scenes sc, float* __restrict__ recGm) {
     __shared__ uint32_t sQ[kQD * kBS];
     __shared__ float4 sBuf[kBS];
     const uint32_t s = slot_of_thread();
@@ -507,7 +521,8 @@ __global__ __launch_bounds__(kBS) void k_xsph_scatter(const float4* __restrict__
         grid = sp.grid;
     }
     // the walk is a wave-level operation: every lane enters it when any lane smooths
-    if (__any(xc > 0.f)) {
+    const bool walk = xc > 0.f || recGm != nullptr;
+    if (__any(walk)) {
         const uint32_t m = segOf[ss];
         const float h2 = dp.kernel.h2, poly6 = dp.kernel.poly6;
         float sx = 0.f, sy = 0.f, sz = 0.f;
@@ -528,10 +543,12 @@ __global__ __launch_bounds__(kBS) void k_xsph_scatter(const float4* __restrict__
             ++n;
         };
         if (useMask)
-            mask_walk(valid && xc > 0.f, s, m, st, mk, Ps, sQ + threadIdx.x, 0, h2, dist2, apply);
+            mask_walk(valid && walk, s, m, st, mk, Ps, sQ + threadIdx.x, 0, h2, dist2, apply);
         else
-            neighbor_walk<false>(valid && xc > 0.f, s, m, st, mk, Ps, sBuf + (threadIdx.x & ~63u),
+            neighbor_walk<false>(valid && walk, s, m, st, mk, Ps, sBuf + (threadIdx.x & ~63u),
                                  sQ + threadIdx.x, 0, h2, dist2, apply);
+        if (recGm && valid)
+            recGm[s] = xsph_gate(n, dp) * ((dp.restDensity > 0.f) ? (dp.particleMass / dp.restDensity) : 0.f);
         if (xc > 0.f) {
             const float3 v = xsph_finish(vi.x, vi.y, vi.z, sx, sy, sz, n, dp, xc);
             vx = v.x; vy = v.y; vz = v.z;
@@ -596,19 +613,24 @@ void launch_spans_gather(CellLookup lookup, const float4* Xn, const uint32_t* id
 }
 
 void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st, const CandMask& mk,
-                          bool useMask, const sim::DeviceParams& dp, uint32_t N, hipStream_t s) {
+                          bool useMask, const sim::DeviceParams& dp, uint32_t N, hipStream_t s, float4* recA,
+                          uint2* recB) {
     if (N == 0) return;
-    k_lambda_sorted<<<blocks_for(N), kBS, 0, s>>>(Ps, segOf, st, mk, useMask ? 1 : 0, dp, N);
+    if ((recA == nullptr) != (recB == nullptr)) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "launch_lambda_sorted: recA and recB go together");
+        return;
+    }
+    k_lambda_sorted<<<blocks_for(N), kBS, 0, s>>>(Ps, segOf, st, mk, useMask ? 1 : 0, dp, N, recA, recB);
     check_launch("lambda_sorted");
 }
 
 void launch_delta_sorted(const float4* Pa, float4* Pb, const uint32_t* segOf, const SpanTable& st,
                          const CandMask& mk, int maskMode, const sim::DeviceParams& dp, uint32_t N, bool last,
                          const float4* X, const uint32_t* idxS, float4* Vs, float invDt, const Scenes& sc,
-                         hipStream_t s, uint32_t* clampBits) {
+                         hipStream_t s, uint32_t* clampBits, float4* recU) {
     if (N == 0) return;
     k_delta_sorted<<<blocks_for(N), kBS, 0, s>>>(Pa, Pb, segOf, st, mk, maskMode, dp, N, last ? 1 : 0, X, idxS, Vs,
-                                                 invDt, sc, clampBits);
+                                                 invDt, sc, clampBits, recU);
     check_launch("delta_sorted");
 }
 
@@ -622,9 +644,11 @@ void launch_finalize_sorted(float4* Ps, const float4* X, const uint32_t* idxS, f
 
 void launch_xsph_scatter(const float4* Ps, const float4* Vs, const uint32_t* segOf, const SpanTable& st,
                          const CandMask& mk, bool useMask, const sim::DeviceParams& dp, uint32_t N,
-                         const uint32_t* idxS, float4* Xn, float4* V, const Scenes& sc, hipStream_t s) {
+                         const uint32_t* idxS, float4* Xn, float4* V, const Scenes& sc, hipStream_t s,
+                         float* recGm) {
     if (N == 0) return;
-    k_xsph_scatter<<<blocks_for(N), kBS, 0, s>>>(Ps, Vs, segOf, st, mk, useMask ? 1 : 0, dp, N, idxS, Xn, V, sc);
+    k_xsph_scatter<<<blocks_for(N), kBS, 0, s>>>(Ps, Vs, segOf, st, mk, useMask ? 1 : 0, dp, N, idxS, Xn, V, sc,
+                                                 recGm);
     check_launch("xsph_scatter");
 }
 

@@ -107,6 +107,31 @@ __device__ __forceinline__ float lambda_finish(const LambdaAcc& a, const sim::De
     return lam;
 }
 
+// Coefficients of the reverse pass at the branch lambda_finish took: with
+// lambda = -C / den, dlambda = b0 dC' + e0 dden' where C' = sum W (so dC = scale dC')
+// and den' = sum |g|^2 + |sum g|^2 (den = scale^2 den' + const):
+//   b0 = -scale / den,  e0 = C scale^2 / den^2.
+// Both are zero where lambda is gated, non-finite or clamped; e0 is zero where den
+// was floored at lambda_denom_eps.
+__device__ __forceinline__ void lambda_adjoint_coefs(const LambdaAcc& a, const sim::DeviceParams& dp, float& b0,
+                                                     float& e0) {
+    b0 = 0.f;
+    e0 = 0.f;
+    const float gradSum2 = a.gx * a.gx + a.gy * a.gy + a.gz * a.gz;
+    const float scale = (dp.restDensity > 0.f) ? (dp.particleMass / dp.restDensity) : 0.f;
+    float den = (scale * scale) * (a.sumGrad2 + gradSum2) + dp.pbf.lambda_denom_eps;
+    if (dp.pbf.compliance > 0.f && dp.dt > 0.f) den += dp.pbf.compliance / (dp.dt * dp.dt);
+    const bool floored = !(den > dp.pbf.lambda_denom_eps);
+    den = fmaxf(den, dp.pbf.lambda_denom_eps);
+    if (a.n < 2 || (a.sumGrad2 + gradSum2) < 1e-12f) return;
+    const float C = (dp.restDensity > 0.f) ? (a.density / dp.restDensity - 1.f) : 0.f;
+    const float lam = -C / den;
+    if (!isfinite(lam)) return;
+    if (dp.pbf.enable_lambda_clamp && fabsf(lam) > dp.pbf.lambda_max_abs) return;
+    b0 = -scale / den;
+    e0 = floored ? 0.f : C * (scale * scale) / (den * den);
+}
+
 // One in-range neighbour of the position-correction sum.
 __device__ __forceinline__ void delta_pair(float& dx, float& dy, float& dz, float rx, float ry, float rz, float r2,
                                            float lam_i, float lam_j, const PairConsts& c) {
@@ -129,8 +154,9 @@ __device__ __forceinline__ void delta_pair(float& dx, float& dy, float& dz, floa
     dz += lij * az;
 }
 
-// Scale, relax and clamp of the summed correction (reference: pbf_delta_apply.cu:93-110).
-__device__ __forceinline__ float3 delta_finish(float dx, float dy, float dz, const sim::DeviceParams& dp) {
+// Scale and relax of the summed correction, before the clamp
+// (reference: pbf_delta_apply.cu:93-100).
+__device__ __forceinline__ float3 delta_scaled(float dx, float dy, float dz, const sim::DeviceParams& dp) {
     const float invRest = (dp.restDensity > 0.f) ? (1.0f / dp.restDensity) : 0.f;
     float ex = invRest * dx, ey = invRest * dy, ez = invRest * dz;
     if (dp.pbf.enable_relax) {
@@ -138,24 +164,32 @@ __device__ __forceinline__ float3 delta_finish(float dx, float dy, float dz, con
         ey *= dp.pbf.relax_omega;
         ez *= dp.pbf.relax_omega;
     }
-    if (dp.pbf.enable_disp_clamp) {
-        const float maxDisp = dp.pbf.disp_clamp_max_h * dp.kernel.h;
-        const float maxDisp2 = maxDisp * maxDisp;
-        const float len2 = ex * ex + ey * ey + ez * ez;
-        if (len2 > maxDisp2) {
-            const float len = sqrtf(len2);
-            const float s = maxDisp / (len + 1e-20f);
-            ex *= s;
-            ey *= s;
-            ez *= s;
-        }
-    }
     return make_float3(ex, ey, ez);
 }
 
-// XSPH update of v_i from the summed (v_j - v_i) W (reference: xsph.cu:80-93).
-__device__ __forceinline__ float3 xsph_finish(float vx, float vy, float vz, float sx, float sy, float sz, int n,
-                                              const sim::DeviceParams& dp, float xsph_c) {
+// Displacement clamp of a scaled correction (reference: pbf_delta_apply.cu:101-110).
+__device__ __forceinline__ float3 delta_clamp(float3 e, const sim::DeviceParams& dp) {
+    if (dp.pbf.enable_disp_clamp) {
+        const float maxDisp = dp.pbf.disp_clamp_max_h * dp.kernel.h;
+        const float maxDisp2 = maxDisp * maxDisp;
+        const float len2 = e.x * e.x + e.y * e.y + e.z * e.z;
+        if (len2 > maxDisp2) {
+            const float len = sqrtf(len2);
+            const float s = maxDisp / (len + 1e-20f);
+            e.x *= s;
+            e.y *= s;
+            e.z *= s;
+        }
+    }
+    return e;
+}
+
+__device__ __forceinline__ float3 delta_finish(float dx, float dy, float dz, const sim::DeviceParams& dp) {
+    return delta_clamp(delta_scaled(dx, dy, dz, dp), dp);
+}
+
+// XSPH neighbour-count gate (reference: xsph.cu:80-88).
+__device__ __forceinline__ float xsph_gate(int n, const sim::DeviceParams& dp) {
     float gate = 1.0f;
     if (dp.pbf.xsph_gate_enable) {
         const int nMin = max(0, dp.pbf.xsph_n_min);
@@ -163,6 +197,13 @@ __device__ __forceinline__ float3 xsph_finish(float vx, float vy, float vz, floa
         const float t = (float(n) - float(nMin)) / float(nMax - nMin);
         gate = fminf(1.f, fmaxf(0.f, t));
     }
+    return gate;
+}
+
+// XSPH update of v_i from the summed (v_j - v_i) W (reference: xsph.cu:80-93).
+__device__ __forceinline__ float3 xsph_finish(float vx, float vy, float vz, float sx, float sy, float sz, int n,
+                                              const sim::DeviceParams& dp, float xsph_c) {
+    const float gate = xsph_gate(n, dp);
     const float mor = (dp.restDensity > 0.f) ? (dp.particleMass / dp.restDensity) : 0.f;
     const float cg = xsph_c * gate;
     return make_float3(vx + cg * mor * sx, vy + cg * mor * sy, vz + cg * mor * sz);

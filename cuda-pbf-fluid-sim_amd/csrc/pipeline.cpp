@@ -192,12 +192,13 @@ void SortedSolve(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream
         float4* Pa = rec ? b.rec.P[it] : b.d_pos_sorted[cur];
         float4* Pb = rec ? b.rec.P[it + 1] : b.d_pos_sorted[cur ^ 1];
         const bool last = it == p.solverIters - 1;
-        pbfx::launch_lambda_sorted(Pa, g.d_segOf, g.spans, mk, useMask && it > 0, dp, N, s);
+        pbfx::launch_lambda_sorted(Pa, g.d_segOf, g.spans, mk, useMask && it > 0, dp, N, s,
+                                   rec ? b.rec.lamA[it] : nullptr, rec ? b.rec.lamB[it] : nullptr);
         mark("lambda");
         const int maskMode = !useMask ? 0 : (it == 0 ? 1 : 2);
         pbfx::launch_delta_sorted(Pa, Pb, g.d_segOf, g.spans, mk, maskMode, dp, N, last, b.d_pos, g.d_indices_sorted,
                                   rec ? b.rec.Vs : b.d_vel_sorted, invDt, g.scenes, s,
-                                  last ? b.rec.finalClamp : nullptr);
+                                  last ? b.rec.finalClamp : nullptr, rec ? b.rec.dispU[it] : nullptr);
         mark(last ? "delta_last" : "delta");
         cur ^= 1;
     }
@@ -213,7 +214,7 @@ void SortedPost(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_
     pbfx::launch_xsph_scatter(rec ? b.rec.P[p.solverIters] : b.d_pos_sorted[fin], rec ? b.rec.Vs : b.d_vel_sorted,
                               g.d_segOf, g.spans, mk, useMask,
                               MakeDeviceParams(p), p.numParticles, g.d_indices_sorted, b.d_pos_pred, b.d_vel,
-                              g.scenes, s);
+                              g.scenes, s, rec ? b.rec.xsphGm : nullptr);
     mark("xsph_scatter");
 }
 
@@ -721,12 +722,15 @@ bool Simulator::densityStats(double* out5) {
 void Simulator::releaseAdjoint() {
     for (auto& t : m_tapes) {
         dfree(t.X); dfree(t.V); dfree(t.idxS); dfree(t.segOf); dfree(t.spans); dfree(t.Vs);
-        dfree(t.predClamp); dfree(t.finalClamp);
+        dfree(t.predClamp); dfree(t.finalClamp); dfree(t.xsphGm);
         for (auto& q : t.P) dfree(q);
+        for (auto& q : t.lamA) dfree(q);
+        for (auto& q : t.lamB) dfree(q);
+        for (auto& q : t.dispU) dfree(q);
     }
     m_tapes.clear();
-    dfree(m_adj.gPf); dfree(m_adj.gVo); dfree(m_adj.gVn); dfree(m_adj.gP); dfree(m_adj.a); dfree(m_adj.gLam);
-    dfree(m_adj.coefA); dfree(m_adj.coefB); dfree(m_adj.kappa); dfree(m_adj.partial); dfree(m_adj.gc);
+    dfree(m_adj.gPf); dfree(m_adj.gVo); dfree(m_adj.gVn); dfree(m_adj.gP); dfree(m_adj.a);
+    dfree(m_adj.lamQ); dfree(m_adj.lamE); dfree(m_adj.partial);
     m_adj = pbfx::AdjointScratch{};
 }
 
@@ -743,13 +747,16 @@ bool Simulator::adjointInit(int slots) {
     bool ok = true;
     for (auto& t : m_tapes) {
         ok = ok && dalloc(t.X, cap) && dalloc(t.V, cap) && dalloc(t.idxS, cap) && dalloc(t.segOf, cap) &&
-             dalloc(t.spans, spanWords) && dalloc(t.Vs, cap) && dalloc(t.predClamp, cap) && dalloc(t.finalClamp, cap);
+             dalloc(t.spans, spanWords) && dalloc(t.Vs, cap) && dalloc(t.predClamp, cap) &&
+             dalloc(t.finalClamp, cap) && dalloc(t.xsphGm, cap);
         for (int it = 0; ok && it <= m_params.solverIters; ++it) ok = dalloc(t.P[it], cap);
+        for (int it = 0; ok && it < m_params.solverIters; ++it)
+            ok = dalloc(t.lamA[it], cap) && dalloc(t.lamB[it], cap) && dalloc(t.dispU[it], cap);
     }
     m_adj.blocks = pbfx::ceil_div(cap, 256);
     ok = ok && dalloc(m_adj.gPf, cap) && dalloc(m_adj.gVo, cap) && dalloc(m_adj.gVn, cap) && dalloc(m_adj.gP, cap) &&
-         dalloc(m_adj.a, cap) && dalloc(m_adj.gLam, cap) && dalloc(m_adj.coefA, cap) && dalloc(m_adj.coefB, cap) &&
-         dalloc(m_adj.kappa, cap) && dalloc(m_adj.partial, m_adj.blocks) && dalloc(m_adj.gc, 1);
+         dalloc(m_adj.a, cap) && dalloc(m_adj.lamQ, cap) && dalloc(m_adj.lamE, cap) &&
+         dalloc(m_adj.partial, m_adj.blocks);
     if (!ok) {
         releaseAdjoint();
         return false;

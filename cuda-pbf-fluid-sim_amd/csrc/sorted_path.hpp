@@ -95,25 +95,32 @@ void launch_spans_gather(CellLookup lookup, const float4* Xn, const uint32_t* id
                          const SpanTable& st, const sim::GridBounds& grid, const Scenes& sc, hipStream_t s);
 
 // lambda of every sorted slot, stored in Ps[s].w (full walk, or the candidate mask's walk).
+// Recording (non-null recA/recB): recA[s] = (sum of spiky gradients, b0) and recB[s] =
+// (bits of e0, slot of the neighbour that reached the cap or ~0), the reverse pass's
+// coefficients (lambda_adjoint_coefs).
 void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st, const CandMask& mk,
-                          bool useMask, const sim::DeviceParams& dp, uint32_t N, hipStream_t s);
+                          bool useMask, const sim::DeviceParams& dp, uint32_t N, hipStream_t s,
+                          float4* recA = nullptr, uint2* recB = nullptr);
 
 // Pb[s] = Pa[s] + delta(s).  On the last iteration also clamps to the box and
 // writes Vs[s] = (Pb[s] - X[idxS[s]]) * invDt.  maskMode 0: full walk; 1: full
-// walk that also builds the candidate mask; 2: walk of the mask.
+// walk that also builds the candidate mask; 2: walk of the mask.  Recording: clampBits[s]
+// (last iteration) and recU[s] = the scaled correction before the displacement clamp.
 void launch_delta_sorted(const float4* Pa, float4* Pb, const uint32_t* segOf, const SpanTable& st,
                          const CandMask& mk, int maskMode, const sim::DeviceParams& dp, uint32_t N, bool last,
                          const float4* X, const uint32_t* idxS, float4* Vs, float invDt, const Scenes& sc,
-                         hipStream_t s, uint32_t* clampBits = nullptr);
+                         hipStream_t s, uint32_t* clampBits = nullptr, float4* recU = nullptr);
 
 // No solver iterations: clamp Ps and compute Vs (the post-solve boundary and velocity phases).
 void launch_finalize_sorted(float4* Ps, const float4* X, const uint32_t* idxS, float4* Vs,
                             const sim::GridBounds& grid, float invDt, uint32_t N, const Scenes& sc, hipStream_t s);
 
 // XSPH (Jacobi over Vs, skipped when xsph_c <= 0), boundary with restitution,
-// scatter Xn[idxS[s]] = (P, 1) and V[idxS[s]] = (v, 0).
+// scatter Xn[idxS[s]] = (P, 1) and V[idxS[s]] = (v, 0).  Recording (non-null recGm):
+// recGm[s] = gate * m / rho0, walked also when xsph_c is 0.
 void launch_xsph_scatter(const float4* Ps, const float4* Vs, const uint32_t* segOf, const SpanTable& st,
                          const CandMask& mk, bool useMask, const sim::DeviceParams& dp, uint32_t N,
-                         const uint32_t* idxS, float4* Xn, float4* V, const Scenes& sc, hipStream_t s);
+                         const uint32_t* idxS, float4* Xn, float4* V, const Scenes& sc, hipStream_t s,
+                         float* recGm = nullptr);
 
 }  // namespace pbfx

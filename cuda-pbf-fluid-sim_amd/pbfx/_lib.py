@@ -80,6 +80,10 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with cuda-pbf-fluid-sim_amd/build.py "
                                "(or __graft_entry__.build())")
+        # torch ships its own HIP runtime under the same soname; loading torch first makes the
+        # library bind to that one, so the process holds a single HIP runtime (device pointers,
+        # streams and the null-stream ordering are then shared with torch).
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

@@ -9,14 +9,34 @@ from pbfx import params as P
 from torch_pbf import rollout
 
 
-def small_case(n=8, steps=3, seed=3):
-    scene = P.small_block((n, n, n))
+def small_case(n=8, steps=3, **kw):
+    scene = P.small_block((n, n, n), **kw)
     sp = P.build_sim_params(scene)
     return scene, sp, steps
 
 
-def test_reference_forward_matches_oracle(oracle):
-    scene, sp, steps = small_case()
+# the default block, a neighbour cap that cuts most walks, the XSPH gate, and c = 0
+VARIANTS = {"default": {}, "cap4": {"max_neighbors": 4},
+            "xsph_gate": {"pbf": {"xsph_gate_enable": 1, "xsph_n_min": 2, "xsph_n_max": 7}, "xsph_c": 0.05},
+            "xsph_off": {"xsph_c": 0.0}}
+
+
+def count_capped(oracle, scene, sp):
+    """Particles whose first-step lambda walk reaches the cap (float64 count on predicted positions)."""
+    X = P.seed_scene_reference(scene, oracle.seed_lattice)
+    h2 = float(sp.kernel.h) ** 2
+    p = X[:, :3].astype(np.float64) + np.array(list(sp.gravity)) * sp.dt * sp.dt
+    d2 = ((p[:, None, :] - p[None, :, :]) ** 2).sum(-1)
+    n = (d2 <= h2).sum(1) - 1
+    return int((n >= sp.maxNeighbors).sum())
+
+
+@pytest.mark.parametrize("variant", sorted(VARIANTS))
+def test_reference_forward_matches_oracle(oracle, variant):
+    # 2 steps: float32 and float64 trajectories part ~10x per step later on
+    scene, sp, steps = small_case(steps=2, **VARIANTS[variant])
+    if variant == "cap4":
+        assert count_capped(oracle, scene, sp) > 100
     X = P.seed_scene_reference(scene, oracle.seed_lattice)
     rng = np.random.default_rng(1)
     V = np.zeros_like(X)
@@ -29,8 +49,9 @@ def test_reference_forward_matches_oracle(oracle):
     assert np.abs(tV.numpy() - eV[:, :3]).max() < 1e-4 * h / sp.dt
 
 
-def test_reference_gradient_matches_finite_differences(oracle):
-    scene, sp, steps = small_case(n=6, steps=2)
+@pytest.mark.parametrize("variant", sorted(VARIANTS))
+def test_reference_gradient_matches_finite_differences(oracle, variant):
+    scene, sp, steps = small_case(n=6, steps=2, **VARIANTS[variant])
     X = P.seed_scene_reference(scene, oracle.seed_lattice)
     rng = np.random.default_rng(4)
     x0 = torch.tensor(X[:, :3], dtype=torch.float64)

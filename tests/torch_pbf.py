@@ -4,11 +4,13 @@ Same semantics as the reference step (predict and clamp, grid candidates from th
 predicted positions, I iterations of lambda / delta, clamp, velocity, XSPH):
 - the candidate pairs are the 27-cell neighbourhood of the grid built on the
   predicted positions;
-- the in-range test r^2 <= h^2 is redone at each stage.
-It uses all in-range pairs, so it is only valid while no particle reaches the
-neighbour cap (asserted). It is written with dense N x N tensors, so it is meant
-for small N. torch.autograd differentiates it; its own gradients are checked
-against float64 central differences in tests.
+- the in-range test r^2 <= h^2 is redone at each stage;
+- lambda takes the first maxNeighbors in-range neighbours in walk order, which is
+  the order of the stable sort by row-major cell key (rows of the 27-cell walk are
+  x-lines of cells visited in increasing key order).
+It is written with dense N x N tensors, so it is meant for small N.
+torch.autograd differentiates it; its own gradients are checked against float64
+central differences in tests.
 """
 import torch
 
@@ -33,6 +35,18 @@ def pbf_step(X, V, sp, c, box_min, box_max):
     cell = torch.minimum(torch.maximum(torch.floor((P.detach() - lo) / cs).long(), torch.zeros(3, dtype=torch.long)),
                          dims - 1)
     cand = ((cell[:, None, :] - cell[None, :, :]).abs() <= 1).all(-1) & ~eye
+    key = (cell[:, 2] * dims[1] + cell[:, 1]) * dims[0] + cell[:, 0]
+    order = torch.argsort(key * n + torch.arange(n), stable=True)  # slot -> particle
+    rank = torch.empty_like(order)
+    rank[order] = torch.arange(n)                                     # particle -> slot
+    cap = int(sp.maxNeighbors)
+
+    def capped(nb):
+        if cap <= 0:
+            return nb
+        s_nb = nb[:, order]
+        keep = s_nb & (s_nb.long().cumsum(1) <= cap)
+        return keep[:, rank]
 
     def pairs(P):
         d = P[:, None, :] - P[None, :, :]
@@ -47,22 +61,24 @@ def pbf_step(X, V, sp, c, box_min, box_max):
     for _ in range(sp.solverIters):
         d, r2, r, nb = pairs(P)
         nbf = nb.to(dt_)
-        assert int(nb.sum(1).max()) < sp.maxNeighbors, "neighbour cap reached: reference not valid"
+        nc = capped(nb)
+        ncf = nc.to(dt_)
         W = poly6 * torch.clamp(h2 - r2, min=0.0) ** 3 * nbf
-        rho = m * W.sum(1)
+        rho = m * (W * ncf).sum(1)
         C = rho / rho0 - 1.0
         rs = torch.clamp(r, min=pbf.grad_r_eps)
         coeff = (-3.0 * spiky * (h - r) ** 2 / rs) * nbf
         gr = coeff[..., None] * d
-        S1 = (gr * gr).sum(-1).sum(1)
-        G = gr.sum(1)
+        grc = gr * ncf[..., None]
+        S1 = (grc * grc).sum(-1).sum(1)
+        G = grc.sum(1)
         scale = m / rho0
         den = scale * scale * (S1 + (G * G).sum(1)) + pbf.lambda_denom_eps
         if pbf.compliance > 0 and dt > 0:
             den = den + pbf.compliance / (dt * dt)
         den = torch.clamp(den, min=pbf.lambda_denom_eps)
         lam = -C / den
-        gate = (nb.sum(1) >= 2) & ((S1 + (G * G).sum(1)) >= 1e-12)
+        gate = (nc.sum(1) >= 2) & ((S1 + (G * G).sum(1)) >= 1e-12)
         lam = torch.where(gate, lam, torch.zeros_like(lam))
         if pbf.enable_lambda_clamp:
             lam = torch.clamp(lam, -pbf.lambda_max_abs, pbf.lambda_max_abs)
@@ -84,7 +100,12 @@ def pbf_step(X, V, sp, c, box_min, box_max):
     d, r2, r, nb = pairs(P)
     W = poly6 * torch.clamp(h2 - r2, min=0.0) ** 3 * nb.to(dt_)
     dv = (W[..., None] * (Vn[None, :, :] - Vn[:, None, :])).sum(1)
-    Vout = Vn + c * (m / rho0) * dv
+    gate = torch.ones(n, dtype=dt_)
+    if pbf.xsph_gate_enable:
+        n_min = max(0, pbf.xsph_n_min)
+        n_max = max(n_min + 1, pbf.xsph_n_max)
+        gate = torch.clamp((nb.sum(1).to(dt_) - n_min) / (n_max - n_min), 0.0, 1.0)
+    Vout = Vn + c * (gate * (m / rho0))[:, None] * dv
     return P, Vout
 
 
