@@ -6,11 +6,13 @@ Workload (the 1M dam-break configuration): 100^3 lattice at spacing 2r
 (r = 0.005 m, seeded jitter +-0.1 r) in a 2x1x1 m box, h = 2.5 r, rho0 = 1000,
 g = -9.81, dt = 1/120, 2 solver iterations, XSPH c = 0.01, hashed cell grid with a
 2^22-slot table, FP32.  A step is one full PBF step (predict, grid build, solve,
-velocity, XSPH, boundary) replayed from a hipGraph; state stays in HBM.
+velocity, XSPH, boundary); state stays in HBM.
 
-Prints ONE JSON line: particle-steps/s, per-launch times (events around every
-launch of eager profiled steps on the simulator's stream), the HBM roofline of the
-dominant kernel, and the CPU oracle timed on this host on a bounded sample.
+Prints ONE JSON line: particle-steps/s over the timed steps (eager, with a HIP
+event after every launch on the simulator's stream, so per-kernel durations are
+those of the timed launches), the same step replayed from the captured graph, the
+HBM roofline of the dominant kernel, the batched ensemble, the adjoint rollout and
+the CPU oracle timed on this host on a bounded sample.
 """
 import argparse
 import json
@@ -38,6 +40,32 @@ BYTES_PER_PARTICLE = {
     "delta_last": 64.0,
     "xsph_scatter": 36.0,
 }
+
+
+# HIP kernels of the launch labels (delta and delta_last are the first and last
+# launches of one kernel).
+KERNELS = {
+    "k_predict_hash": ["predict_hash"],
+    "k_lambda_sorted": ["lambda"],
+    "k_delta_sorted": ["delta", "delta_last"],
+    "k_xsph_scatter": ["xsph_scatter"],
+}
+
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "hbm_traffic_dam_break_1m.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the profiler counter summary kept next to
+    the other profiles (FETCH_SIZE + WRITE_SIZE, each scaled by a known-size copy), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None, "no counter summary"
+    k = rec.get("kernels", {}).get(kernel)
+    if not k:
+        return None, "kernel not in the counter summary"
+    return k["bytes_per_launch"], f"{os.path.relpath(TRAFFIC_FILE, REPO)} ({rec.get('command', '')})"
 
 
 def sort_bytes(key_bits):
@@ -149,7 +177,6 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--profile-reps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--grid", default="hashed", choices=["hashed", "compact", "dense"])
@@ -184,21 +211,25 @@ def main():
     print(f"[bench] seeded {N} particles, grid={a.grid}, warmup {a.warmup}", file=sys.stderr, flush=True)
     sim.step(a.warmup)
     sim.synchronize()
+    # Timed region: K eager steps on the simulator's stream with a HIP event after every
+    # launch, so the per-kernel durations are those of exactly the timed launches.
+    agg, cnt = {}, {}
     t0 = time.perf_counter()
-    sim.step(a.steps)
+    for _ in range(a.steps):
+        for label, ms in sim.profile_launches():
+            agg[label] = agg.get(label, 0.0) + ms
+            cnt[label] = cnt.get(label, 0) + 1
     sim.synchronize()
     dt_s = time.perf_counter() - t0
     value = N * a.steps / dt_s
     print(f"[bench] timed {a.steps} steps in {dt_s:.4f} s", file=sys.stderr, flush=True)
-
-    # per-launch times: eager steps with an event after every launch
-    agg, cnt = {}, {}
-    for _ in range(a.profile_reps):
-        for label, ms in sim.profile_launches():
-            agg[label] = agg.get(label, 0.0) + ms
-            cnt[label] = cnt.get(label, 0) + 1
-    per_step = {k: v / a.profile_reps for k, v in agg.items()}
-    avg_launch = {k: agg[k] / cnt[k] for k in agg}
+    # the same step replayed from the captured graph, for comparison
+    gsteps = max(1, a.steps // 5)
+    t0 = time.perf_counter()
+    sim.step(gsteps)
+    sim.synchronize()
+    graph_value = N * gsteps / (time.perf_counter() - t0)
+    per_step = {k: v / a.steps for k, v in agg.items()}
     overflow = sim.hash_overflow()
     X, V = sim.state()
     sim.close()
@@ -207,15 +238,20 @@ def main():
     bpp = dict(BYTES_PER_PARTICLE)
     bpp["sort"] = sort_bytes(key_bits)
     kernels = {}
-    for k, ms in avg_launch.items():
-        if k in bpp:
-            gbs = bpp[k] * N / (ms * 1e-3) / 1e9
-            kernels[k] = {"avg_ms": round(ms, 5), "GB/s": round(gbs, 1), "frac": round(gbs * 1e9 / HBM_PEAK, 4)}
-        else:
-            kernels[k] = {"avg_ms": round(ms, 5)}
-    dominant = max((k for k in per_step if k in bpp), key=lambda k: per_step[k])
-    dom_ms = avg_launch[dominant]
-    achieved = bpp[dominant] * N / (dom_ms * 1e-3)
+    for name, labels in KERNELS.items():
+        launches = sum(cnt.get(l, 0) for l in labels)
+        if not launches:
+            continue
+        ms = sum(agg.get(l, 0.0) for l in labels)
+        nbytes = sum(bpp[l] * N * cnt.get(l, 0) for l in labels)
+        kernels[name] = {"labels": labels, "launches_per_step": launches / a.steps,
+                         "avg_launch_ms": round(ms / launches, 5), "ms_per_step": round(ms / a.steps, 5),
+                         "bytes_per_launch": nbytes / launches, "GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                         "frac": round(nbytes / (ms * 1e-3) / HBM_PEAK, 5)}
+    dominant = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    dk = kernels[dominant]
+    achieved = dk["bytes_per_launch"] / (dk["avg_launch_ms"] * 1e-3)
+    traffic, traffic_note = measured_traffic(dominant)
 
     out = {
         "metric": "particle-steps/s @1M (2 iters); HBM GB/s vs peak; batched & adjoint/fwd ratio",
@@ -235,9 +271,11 @@ def main():
                    "dt": sp.dt, "xsph_c": sp.xsph_c, "grid": a.grid, "table_log2": scene.table_log2,
                    "cells": list(sp.grid.dim), "graph": True},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5), "traffic": None,
-                     "bytes_per_particle": bpp[dominant], "avg_launch_ms": round(dom_ms, 5)},
-        "phase_ms": {k: round(v, 5) for k, v in per_step.items()},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5), "traffic": traffic,
+                     "traffic_note": traffic_note, "bytes_per_launch": dk["bytes_per_launch"],
+                     "avg_launch_ms": dk["avg_launch_ms"]},
+        "graph_replay_particle_steps_per_s": round(graph_value, 1),
+        "phase_ms_per_step": {k: round(v, 5) for k, v in per_step.items()},
         "kernels": kernels,
         "hash_overflow": overflow,
         "finite": bool((X == X).all() and (V == V).all()),
