@@ -39,6 +39,29 @@ extern "C" int pbfSimSeedLattice(PbfSim* sim, uint32_t first, uint32_t nx, uint3
     return sim->s.seedBoxLattice(first, nx, ny, nz, make_float3(ox, oy, oz), spacing, jitterAmp, seed) ? 0 : -1;
 }
 
+extern "C" int pbfSimSeedCubeMix(PbfSim* sim, uint32_t first, uint32_t groupCount, const float* centers,
+                                 uint32_t edgeParticles, float spacing, float jitterAmp, uint64_t seed) {
+    if (!sim || (groupCount && !centers)) return -1;
+    std::vector<float3> c(groupCount);
+    for (uint32_t g = 0; g < groupCount; ++g) c[g] = make_float3(centers[3 * g], centers[3 * g + 1], centers[3 * g + 2]);
+    return sim->s.seedCubeMix(first, groupCount, c.data(), edgeParticles, spacing, jitterAmp, seed) ? 0 : -1;
+}
+
+extern "C" int pbfSimBindExternalPositions(PbfSim* sim, float4* posA, float4* posB, uint32_t capacity) {
+    if (!sim) return -1;
+    return sim->s.bindExternalPositions(posA, posB, capacity) ? 0 : -1;
+}
+
+extern "C" int pbfSimBindExternalVelocity(PbfSim* sim, void* dst, size_t bytes, uint32_t strideBytes) {
+    if (!sim) return -1;
+    return sim->s.bindExternalVelocity(dst, bytes, strideBytes) ? 0 : -1;
+}
+
+extern "C" int pbfSimPublishVelocity(PbfSim* sim, uint32_t count) {
+    if (!sim) return -1;
+    return sim->s.publishVelocity(count) ? 0 : -1;
+}
+
 extern "C" int pbfSimSetState(PbfSim* sim, const float4* pos, const float4* vel, uint32_t N, int fromDevice) {
     if (!sim) return -1;
     return sim->s.setState(pos, vel, N, fromDevice != 0) ? 0 : -1;

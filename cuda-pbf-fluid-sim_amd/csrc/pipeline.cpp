@@ -9,6 +9,7 @@
 #include "pbf_hip.h"
 #include "radix_sort.hpp"
 #include "sim_host.h"
+#include "scenes.hpp"
 #include "stats.hpp"
 
 namespace sim {
@@ -262,10 +263,26 @@ bool DeviceBuffers::allocate(uint32_t cap) {
 }
 
 void DeviceBuffers::release() {
+    if (externalPos) {
+        d_pos_curr = d_pos_next = nullptr;
+        externalPos = false;
+    }
     dfree(d_pos_curr); dfree(d_pos_next); dfree(d_vel); dfree(d_lambda); dfree(d_delta); dfree(d_vel_snapshot);
     dfree(d_pos_sorted[0]); dfree(d_pos_sorted[1]); dfree(d_vel_sorted);
     d_pos = d_pos_pred = nullptr;
     capacity = 0;
+}
+
+void DeviceBuffers::adoptExternalPositions(float4* a, float4* b) {
+    if (!externalPos) {
+        dfree(d_pos_curr);
+        dfree(d_pos_next);
+    }
+    d_pos_curr = a;
+    d_pos_next = b;
+    d_pos = d_pos_curr;
+    d_pos_pred = d_pos_next;
+    externalPos = true;
 }
 
 void DeviceBuffers::swapPositionPingPong() {
@@ -369,12 +386,7 @@ void BuildDefaultPipelines(PhasePipeline& pl, PathKind path) {
 
 // ---------------------------------------------------------------- simulator
 float JitterUniform(uint64_t seed, uint64_t particle, uint32_t axis) {
-    // SplitMix64 finaliser (Steele, Lea, Flood 2014) over (seed, 3*particle+axis+1).
-    uint64_t z = seed + 0x9E3779B97F4A7C15ull * (particle * 3ull + (uint64_t)axis + 1ull);
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    z = z ^ (z >> 31);
-    return (float)(uint32_t)(z >> 40) * (1.0f / 16777216.0f);
+    return pbfx::jitter_uniform(seed, particle, axis);
 }
 
 bool Simulator::initialize(const SimParams& p, const SimOptions& opt) {
@@ -659,12 +671,80 @@ bool Simulator::fillLattice(const GridBounds& g, uint32_t first, uint32_t nx, ui
             h[i].z = std::fmin(std::fmax(h[i].z + jz, g.mins.z), g.maxs.z);
         }
     }
-    const size_t bytes = sizeof(float4) * n;
+    return uploadBlock(first, h, "seedBoxLattice: copy failed");
+}
+
+bool Simulator::uploadBlock(uint32_t first, const std::vector<float4>& h, const char* where) {
+    const size_t bytes = sizeof(float4) * h.size();
+    if (hipStreamSynchronize(m_stream) != hipSuccess) {
+        set_error(PBFX_E_INVALID_ARGUMENT, where);
+        return false;
+    }
     bool ok = hipMemcpy(m_bufs.d_pos + first, h.data(), bytes, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(m_bufs.d_pos_pred + first, h.data(), bytes, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemset(m_bufs.d_vel + first, 0, bytes) == hipSuccess;
-    if (!ok) set_error(PBFX_E_INVALID_ARGUMENT, "seedBoxLattice: copy failed");
+    if (!ok) set_error(PBFX_E_INVALID_ARGUMENT, where);
     return ok;
+}
+
+bool Simulator::seedCubeMix(uint32_t first, uint32_t groups, const float3* centers, uint32_t edge, float spacing,
+                            float amp, uint64_t seed) {
+    const uint64_t n64 = (uint64_t)groups * edge * edge * edge;
+    if (!m_initialized || (groups && !centers) || first + n64 > m_bufs.capacity) {
+        set_error(PBFX_E_CAPACITY, "seedCubeMix: blocks exceed capacity");
+        return false;
+    }
+    std::vector<float4> h((size_t)n64);
+    pbfx::cube_mix_fill(m_params.grid, groups, centers, edge, spacing, amp, seed, h.data());
+    return uploadBlock(first, h, "seedCubeMix: copy failed");
+}
+
+bool Simulator::bindExternalPositions(float4* a, float4* b, uint32_t capacity) {
+    if (!m_initialized || !a || !b || a == b || capacity < m_bufs.capacity) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "bindExternalPositions: two distinct buffers of >= capacity float4");
+        return false;
+    }
+    const size_t bytes = sizeof(float4) * m_bufs.capacity;
+    bool ok = hipMemcpyAsync(a, m_bufs.d_pos_curr, bytes, hipMemcpyDeviceToDevice, m_stream) == hipSuccess &&
+              hipMemcpyAsync(b, m_bufs.d_pos_next, bytes, hipMemcpyDeviceToDevice, m_stream) == hipSuccess &&
+              hipStreamSynchronize(m_stream) == hipSuccess;
+    if (!ok) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "bindExternalPositions: copy failed");
+        return false;
+    }
+    m_bufs.adoptExternalPositions(a, b);
+    m_graphDirty = true;  // the captured launches hold the old pointers
+    return true;
+}
+
+bool Simulator::bindExternalVelocity(void* dst, size_t bytes, uint32_t strideBytes) {
+    if (!m_initialized || !dst || bytes == 0 || strideBytes < sizeof(float4)) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "bindExternalVelocity: buffer, size and stride >= 16 needed");
+        return false;
+    }
+    m_extVel = dst;
+    m_extVelBytes = bytes;
+    m_extVelStride = strideBytes;
+    return true;
+}
+
+bool Simulator::publishVelocity(uint32_t count) {
+    if (!m_initialized || !m_extVel) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "publishVelocity: no external velocity buffer bound");
+        return false;
+    }
+    count = std::min<uint32_t>(count, m_params.numParticles);
+    // rows that fit: the last one needs 16 bytes, the others a full stride
+    uint64_t rows = m_extVelBytes < sizeof(float4) ? 0 : (m_extVelBytes - sizeof(float4)) / m_extVelStride + 1;
+    rows = std::min<uint64_t>(rows, count);
+    if (rows == 0) return true;
+    const hipError_t e = hipMemcpy2DAsync(m_extVel, m_extVelStride, m_bufs.d_vel, sizeof(float4), sizeof(float4),
+                                          (size_t)rows, hipMemcpyDeviceToDevice, m_stream);
+    if (e != hipSuccess) {
+        set_error((int)e, "publishVelocity");
+        return false;
+    }
+    return true;
 }
 
 bool Simulator::setState(const float4* pos, const float4* vel, uint32_t N, bool fromDevice) {

@@ -52,8 +52,10 @@ struct DeviceBuffers {
     // rec.Vs, and the clamps write their bits.
     pbfx::StepTape rec{};
     uint32_t capacity = 0;
+    bool externalPos = false;  // d_pos_curr / d_pos_next belong to the caller
 
     bool allocate(uint32_t cap);
+    void adoptExternalPositions(float4* a, float4* b);
     void release();
     void swapPositionPingPong();
     ~DeviceBuffers() { release(); }
@@ -197,6 +199,13 @@ public:
     bool seedSceneLattice(uint32_t scene, uint32_t nx, uint32_t ny, uint32_t nz, float3 origin, float spacing,
                           float jitterAmp, uint64_t seed);
     bool getSceneState(uint32_t scene, float4* pos, float4* vel) const;
+    // CubeMix seeding (reference: simulator.cpp:920-975) at particle offset `first`.
+    bool seedCubeMix(uint32_t first, uint32_t groups, const float3* centers, uint32_t edge, float spacing,
+                     float jitterAmp, uint64_t seed);
+    // External buffers (reference: simulator.cpp:676-802, 1069-1079) as device pointers.
+    bool bindExternalPositions(float4* a, float4* b, uint32_t capacity);
+    bool bindExternalVelocity(void* dst, size_t bytes, uint32_t strideBytes);
+    bool publishVelocity(uint32_t count);
     uint32_t sceneCount() const { return (uint32_t)m_scenes.size(); }
     const pbfx::SceneParams* sceneParams(uint32_t scene) const {
         return scene < m_scenes.size() ? &m_scenes[scene] : nullptr;
@@ -232,6 +241,10 @@ private:
     bool initCommon(const SimParams& p, const SimOptions& opt, uint64_t cells);
     bool fillLattice(const GridBounds& g, uint32_t first, uint32_t nx, uint32_t ny, uint32_t nz, float3 o,
                      float spacing, float amp, uint64_t seed);
+    bool uploadBlock(uint32_t first, const std::vector<float4>& h, const char* where);
+    void* m_extVel = nullptr;
+    size_t m_extVelBytes = 0;
+    uint32_t m_extVelStride = 0;
     void releaseScenes();
 
     std::vector<pbfx::StepTape> m_tapes;

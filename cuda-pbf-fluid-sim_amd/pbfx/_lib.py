@@ -65,6 +65,13 @@ SIGNATURES = {
     "pbfEnsembleSeedLattice": (_I, [_P, _U32, _U32, _U32, _U32, _F, _F, _F, _F, _F, ctypes.c_uint64]),
     "pbfEnsembleGetScene": (_I, [_P, _U32, _P, _P]),
     "pbfEnsembleSceneInfo": (_I, [_P, _U32, _P, _P, _P]),
+    # scene generators and external buffers
+    "pbfLatticeAutoDims": (_I, [_U32, _P, _P, _F, _P, _P]),
+    "pbfCubeMixCenters": (_I, [_P, _P]),
+    "pbfSimSeedCubeMix": (_I, [_P, _U32, _U32, _P, _U32, _F, _F, ctypes.c_uint64]),
+    "pbfSimBindExternalPositions": (_I, [_P, _P, _P, _U32]),
+    "pbfSimBindExternalVelocity": (_I, [_P, _P, ctypes.c_size_t, _U32]),
+    "pbfSimPublishVelocity": (_I, [_P, _U32]),
     "pbfDebugEchoDeviceParams": (None, [DeviceParams, ctypes.POINTER(DeviceParams)]),
     "pbfDebugEchoGridBounds": (None, [GridBounds, ctypes.POINTER(GridBounds)]),
     "pbfDebugEchoFloat3": (None, [Float3, ctypes.POINTER(ctypes.c_float)]),

@@ -10,6 +10,22 @@ GRID_KINDS = {"dense": 0, "compact": 1, "hashed": 2}
 PATHS = {"reordered": 0, "reference": 1}
 
 
+class _CudaArray:
+    """__cuda_array_interface__ holder for a device pointer owned by a simulator."""
+
+    def __init__(self, ptr, n, owner):
+        self.owner = owner
+        self.__cuda_array_interface__ = {"shape": (n, 4), "typestr": "<f4", "data": (int(ptr or 0), False),
+                                         "version": 2, "strides": None}
+
+
+def _device_view(ptr, n, owner):
+    import torch
+    if not ptr:
+        raise _lib.PbfError("no device buffer")
+    return torch.as_tensor(_CudaArray(ptr, n, owner), device="cuda")
+
+
 class Simulator:
     """sim::Simulator over the C ABI: initialize, seed, step, read back."""
 
@@ -53,6 +69,51 @@ class Simulator:
                 _lib.check("pbfSimSeedLattice")
                 raise _lib.PbfError("seed failed")
             first += nx * ny * nz
+
+    def seed_cube_mix(self, centers, edge, spacing, jitter_amp=0.0, seed=0, first=0):
+        """groups * edge^3 particles in cubes around `centers` ((G, 3)); see pbfSimSeedCubeMix."""
+        c = np.ascontiguousarray(centers, np.float32).reshape(-1, 3)
+        if _lib.lib().pbfSimSeedCubeMix(self._h, int(first), len(c), c.ctypes.data, int(edge), float(spacing),
+                                        float(jitter_amp), int(seed)) != 0:
+            _lib.check("pbfSimSeedCubeMix")
+            raise _lib.PbfError("seed_cube_mix failed")
+
+    def bind_external_positions(self, a, b):
+        """Adopt two caller-owned (>= N, 4) float32 CUDA tensors as the position ping-pong
+        (reference: bindExternalPosPingPong).  The tensors must outlive the simulator."""
+        import torch
+        for t in (a, b):
+            if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous() and t.dim() == 2
+                    and t.shape[1] == 4 and t.shape[0] >= self.n):
+                raise ValueError("external positions: contiguous (>= N, 4) float32 CUDA tensors")
+        torch.cuda.synchronize()
+        if _lib.lib().pbfSimBindExternalPositions(self._h, a.data_ptr(), b.data_ptr(), min(a.shape[0],
+                                                                                            b.shape[0])) != 0:
+            _lib.check("pbfSimBindExternalPositions")
+            raise _lib.PbfError("bind_external_positions failed")
+        self._external = (a, b)
+
+    def bind_external_velocity(self, buf, stride_bytes=16):
+        """Caller-owned CUDA tensor that publish_velocity() fills, one float4 every stride_bytes."""
+        if _lib.lib().pbfSimBindExternalVelocity(self._h, buf.data_ptr(), buf.numel() * buf.element_size(),
+                                                 int(stride_bytes)) != 0:
+            _lib.check("pbfSimBindExternalVelocity")
+            raise _lib.PbfError("bind_external_velocity failed")
+        self._external_vel = buf
+
+    def publish_velocity(self, count=None):
+        if _lib.lib().pbfSimPublishVelocity(self._h, self.n if count is None else int(count)) != 0:
+            _lib.check("pbfSimPublishVelocity")
+            raise _lib.PbfError("publish_velocity failed")
+
+    def positions_view(self):
+        """(N, 4) float32 CUDA tensor viewing the current position buffer (no copy).  The
+        buffer alternates between steps; take a new view after stepping."""
+        return _device_view(_lib.lib().pbfSimDevicePositions(self._h), self.n, self)
+
+    def velocities_view(self):
+        """(N, 4) float32 CUDA tensor viewing the velocity buffer (no copy)."""
+        return _device_view(_lib.lib().pbfSimDeviceVelocities(self._h), self.n, self)
 
     def set_state(self, pos, vel):
         pos = np.ascontiguousarray(pos, np.float32)

@@ -102,6 +102,53 @@ int pbfEnsembleGetScene(PbfSim* sim, uint32_t scene, float4* pos, float4* vel);
 // first particle, particle count and grid dims (3 ints) of a scene
 int pbfEnsembleSceneInfo(PbfSim* sim, uint32_t scene, uint32_t* first, uint32_t* count, int* dims3);
 
+// ---- scene generators -----------------------------------------------------
+// reference: Simulator::seedBoxLatticeAuto (simulator.cpp:882-941).  Host only.  The
+// lattice dims (nx, ny, nz) for about `total` particles at `spacing` in the box, and
+// originInOut pulled 0.25 spacing inside it.  The reference then reseeds its
+// simulator with nx*ny*nz particles; here the caller creates the simulator with
+// that count and seeds it with pbfSimSeedLattice.
+int pbfLatticeAutoDims(uint32_t total, const float* boxMin3, const float* boxMax3, float spacing,
+                       float* originInOut3, uint32_t* dimsOut3);
+// reference: the CubeMix settings of RuntimeConsole::Simulation (console.h:184-208).
+typedef struct {
+    uint32_t groupCount;
+    uint32_t layers;
+    float groupSpacing;   // x/z distance between group centres
+    float layerSpacing;   // y distance between layers
+    float baseHeight;     // y of the first layer
+    int layerJitter;      // per-layer y offset of +-layerJitterAmp
+    float layerJitterAmp;
+    uint32_t layerJitterSeed;
+} PbfCubeMixLayout;
+// reference: GenerateCubeMixCenters (console.cpp:277-316).  Host only; writes
+// 3 * groupCount floats.  The layer jitter draws from SplitMix64(seed ^ layer)
+// instead of std::mt19937.
+int pbfCubeMixCenters(const PbfCubeMixLayout* layout, float* centersOut);
+// reference: Simulator::seedCubeMix (simulator.cpp:920-975), at particle offset
+// `first`: groupCount blocks of edge^3 particles centred on centers (3 floats each),
+// x fastest, clamped to the box, w = group index, velocity 0, per-axis jitter of
+// +-jitterAmp from SplitMix64(seed, particle, axis) (the reference: unit-ball rejection
+// from std::mt19937).
+int pbfSimSeedCubeMix(PbfSim* sim, uint32_t first, uint32_t groupCount, const float* centers,
+                      uint32_t edgeParticles, float spacing, float jitterAmp, uint64_t seed);
+
+// ---- external buffers -----------------------------------------------------
+// reference: Simulator::bindExternalPosPingPong (simulator.cpp:676-757), with device
+// pointers in place of imported D3D12 resources.  The simulator adopts posA (current)
+// and posB (next) as its position ping-pong; each holds >= capacity float4 (capacity
+// >= numParticles).  The current state is copied into them; the simulator's own
+// position buffers are released; the buffers stay owned by the caller and must
+// outlive the simulator or the next bind.  Which of the two holds the current state
+// after a step is given by pbfSimDevicePositions.
+int pbfSimBindExternalPositions(PbfSim* sim, float4* posA, float4* posB, uint32_t capacity);
+// reference: bindExternalVelocityBuffer / publishExternalVelocity (simulator.cpp:
+// 760-802, 1069-1079).  publish copies `count` velocities (float4, original particle
+// order) to dst, one every strideBytes (>= 16), clipped to `bytes`, on the
+// simulator's stream.
+int pbfSimBindExternalVelocity(PbfSim* sim, void* dst, size_t bytes, uint32_t strideBytes);
+int pbfSimPublishVelocity(PbfSim* sim, uint32_t count);
+
 // ABI self-checks (host only): copy by-value structs back out, so a foreign
 // caller can verify it passes them with the same layout and calling convention.
 void pbfDebugEchoDeviceParams(sim::DeviceParams dp, sim::DeviceParams* out);

@@ -274,3 +274,94 @@ def pow_scorr(x, n):
 
 def num_threads():
     return lib().oracle_num_threads()
+
+
+# ------------------------------------------------------------ scene generators
+# Restated in numpy / Python integers (float32 where the reference computes in float).
+
+def _splitmix_uniform(seed, index):
+    """The seeders' variate, vectorised: SplitMix64 finaliser over seed + golden * (index + 1)
+    (index = 3 * particle + axis), top 24 bits as a float in [0, 1)."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + np.uint64(0x9E3779B97F4A7C15) * (np.asarray(index, np.uint64) + np.uint64(1))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(40)).astype(np.uint32).astype(np.float32) * np.float32(1.0 / 16777216.0)
+
+
+def lattice_auto_dims(total, box_min, box_max, spacing, origin):
+    """reference: simulator.cpp:882-941 (seedBoxLatticeAuto): returns ((nx, ny, nz), origin)."""
+    f = np.float32
+    sp = f(spacing)
+    size = [max(f(0), f(box_max[a]) - f(box_min[a])) for a in range(3)]
+    mx = [max(1, int(np.floor(f(s) / sp))) for s in size]
+    T = max(1, int(total))
+    nx = min(mx[0], int(np.ceil(np.power(float(T), 1.0 / 3.0))))
+    rem = (T + nx - 1) // nx
+    ny = min(mx[1], int(np.ceil(np.sqrt(float(rem)))))
+    nz = min(mx[2], int(np.ceil(float(rem) / max(1, ny))))
+    while nx * ny * nz < T:
+        if nx < mx[0]:
+            nx += 1
+        elif ny < mx[1]:
+            ny += 1
+        elif nz < mx[2]:
+            nz += 1
+        else:
+            break
+    m = sp * f(0.25)
+    o = tuple(float(min(max(f(origin[a]), f(box_min[a]) + m), f(box_max[a]) - m)) for a in range(3))
+    return (nx, ny, nz), o
+
+
+def cube_mix_centers(group_count, layers, group_spacing, layer_spacing, base_height, layer_jitter=False,
+                     layer_jitter_amp=0.0, layer_jitter_seed=0x13579BDF):
+    """reference: console.cpp:277-316 (GenerateCubeMixCenters).  Per-layer counts use
+    uint32 arithmetic (remaining wraps when layers outnumber groups); the layer jitter
+    is one seeder variate per layer, from seed ^ layer."""
+    f = np.float32
+    counts, remaining = [], group_count & 0xFFFFFFFF
+    for L in range(layers):
+        take = remaining // (layers - L) or 1
+        counts.append(take)
+        remaining = (remaining - take) & 0xFFFFFFFF
+    out = []
+    for layer in range(layers):
+        if len(out) >= group_count:
+            break
+        n = counts[layer]
+        nx = int(np.ceil(np.sqrt(float(n))))
+        y = f(base_height) + f(layer) * f(layer_spacing)
+        if layer_jitter:
+            u = _splitmix_uniform((layer_jitter_seed ^ layer) & 0xFFFFFFFFFFFFFFFF, 1)
+            y = f(y + (f(2) * u - f(1)) * f(layer_jitter_amp))
+        for k in range(n):
+            if len(out) >= group_count:
+                break
+            out.append((f(k % nx) * f(group_spacing), y, f(k // nx) * f(group_spacing)))
+    return np.array(out, np.float32).reshape(-1, 3)
+
+
+def seed_cube_mix(box_min, box_max, centers, edge, spacing, jitter_amp, seed):
+    """reference: simulator.cpp:920-975 (seedCubeMix): (groups * edge^3, 4) float32, x fastest,
+    w = group index, clamped; per-axis jitter of +-jitter_amp, clamped again."""
+    f = np.float32
+    lo = np.array(box_min, f)
+    hi = np.array(box_max, f)
+    sp = f(spacing)
+    start = f(-0.5) * sp * f(edge - 1)
+    iz, iy, ix = np.meshgrid(np.arange(edge), np.arange(edge), np.arange(edge), indexing="ij")
+    offs = np.stack([ix.ravel(), iy.ravel(), iz.ravel()], 1).astype(f) * sp
+    blocks = []
+    for g, c in enumerate(np.asarray(centers, f).reshape(-1, 3)):
+        p = (c + start) + offs
+        p = np.minimum(np.maximum(p, lo), hi)
+        blocks.append(np.concatenate([p, np.full((len(p), 1), f(g))], 1))
+    X = np.concatenate(blocks).astype(f) if blocks else np.zeros((0, 4), f)
+    if jitter_amp > 0:
+        n = len(X)
+        idx = np.arange(n, dtype=np.uint64)[:, None] * np.uint64(3) + np.arange(3, dtype=np.uint64)[None, :]
+        j = (f(2) * _splitmix_uniform(seed, idx) - f(1)) * f(jitter_amp)
+        X[:, :3] = np.minimum(np.maximum(X[:, :3] + j, lo), hi)
+    return X
