@@ -73,7 +73,7 @@ def sort_bytes(key_bits):
     return 16.0 * passes + 4.0
 
 
-def bench_batched(count, warmup, steps):
+def bench_batched(count, warmup, steps, tune_steps=0):
     """The batched ensemble (one launch per phase for all scenes, graph replay) against the
     scene-by-scene plain path: the same scenes as separate simulators, each replaying its own
     graph on its own stream, all enqueued before one synchronisation (and, as a second
@@ -84,6 +84,10 @@ def bench_batched(count, warmup, steps):
     ens = Ensemble(scenes, grid="hashed")
     ens.step(warmup)
     ens.synchronize()
+    best = None
+    if tune_steps:
+        from pbfx import tune
+        best, _, _ = tune.autotune(ens, steps=tune_steps, key=tune.workload_key(ens, f"ensemble{count}"))
     t0 = time.perf_counter()
     ens.step(steps)
     ens.synchronize()
@@ -93,6 +97,8 @@ def bench_batched(count, warmup, steps):
           file=sys.stderr, flush=True)
     sims = [solo(sc, grid="hashed") for sc in scenes]
     for sim in sims:
+        if best:
+            sim.set_tuning(**best)
         sim.step(warmup)
     for sim in sims:
         sim.synchronize()
@@ -117,7 +123,7 @@ def bench_batched(count, warmup, steps):
             "particle_steps_per_s": round(total * steps / t_ens, 1),
             "scene_by_scene_particle_steps_per_s": round(total * steps / t_streams, 1),
             "speedup_vs_scene_by_scene": round(t_streams / t_ens, 3),
-            "speedup_vs_serial_scenes": round(t_serial / t_ens, 3)}
+            "speedup_vs_serial_scenes": round(t_serial / t_ens, 3), "tuning": best}
 
 
 def bench_adjoint(steps=60, checkpoint=10):
@@ -176,7 +182,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=60)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch knobs")
+    ap.add_argument("--tune-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--grid", default="hashed", choices=["hashed", "compact", "dense"])
@@ -211,6 +219,13 @@ def main():
     print(f"[bench] seeded {N} particles, grid={a.grid}, warmup {a.warmup}", file=sys.stderr, flush=True)
     sim.step(a.warmup)
     sim.synchronize()
+    tuning = None
+    if not a.no_tune:
+        from pbfx import tune
+        best, best_ms, table = tune.autotune(sim, steps=a.tune_steps, key=tune.workload_key(sim, "dam_break_1m"),
+                                             log=lambda m: print(m, file=sys.stderr, flush=True))
+        tuning = {"chosen": best, "ms_per_step_at_tuning": round(best_ms, 4), "settings_tried": len(table),
+                  "steps_per_setting": a.tune_steps, "state": f"after the {a.warmup} warm-up steps"}
     # Timed region: K eager steps on the simulator's stream with a HIP event after every
     # launch, so the per-kernel durations are those of exactly the timed launches.
     agg, cnt = {}, {}
@@ -275,6 +290,7 @@ def main():
                      "traffic_note": traffic_note, "bytes_per_launch": dk["bytes_per_launch"],
                      "avg_launch_ms": dk["avg_launch_ms"]},
         "graph_replay_particle_steps_per_s": round(graph_value, 1),
+        "tuning": tuning,
         "phase_ms_per_step": {k: round(v, 5) for k, v in per_step.items()},
         "kernels": kernels,
         "hash_overflow": overflow,
@@ -282,7 +298,8 @@ def main():
     }
 
     if not a.no_batched:
-        out["batched"] = bench_batched(a.batch_scenes, a.batch_warmup, a.batch_steps)
+        out["batched"] = bench_batched(a.batch_scenes, a.batch_warmup, a.batch_steps,
+                                       0 if a.no_tune else a.tune_steps)
 
     if not a.no_adjoint:
         out["adjoint"] = bench_adjoint()

@@ -62,6 +62,24 @@ extern "C" int pbfSimPublishVelocity(PbfSim* sim, uint32_t count) {
     return sim->s.publishVelocity(count) ? 0 : -1;
 }
 
+extern "C" int pbfSimSetTuning(PbfSim* sim, const PbfSimTuning* t) {
+    if (!sim || !t) return -1;
+    return sim->s.setTuning(t->candidateMask != 0, t->streamRatio, t->chunkBits != 0, t->sortItems) ? 0 : -1;
+}
+
+extern "C" int pbfSimGetTuning(PbfSim* sim, PbfSimTuning* t) {
+    if (!sim || !t) return -1;
+    bool cm = false, cb = false;
+    uint32_t ratio = 0;
+    int items = 0;
+    sim->s.getTuning(cm, ratio, cb, items);
+    t->sortItems = items;
+    t->candidateMask = cm ? 1 : 0;
+    t->streamRatio = ratio;
+    t->chunkBits = cb ? 1 : 0;
+    return 0;
+}
+
 extern "C" int pbfSimSetState(PbfSim* sim, const float4* pos, const float4* vel, uint32_t N, int fromDevice) {
     if (!sim) return -1;
     return sim->s.setState(pos, vel, N, fromDevice != 0) ? 0 : -1;

@@ -210,8 +210,8 @@ struct HashRowsDev {
 // Order.  Each lane still sees its candidates in reference order (rows dz-major
 // then dy, slots ascending within a row) and consumes its FIFO in order, so
 // every sum is formed in the reference's order.
-constexpr int kQD = 8;           // FIFO depth per lane
-constexpr uint32_t kStreamRatio = 3;  // stream a row when union <= ratio * longest lane span
+constexpr int kQD = 16;          // FIFO depth per lane
+constexpr uint32_t kUnroll = 4;  // candidates per round of a lane's own walk
 
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
 #pragma unroll
@@ -234,6 +234,9 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 // The mask covers the first 32 * kMaskWords candidates; candidates past that are
 // tested directly.
 constexpr uint32_t kMaskWords = kCandMaskWords;
+
+// Bits [0, n) of a 64-bit word (n <= 64).
+__device__ __forceinline__ uint64_t low_bits64(uint32_t n) { return n >= 64u ? ~0ull : ((1ull << n) - 1ull); }
 
 template <bool kBuild, class Dist2, class Apply>
 __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, uint32_t m, const SpanTable& st,
@@ -284,18 +287,56 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, uint32_t m
         const uint32_t uhi = wave_max_u32(has ? hi : 0u);
         if (uhi <= ulo) continue;
         const uint32_t lmax = wave_max_u32(hi - lo);
-        if (uhi - ulo > kStreamRatio * lmax) {
-            // sparse row: per-lane walk over the lane's own span
+        if (uhi - ulo > st.streamRatio * lmax) {
+            // sparse row: each lane walks its own span, kUnroll candidates a round so
+            // that their loads are in flight together; in-range ones are queued in order
             uint32_t k = lo;
             while (__any(k < hi)) {
-                if (__any(qn == kQD)) pop_one();
-                if (k < hi && qn < kQD) {
-                    const uint32_t cand = k++;
-                    visit(cand, dist2(P[cand]), hi, lo);
+                const uint32_t cnt = k < hi ? min(kUnroll, hi - k) : 0u;
+                float4 pj[kUnroll];
+#pragma unroll
+                for (uint32_t u = 0; u < kUnroll; ++u) {
+                    pj[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (u < cnt) pj[u] = P[k + u];
+                }
+                uint32_t bits = 0u, skin = 0u;
+#pragma unroll
+                for (uint32_t u = 0; u < kUnroll; ++u) {
+                    const float r2 = dist2(pj[u]);
+                    if (u < cnt && r2 <= h2 && k + u != s) bits |= 1u << u;
+                    if (kBuild && u < cnt && r2 <= mk.rs2) skin |= 1u << u;
+                }
+                if (kBuild && cnt > 0) {
+                    const uint32_t sh = cidx & 31u;
+                    uint64_t w = (uint64_t)word | ((uint64_t)skin << sh);
+                    if (sh + cnt >= 32u) {
+                        const uint32_t wi = cidx >> 5;
+                        if (wi < kMaskWords) mk.words[(uint64_t)wi * mk.stride + s] = (uint32_t)w;
+                        w >>= 32;
+                    }
+                    word = (uint32_t)w;
+                    cidx += cnt;
+                }
+                if (cap > 0) {
+                    const int room = cap - found;
+                    while (__popc(bits) > room) bits &= ~(1u << (31 - __clz(bits)));
+                    found += __popc(bits);
+                    if (found >= cap) hi = k;  // nothing more to take in this row
+                }
+                k += cnt;
+                while (__any(bits != 0u)) {
+                    if (__any(qn == kQD)) pop_one();
+                    if (bits != 0u && qn < kQD) {
+                        const uint32_t t = (uint32_t)__ffs((int)bits) - 1u;
+                        bits &= bits - 1u;
+                        q[((qh + qn) & (kQD - 1)) * kBS] = k - cnt + t;
+                        ++qn;
+                    }
                 }
             }
-        } else {
-            // dense row: stream the union through LDS
+        } else if (kBuild || st.chunkBits == 0) {
+            // dense row, candidate by candidate (and building the candidate mask):
+            // stream the union through LDS
             for (uint32_t base = ulo; base < uhi; base += 64) {
                 const uint32_t c = base + lane;
                 if (c < uhi) buf[lane] = P[c];
@@ -308,6 +349,43 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, uint32_t m
                 }
                 __builtin_amdgcn_wave_barrier();
             }
+        } else {
+            // dense row: stream the union through LDS, 64 candidates at a time.  Every
+            // lane tests every candidate of a chunk in a branch-free wave-uniform loop
+            // and keeps the in-range ones as bits; then only its own span's bits, minus
+            // its own slot and anything past the neighbour cap, enter its FIFO in order.
+            for (uint32_t base = ulo; base < uhi; base += 64) {
+                const uint32_t c = base + lane;
+                if (c < uhi) buf[lane] = P[c];
+                __builtin_amdgcn_wave_barrier();
+                const uint32_t n = min(64u, uhi - base);
+                const uint32_t n0 = min(n, 32u);
+                uint32_t m0 = 0u, m1 = 0u;
+                for (uint32_t t = 0; t < n0; ++t) m0 |= (dist2(buf[t]) <= h2) ? (1u << t) : 0u;
+                for (uint32_t t = 32; t < n; ++t) m1 |= (dist2(buf[t]) <= h2) ? (1u << (t - 32)) : 0u;
+                __builtin_amdgcn_wave_barrier();
+                const uint32_t a = lo > base ? min(lo - base, n) : 0u;
+                const uint32_t b = hi > base ? min(hi - base, n) : 0u;
+                uint64_t bits = (((uint64_t)m1 << 32) | m0) & low_bits64(b) & ~low_bits64(a);
+                if (s - base < n) bits &= ~(1ull << (s - base));
+                if (cap > 0) {
+                    const int room = cap - found;
+                    while (__popcll(bits) > room) bits &= ~(1ull << (63 - __clzll(bits)));
+                    found += __popcll(bits);
+                }
+                while (__any(bits != 0ull)) {
+                    if (__any(qn == kQD)) pop_one();
+                    if (bits != 0ull && qn < kQD) {
+                        const uint32_t t = (uint32_t)__ffsll((long long)bits) - 1u;
+                        bits &= bits - 1ull;
+                        q[((qh + qn) & (kQD - 1)) * kBS] = base + t;
+                        ++qn;
+                    }
+                }
+                // stop the row once no lane needs more of it
+                if (!__any((cap <= 0 || found < cap) && hi > base + 64)) break;
+            }
+            if (cap > 0 && found >= cap) hi = lo;
         }
     }
     if (kBuild && valid && (cidx & 31) != 0) {

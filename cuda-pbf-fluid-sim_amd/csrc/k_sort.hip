@@ -215,9 +215,9 @@ struct Layout {
     size_t offK, offV, offCounts, offTotals, bytes;
 };
 
-Layout layout_for(uint32_t N) {
+Layout layout_for(uint32_t N, int items = 0) {
     Layout L{};
-    L.items = items_for(N);
+    L.items = (items == 4 || items == 8 || items == 16) ? items : items_for(N);
     L.tile = kBS * L.items;
     L.numTiles = ceil_div(N > 0 ? N : 1, L.tile);
     L.offK = 0;
@@ -239,12 +239,14 @@ void launch_pass(const Layout& L, const uint32_t* kin, const uint32_t* vin, uint
 
 }  // namespace
 
-size_t radix_sort_temp_bytes(uint32_t N, int, int) { return layout_for(N).bytes; }
+// Sized for the smallest tile (4 items a thread: the most tiles), so that every
+// items-per-thread choice fits.
+size_t radix_sort_temp_bytes(uint32_t N, int, int) { return layout_for(N, 4).bytes; }
 
 bool radix_sort_pairs(void* temp, size_t tempBytes, const uint32_t* kin, uint32_t* kout, const uint32_t* vin,
-                      uint32_t* vout, uint32_t N, int beginBit, int endBit, hipStream_t s) {
+                      uint32_t* vout, uint32_t N, int beginBit, int endBit, hipStream_t s, int items) {
     if (N == 0) return true;
-    const Layout L = layout_for(N);
+    const Layout L = layout_for(N, items);
     if (!temp || tempBytes < L.bytes) {
         set_error(PBFX_E_CAPACITY, "radix sort: temp storage too small");
         return false;

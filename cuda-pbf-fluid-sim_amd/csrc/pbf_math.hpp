@@ -22,6 +22,19 @@ namespace pbfx {
 // same product in the same order, so both sides produce the same bits.  The
 // reference calls powf (CUDA: <= 4 ulp), pbf_delta_apply.cu:81.
 __device__ __forceinline__ float pow_scorr(float x, float n, int ni) {
+    // the common exponents, straight-line: the same products as the loop below
+    // (1 * y is exact), without its loop overhead
+    const double xd = (double)x;
+    switch (ni) {
+        case 1: return x;
+        case 2: return (float)(xd * xd);
+        case 3: return (float)(xd * (xd * xd));
+        case 4: {
+            const double x2 = xd * xd;
+            return (float)(x2 * x2);
+        }
+        default: break;
+    }
     if (ni > 0) {
         double b = (double)x, r = 1.0;
         int e = ni;

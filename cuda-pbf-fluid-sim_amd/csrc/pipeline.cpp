@@ -152,7 +152,7 @@ void SortedPredict(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStre
 void SortedGridBuild(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_t s) {
     const uint32_t N = p.numParticles;
     pbfx::radix_sort_pairs(g.d_sortTemp, g.sortTempBytes, g.d_cellKeys, g.d_cellKeys_sorted, nullptr,
-                           g.d_indices_sorted, N, 0, g.keyBits, s);
+                           g.d_indices_sorted, N, 0, g.keyBits, s, g.sortItems);
     mark("sort");
     pbfx::compact_segments(g.d_cellUniqueKeys, g.d_cellOffsets, g.d_compactCount, g.d_segOf, g.d_cellKeys_sorted, N,
                            g.d_blockCounts, g.d_blockOffsets, s);
@@ -186,7 +186,7 @@ void SortedSolve(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream
     // the first delta pass records the candidate mask the later passes walk
     pbfx::CandMask mk = g.mask;
     mk.rs2 = pbfx::mask_skin_radius2(p);
-    const bool useMask = g.mask.words != nullptr && mk.rs2 > 0.f;
+    const bool useMask = g.candidateMask && g.mask.words != nullptr && mk.rs2 > 0.f;
     const bool rec = b.rec.P[0] != nullptr;
     int cur = 0;
     for (int it = 0; it < p.solverIters; ++it) {
@@ -717,6 +717,19 @@ bool Simulator::bindExternalPositions(float4* a, float4* b, uint32_t capacity) {
     return true;
 }
 
+bool Simulator::setTuning(bool candidateMask, uint32_t streamRatio, bool chunkBits, int sortItems) {
+    if (!m_initialized || streamRatio < 1 || !(sortItems == 0 || sortItems == 4 || sortItems == 8 || sortItems == 16)) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "setTuning: streamRatio >= 1, sortItems in {0, 4, 8, 16}");
+        return false;
+    }
+    m_grid.sortItems = sortItems;
+    m_grid.candidateMask = candidateMask;
+    m_grid.spans.streamRatio = streamRatio;
+    m_grid.spans.chunkBits = chunkBits ? 1u : 0u;
+    m_graphDirty = true;
+    return true;
+}
+
 bool Simulator::bindExternalVelocity(void* dst, size_t bytes, uint32_t strideBytes) {
     if (!m_initialized || !dst || bytes == 0 || strideBytes < sizeof(float4)) {
         set_error(PBFX_E_INVALID_ARGUMENT, "bindExternalVelocity: buffer, size and stride >= 16 needed");
@@ -754,7 +767,7 @@ bool Simulator::setState(const float4* pos, const float4* vel, uint32_t N, bool 
     }
     const hipMemcpyKind k = fromDevice ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     const size_t bytes = sizeof(float4) * N;
-    bool ok = true;
+    bool ok = hipStreamSynchronize(m_stream) == hipSuccess;  // no step may still use the buffers
     if (pos) {
         ok = ok && hipMemcpy(m_bufs.d_pos, pos, bytes, k) == hipSuccess;
         ok = ok && hipMemcpy(m_bufs.d_pos_pred, pos, bytes, k) == hipSuccess;

@@ -37,6 +37,12 @@ struct SpanTable {
     uint32_t stride = 0;
     int reach = 1;
     int rows = 9;
+    // a wave streams a row through LDS when the union of its lanes' spans is at most
+    // streamRatio times the longest one; otherwise each lane walks its own span
+    uint32_t streamRatio = 3;
+    // streamed rows: 1 = each lane tests a whole 64-candidate chunk into a bit mask,
+    // then queues its set bits; 0 = each candidate is tested and queued in turn
+    uint32_t chunkBits = 1;
 };
 
 // Candidate mask words per particle: the mask covers the first 32 * kCandMaskWords

@@ -72,6 +72,8 @@ SIGNATURES = {
     "pbfSimBindExternalPositions": (_I, [_P, _P, _P, _U32]),
     "pbfSimBindExternalVelocity": (_I, [_P, _P, ctypes.c_size_t, _U32]),
     "pbfSimPublishVelocity": (_I, [_P, _U32]),
+    "pbfSimSetTuning": (_I, [_P, _P]),
+    "pbfSimGetTuning": (_I, [_P, _P]),
     "pbfDebugEchoDeviceParams": (None, [DeviceParams, ctypes.POINTER(DeviceParams)]),
     "pbfDebugEchoGridBounds": (None, [GridBounds, ctypes.POINTER(GridBounds)]),
     "pbfDebugEchoFloat3": (None, [Float3, ctypes.POINTER(ctypes.c_float)]),

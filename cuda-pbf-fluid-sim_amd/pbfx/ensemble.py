@@ -67,6 +67,8 @@ class Ensemble:
             d.numParticles = sc.num_particles
             total += sc.num_particles
         self.n = total
+        self.grid = grid
+        self.params = base
         if table_log2 is None:
             table_log2 = max(16, int(math.ceil(math.log2(max(total, 1)))) + 1)
         L = _lib.lib()
@@ -122,6 +124,12 @@ class Ensemble:
 
     def profile_launches(self):
         return Simulator.profile_launches(self)
+
+    # whole-ensemble state (all scenes, scene order) and launch knobs, as for one scene
+    state = Simulator.state
+    set_state = Simulator.set_state
+    tuning = Simulator.tuning
+    set_tuning = Simulator.set_tuning
 
 
 def solo(scene, grid="hashed", use_graph=True):

@@ -10,6 +10,11 @@ GRID_KINDS = {"dense": 0, "compact": 1, "hashed": 2}
 PATHS = {"reordered": 0, "reference": 1}
 
 
+class _Tuning(ctypes.Structure):
+    _fields_ = [("candidateMask", ctypes.c_int), ("streamRatio", ctypes.c_uint32), ("chunkBits", ctypes.c_int),
+                ("sortItems", ctypes.c_int)]
+
+
 class _CudaArray:
     """__cuda_array_interface__ holder for a device pointer owned by a simulator."""
 
@@ -105,6 +110,29 @@ class Simulator:
         if _lib.lib().pbfSimPublishVelocity(self._h, self.n if count is None else int(count)) != 0:
             _lib.check("pbfSimPublishVelocity")
             raise _lib.PbfError("publish_velocity failed")
+
+    def set_tuning(self, candidate_mask=None, stream_ratio=None, chunk_bits=None, sort_items=None):
+        """Change launch knobs (results are bit-identical for every setting)."""
+        t = self.tuning()
+        if candidate_mask is not None:
+            t["candidate_mask"] = bool(candidate_mask)
+        if stream_ratio is not None:
+            t["stream_ratio"] = int(stream_ratio)
+        if chunk_bits is not None:
+            t["chunk_bits"] = bool(chunk_bits)
+        if sort_items is not None:
+            t["sort_items"] = int(sort_items)
+        c = _Tuning(int(t["candidate_mask"]), t["stream_ratio"], int(t["chunk_bits"]), t["sort_items"])
+        if _lib.lib().pbfSimSetTuning(self._h, ctypes.byref(c)) != 0:
+            _lib.check("pbfSimSetTuning")
+            raise _lib.PbfError("set_tuning failed")
+
+    def tuning(self):
+        c = _Tuning()
+        if _lib.lib().pbfSimGetTuning(self._h, ctypes.byref(c)) != 0:
+            raise _lib.PbfError("get_tuning failed")
+        return {"candidate_mask": bool(c.candidateMask), "stream_ratio": int(c.streamRatio),
+                "chunk_bits": bool(c.chunkBits), "sort_items": int(c.sortItems)}
 
     def positions_view(self):
         """(N, 4) float32 CUDA tensor viewing the current position buffer (no copy).  The

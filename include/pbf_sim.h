@@ -102,6 +102,22 @@ int pbfEnsembleGetScene(PbfSim* sim, uint32_t scene, float4* pos, float4* vel);
 // first particle, particle count and grid dims (3 ints) of a scene
 int pbfEnsembleSceneInfo(PbfSim* sim, uint32_t scene, uint32_t* first, uint32_t* count, int* dims3);
 
+// ---- launch tuning --------------------------------------------------------
+// Knobs of the reordered solver that change speed, never results (every setting
+// gives the same bits).  No reference counterpart; picked per workload by the
+// autotuner (pbfx/tune.py).
+typedef struct {
+    int candidateMask;     // 1: the first delta pass records which candidates can come within h
+                           // this step; the later lambda / delta passes walk only those
+    uint32_t streamRatio;  // a wave streams a neighbour row through LDS when the union of its
+                           // lanes' spans is <= streamRatio x the longest span (>= 1)
+    int chunkBits;         // streamed rows: 1 = test a 64-candidate chunk into a bit mask, then
+                           // queue the set bits; 0 = test and queue candidate by candidate
+    int sortItems;         // radix sort keys per thread: 4, 8 or 16 (0 = chosen by N)
+} PbfSimTuning;
+int pbfSimSetTuning(PbfSim* sim, const PbfSimTuning* tuning);
+int pbfSimGetTuning(PbfSim* sim, PbfSimTuning* tuning);
+
 // ---- scene generators -----------------------------------------------------
 // reference: Simulator::seedBoxLatticeAuto (simulator.cpp:882-941).  Host only.  The
 // lattice dims (nx, ny, nz) for about `total` particles at `spacing` in the box, and
