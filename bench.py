@@ -73,15 +73,17 @@ def sort_bytes(key_bits):
     return 16.0 * passes + 4.0
 
 
-def bench_batched(count, warmup, steps, tune_steps=0):
+def bench_batched(count, warmup, steps, tune_steps=0, grid="dense"):
     """The batched ensemble (one launch per phase for all scenes, graph replay) against the
-    scene-by-scene plain path: the same scenes as separate simulators, each replaying its own
-    graph on its own stream, all enqueued before one synchronisation (and, as a second
-    figure, synchronised scene by scene)."""
+    same scenes as separate plain simulators, each replaying its own captured step:
+    - scene by scene: one scene after another, `steps` steps each, one synchronisation per
+      scene (the plain path run over the scenes);
+    - concurrent streams: every scene's steps enqueued on its own stream before a single
+      synchronisation (the strongest unbatched arrangement)."""
     from pbfx.ensemble import Ensemble, ensemble_scenes, solo
     scenes = ensemble_scenes(count, seed=2024)
     total = sum(sc.num_particles for sc in scenes)
-    ens = Ensemble(scenes, grid="hashed")
+    ens = Ensemble(scenes, grid=grid)
     ens.step(warmup)
     ens.synchronize()
     best = None
@@ -95,11 +97,21 @@ def bench_batched(count, warmup, steps, tune_steps=0):
     ens.close()
     print(f"[bench] ensemble {count} scenes / {total} particles: {steps} steps in {t_ens:.4f} s",
           file=sys.stderr, flush=True)
-    sims = [solo(sc, grid="hashed") for sc in scenes]
+    sims = [solo(sc, grid=grid) for sc in scenes]
     for sim in sims:
         if best:
             sim.set_tuning(**best)
         sim.step(warmup)
+    for sim in sims:
+        sim.synchronize()
+    snaps = [sim.state() for sim in sims]
+    t0 = time.perf_counter()
+    for sim in sims:
+        sim.step(steps)
+        sim.synchronize()
+    t_seq = time.perf_counter() - t0
+    for sim, (X, V) in zip(sims, snaps):
+        sim.set_state(X, V)
     for sim in sims:
         sim.synchronize()
     t0 = time.perf_counter()
@@ -109,21 +121,16 @@ def bench_batched(count, warmup, steps, tune_steps=0):
     for sim in sims:
         sim.synchronize()
     t_streams = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        for sim in sims:
-            sim.step(1)
-            sim.synchronize()
-    t_serial = time.perf_counter() - t0
     for sim in sims:
         sim.close()
-    print(f"[bench] scene-by-scene: {t_streams:.4f} s (streams), {t_serial:.4f} s (serial)", file=sys.stderr,
+    print(f"[bench] scene by scene: {t_seq:.4f} s; concurrent streams: {t_streams:.4f} s", file=sys.stderr,
           flush=True)
-    return {"scenes": count, "particles": total, "steps": steps, "after_warmup_steps": warmup,
+    return {"scenes": count, "particles": total, "steps": steps, "after_warmup_steps": warmup, "grid": grid,
             "particle_steps_per_s": round(total * steps / t_ens, 1),
-            "scene_by_scene_particle_steps_per_s": round(total * steps / t_streams, 1),
-            "speedup_vs_scene_by_scene": round(t_streams / t_ens, 3),
-            "speedup_vs_serial_scenes": round(t_serial / t_ens, 3), "tuning": best}
+            "scene_by_scene_particle_steps_per_s": round(total * steps / t_seq, 1),
+            "speedup_vs_scene_by_scene": round(t_seq / t_ens, 3),
+            "concurrent_streams_particle_steps_per_s": round(total * steps / t_streams, 1),
+            "speedup_vs_concurrent_streams": round(t_streams / t_ens, 3), "tuning": best}
 
 
 def bench_adjoint(steps=60, checkpoint=10):
@@ -190,8 +197,8 @@ def main():
     ap.add_argument("--grid", default="hashed", choices=["hashed", "compact", "dense"])
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-scenes", type=int, default=256)
-    ap.add_argument("--batch-steps", type=int, default=10)
-    ap.add_argument("--batch-warmup", type=int, default=10)
+    ap.add_argument("--batch-steps", type=int, default=30)
+    ap.add_argument("--batch-warmup", type=int, default=20)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel that "
