@@ -396,6 +396,8 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, uint32_t m
 }
 
 // Walk of the set mask bits (then the unmasked tail) in walk order, per lane.
+// Each round takes up to kUnroll candidates, so their position loads are in flight
+// together; the next mask word is loaded while the current one is consumed.
 template <class Dist2, class Apply>
 __device__ __forceinline__ void mask_walk(bool valid, uint32_t s, uint32_t m, const SpanTable& st,
                                           const CandMask& mk, const float4* P, uint32_t* q, int cap, float h2,
@@ -416,54 +418,73 @@ __device__ __forceinline__ void mask_walk(bool valid, uint32_t s, uint32_t m, co
             total += sp.y - sp.x;
         }
     }
-    // row cursor: candidate indices [rowStart, rowStart + rowLen) are slots rowLo + (ci - rowStart)
+    // row cursor: candidate indices [rowStart, rowStart + rowLen) are slots rowLo + (ci - rowStart);
+    // candidate indices only grow, so the cursor only moves forward
     int r = -1;
     uint32_t rowStart = 0, rowLen = 0, rowLo = 0;
-    const uint32_t nw = min((total + 31u) >> 5, kMaskWords);
-    uint32_t w = 0, wbase = 0, word = 0, tail = kMaskWords * 32u;
-    bool inWords = true;
+    auto slot_of = [&](uint32_t ci) {
+        while (ci >= rowStart + rowLen) {
+            rowStart += rowLen;
+            ++r;
+            const uint2 sp = st.spans[(uint64_t)r * st.stride + m];
+            rowLo = sp.x;
+            rowLen = sp.y - sp.x;
+        }
+        return rowLo + (ci - rowStart);
+    };
+    const uint32_t nw = valid ? min((total + 31u) >> 5, kMaskWords) : 0u;
+    const uint32_t covered = kMaskWords * 32u;
+    uint32_t w = 0;
+    uint32_t word = nw > 0 ? mk.words[s] : 0u;
+    uint32_t next = nw > 1 ? mk.words[(uint64_t)mk.stride + s] : 0u;
+    uint32_t tail = covered;  // next unmasked candidate index past the mask
     bool active = valid && total > 0;
     while (__any(active)) {
-        if (__any(qn == kQD)) pop_one();
-        if (active && qn < kQD) {
-            bool have = false;
-            uint32_t ci = 0;
-            if (inWords) {
-                if (word == 0) {
-                    if (w < nw) {
-                        word = mk.words[(uint64_t)w * mk.stride + s];
-                        wbase = w * 32u;
-                        ++w;
-                    } else {
-                        inWords = false;
-                    }
-                } else {
-                    ci = wbase + (uint32_t)__builtin_ctz(word);
+        while (__any(qn > kQD - (int)kUnroll)) pop_one();
+        uint32_t ks[kUnroll];
+        uint32_t cnt = 0;
+        if (active) {
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) {
+                while (word == 0u && w + 1 < nw) {
+                    ++w;
+                    word = next;
+                    next = (w + 1 < nw) ? mk.words[(uint64_t)(w + 1) * mk.stride + s] : 0u;
+                }
+                uint32_t ci;
+                if (word != 0u) {
+                    ci = w * 32u + (uint32_t)__ffs((int)word) - 1u;
                     word &= word - 1u;
-                    have = true;
+                } else if (tail < total) {
+                    ci = tail++;
+                } else {
+                    break;
                 }
-            } else if (tail < total) {
-                ci = tail++;
-                have = true;
-            } else {
-                active = false;
+                ks[cnt++] = slot_of(ci);
             }
-            if (have) {
-                while (ci >= rowStart + rowLen) {
-                    rowStart += rowLen;
-                    ++r;
-                    const uint2 sp = st.spans[(uint64_t)r * st.stride + m];
-                    rowLo = sp.x;
-                    rowLen = sp.y - sp.x;
-                }
-                const uint32_t k = rowLo + (ci - rowStart);
-                if (k != s && dist2(P[k]) <= h2) {
-                    q[((qh + qn) & (kQD - 1)) * kBS] = k;
-                    ++qn;
-                    ++found;
-                    if (cap > 0 && found >= cap) active = false;
-                }
-            }
+            if (cnt < kUnroll) active = false;  // every candidate taken
+        }
+        float4 pj[kUnroll];
+#pragma unroll
+        for (uint32_t u = 0; u < kUnroll; ++u) {
+            pj[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (u < cnt) pj[u] = P[ks[u]];
+        }
+        uint32_t bits = 0u;
+#pragma unroll
+        for (uint32_t u = 0; u < kUnroll; ++u)
+            if (u < cnt && ks[u] != s && dist2(pj[u]) <= h2) bits |= 1u << u;
+        if (cap > 0) {
+            const int room = cap - found;
+            while (__popc(bits) > room) bits &= ~(1u << (31 - __clz(bits)));
+            found += __popc(bits);
+            if (found >= cap) active = false;
+        }
+        while (bits != 0u) {  // room for kUnroll pushes was made above
+            const uint32_t t = (uint32_t)__ffs((int)bits) - 1u;
+            bits &= bits - 1u;
+            q[((qh + qn) & (kQD - 1)) * kBS] = ks[t];
+            ++qn;
         }
     }
     while (__any(qn > 0)) pop_one();
