@@ -192,6 +192,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=60)
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch knobs")
     ap.add_argument("--tune-steps", type=int, default=3)
+    ap.add_argument("--tuning", default=None,
+                    help='fixed launch knobs as JSON (e.g. {"candidate_mask": true, "stream_ratio": 1}); '
+                         'skips the autotuner, as profiler runs do to time one setting')
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--grid", default="hashed", choices=["hashed", "compact", "dense"])
@@ -227,7 +230,10 @@ def main():
     sim.step(a.warmup)
     sim.synchronize()
     tuning = None
-    if not a.no_tune:
+    if a.tuning:
+        sim.set_tuning(**json.loads(a.tuning))
+        tuning = {"chosen": sim.tuning(), "fixed": True}
+    elif not a.no_tune:
         from pbfx import tune
         best, best_ms, table = tune.autotune(sim, steps=a.tune_steps, key=tune.workload_key(sim, "dam_break_1m"),
                                              log=lambda m: print(m, file=sys.stderr, flush=True))
