@@ -21,6 +21,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "cuda-pbf-fluid-sim_amd"))
 sys.path.insert(0, REPO)
@@ -133,7 +135,50 @@ def bench_batched(count, warmup, steps, tune_steps=0, grid="dense"):
             "speedup_vs_concurrent_streams": round(t_streams / t_ens, 3), "tuning": best}
 
 
-def bench_adjoint(steps=60, checkpoint=10):
+def bench_adjoint_batched(steps, checkpoint, rollouts=64):
+    """64 differentiable rollouts (each the 64x32x32 configuration, own jitter seed and
+    target velocity field) advanced and differentiated together as one ensemble."""
+    import torch
+    from pbfx.diff import EnsembleRollout, differentiable_scene, occupancy_grid, seeded_velocity
+    scenes = [differentiable_scene(seed=7 + k) for k in range(rollouts)]
+    roll = EnsembleRollout(scenes, steps, checkpoint=checkpoint)
+    try:
+        X0, _ = roll.ensemble.state()
+        n = len(X0)
+        h = float(roll.params.kernel.h)
+        x0 = torch.from_numpy(X0).cuda()
+        vt = torch.from_numpy(np.concatenate([seeded_velocity(sc.num_particles, 1000 + k, 0.5)
+                                              for k, sc in enumerate(scenes)])).cuda()
+        XT, _ = roll.forward(x0, vt)
+        spans = [(f, f + sc.num_particles) for f, sc in zip(roll.firsts, scenes)]
+        targets = [occupancy_grid(XT[a:b, :3].double(), sc.box_min, sc.box_max, cells=32, sigma=h)
+                   for (a, b), sc in zip(spans, scenes)]
+        v0 = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
+        roll.forward(x0, v0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        XT, _ = roll.forward(x0, v0)
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+        xT = XT[:, :3].double().requires_grad_(True)
+        loss = sum(((occupancy_grid(xT[a:b], sc.box_min, sc.box_max, cells=32, sigma=h) - tg) ** 2).sum()
+                   for (a, b), sc, tg in zip(spans, scenes, targets))
+        loss.backward()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        roll.backward(xT.grad.float())
+        torch.cuda.synchronize()
+        t_bwd = time.perf_counter() - t0
+    finally:
+        roll.close()
+    print(f"[bench] {rollouts} batched rollouts, {n} particles: forward {t_fwd:.4f} s, backward {t_bwd:.4f} s",
+          file=sys.stderr, flush=True)
+    return {"rollouts": rollouts, "particles": n, "forward_ms_per_step": round(t_fwd * 1e3 / steps, 4),
+            "backward_over_forward": round(t_bwd / t_fwd, 3),
+            "rollout_steps_per_s": round(rollouts * steps / (t_fwd + t_bwd), 1)}
+
+
+def bench_adjoint(steps=60, checkpoint=10, rollouts=64):
     """The differentiable rollout: 65,536 particles (64x32x32 lattice), 60 steps, checkpoint every
     10.  Loss: squared L2 distance between the 32^3 Gaussian occupancy grid (sigma = h) of X_T and
     that of a target rollout from a seeded perturbed initial velocity.  Times the checkpointed
@@ -173,7 +218,8 @@ def bench_adjoint(steps=60, checkpoint=10):
         roll.close()
     print(f"[bench] adjoint rollout {n} particles x {steps} steps: forward {t_fwd:.4f} s, backward {t_bwd:.4f} s",
           file=sys.stderr, flush=True)
-    return {"particles": n, "steps": steps, "checkpoint": checkpoint, "loss": float(loss.detach()),
+    batched = bench_adjoint_batched(steps, checkpoint) if rollouts > 1 else None
+    return {"particles": n, "steps": steps, "checkpoint": checkpoint, "loss": float(loss.detach()), "batched": batched,
             "forward_ms_per_step": round(t_fwd * 1e3 / steps, 4),
             "record_ms_per_step": round(tm["record_s"] * 1e3 / steps, 4),
             "reverse_ms_per_step": round(tm["reverse_s"] * 1e3 / steps, 4),

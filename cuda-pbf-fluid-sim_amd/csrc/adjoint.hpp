@@ -34,6 +34,7 @@ struct StepTape {
     uint2* lamB[kMaxTapeIters] = {};    // (bits of e0, slot that reached the neighbour cap or ~0)
     float4* dispU[kMaxTapeIters] = {};  // scaled correction before the displacement clamp
     float* xsphGm = nullptr;            // XSPH gate * m / rho0
+    uint32_t* sceneOfSlot = nullptr;    // ensembles: scene of each sorted slot
 };
 
 // Scratch of the reverse pass (sorted order unless noted).
@@ -45,16 +46,19 @@ struct AdjointScratch {
     float4* a = nullptr;      // delta adjoint vectors
     float4* lamQ = nullptr;   // lambda adjoint: (sum of spiky gradients, beta = gLam b0)
     uint2* lamE = nullptr;    // lambda adjoint: (bits of eps = gLam e0, cap cut-off slot)
-    double* partial = nullptr;  // per-block partial sums of dL/dc
+    double* partial = nullptr;  // per-block partial sums of dL/dc (one scene)
+    double* gcSlot = nullptr;   // ensembles: dL/dc contribution of each sorted slot
     uint32_t blocks = 0;
 };
 
 // Reverse of one recorded step.  gXin/gVin: dL/dX_{t+1}, dL/dV_{t+1} (particle
 // order).  gXout/gVout: dL/dX_t, dL/dV_t (particle order; may not alias the
-// inputs).  Adds dL/dc of the step into *gcAccum (device double).
-void adjoint_step(const StepTape& t, const SpanTable& st, const sim::SimParams& p, const float4* gXin,
-                  const float4* gVin, float4* gXout, float4* gVout, double* gcAccum, AdjointScratch& w,
-                  hipStream_t s);
+// inputs).  Adds dL/dc of the step into gcAccum (device doubles: one, or one per
+// scene of an ensemble, whose scene s owns sorted slots [particleBase_s,
+// particleBase_s + n_s) because its keys follow the previous scene's).
+void adjoint_step(const StepTape& t, const SpanTable& st, const sim::SimParams& p, const Scenes& sc,
+                  const float4* gXin, const float4* gVin, float4* gXout, float4* gVout, double* gcAccum,
+                  AdjointScratch& w, hipStream_t s);
 
 // Clamp bits (bit a: axis a was moved by the box clamp) are written by the
 // forward kernels when recording: launch_predict_hash(..., clampBits) in particle

@@ -123,16 +123,26 @@ __device__ __forceinline__ float3 disp_adjoint(float3 g, const float4& e4, const
     return mul3(g, sc);
 }
 
+// XSPH coefficient of a sorted slot: the scene's in an ensemble.
+__device__ __forceinline__ float xsph_c_of(uint32_t s, const sim::DeviceParams& dp, const Scenes& sc,
+                                          const uint32_t* sceneOfSlot) {
+    return sc.params ? sc.params[sceneOfSlot[s]].xsph_c : dp.xsph_c;
+}
+
 __global__ __launch_bounds__(kBS) void k_adj_xsph(const float4* __restrict__ Pf, const float4* __restrict__ Vn,
                                                   const uint32_t* __restrict__ segOf, SpanTable st,
-                                                  sim::DeviceParams dp, uint32_t N, const float* __restrict__ gm,
-                                                  const float4* __restrict__ gVo, float4* __restrict__ gVn,
-                                                  float4* __restrict__ gPf, double* __restrict__ partial) {
+                                                  sim::DeviceParams dp, Scenes sc,
+                                                  const uint32_t* __restrict__ sceneOfSlot, uint32_t N,
+                                                  const float* __restrict__ gm, const float4* __restrict__ gVo,
+                                                  float4* __restrict__ gVn, float4* __restrict__ gPf,
+                                                  double* __restrict__ partial, double* __restrict__ gcSlot) {
     __shared__ double sh[kBS];
     const uint32_t s = blockIdx.x * kBS + threadIdx.x;
     double gcl = 0.0;
     if (s < N) {
-        const float h2 = dp.kernel.h2, poly6 = dp.kernel.poly6, c = dp.xsph_c;
+        const float h2 = dp.kernel.h2, poly6 = dp.kernel.poly6;
+        // neighbours share the slot's scene: rows never leave a scene's key range
+        const float c = xsph_c_of(s, dp, sc, sceneOfSlot);
         const float3 pk = f3(Pf[s]), vk = f3(Vn[s]), gk = f3(gVo[s]);
         const float gmk = gm[s];
         const float kk = c * gmk;
@@ -160,7 +170,9 @@ __global__ __launch_bounds__(kBS) void k_adj_xsph(const float4* __restrict__ Pf,
         gVn[s] = mk4(add3(mul3(gk, 1.0f - kk * sumW), accV), 0.f);
         gPf[s] = mk4(add3(f3(gPf[s]), accP), 0.f);
         gcl = (double)gmk * (double)dot3(gk, dv);
+        if (gcSlot) gcSlot[s] = gcl;
     }
+    if (gcSlot) return;  // ensembles reduce per scene (k_adj_scene_sum)
     sh[threadIdx.x] = gcl;
     __syncthreads();
     for (int w = kBS / 2; w > 0; w >>= 1) {
@@ -175,6 +187,22 @@ __global__ void k_adj_sum(const double* __restrict__ partial, uint32_t n, double
     double t = 0.0;
     for (uint32_t b = 0; b < n; ++b) t += partial[b];
     *accum += t;
+}
+
+// One block per scene: dL/dc of the scene's slots, summed in a fixed order.
+__global__ __launch_bounds__(kBS) void k_adj_scene_sum(const double* __restrict__ gcSlot, Scenes sc,
+                                                       double* __restrict__ accum) {
+    __shared__ double sh[kBS];
+    const SceneParams& sp = sc.params[blockIdx.x];
+    double t = 0.0;
+    for (uint32_t i = threadIdx.x; i < sp.numParticles; i += kBS) t += gcSlot[sp.particleBase + i];
+    sh[threadIdx.x] = t;
+    __syncthreads();
+    for (int w = kBS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) accum[blockIdx.x] += sh[0];
 }
 
 __global__ __launch_bounds__(kBS) void k_adj_velocity(const uint32_t* __restrict__ idxS,
@@ -296,9 +324,9 @@ __global__ __launch_bounds__(kBS) void k_adj_predict(const uint32_t* __restrict_
 
 }  // namespace
 
-void adjoint_step(const StepTape& t, const SpanTable& stIn, const sim::SimParams& p, const float4* gXin,
-                  const float4* gVin, float4* gXout, float4* gVout, double* gcAccum, AdjointScratch& w,
-                  hipStream_t s) {
+void adjoint_step(const StepTape& t, const SpanTable& stIn, const sim::SimParams& p, const Scenes& sc,
+                  const float4* gXin, const float4* gVin, float4* gXout, float4* gVout, double* gcAccum,
+                  AdjointScratch& w, hipStream_t s) {
     const uint32_t N = p.numParticles;
     if (N == 0) return;
     const int I = p.solverIters;
@@ -316,8 +344,17 @@ void adjoint_step(const StepTape& t, const SpanTable& stIn, const sim::SimParams
     st.spans = t.spans;
     const float4* Pf = t.P[I];
     k_adj_gather<<<B, kBS, 0, s>>>(t.idxS, gXin, gVin, w.gPf, w.gVo, N);
-    k_adj_xsph<<<B, kBS, 0, s>>>(Pf, t.Vs, t.segOf, st, dp, N, t.xsphGm, w.gVo, w.gVn, w.gPf, w.partial);
-    k_adj_sum<<<1, 1, 0, s>>>(w.partial, B, gcAccum);
+    const bool ens = sc.params != nullptr;
+    if (ens && (!t.sceneOfSlot || !w.gcSlot)) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "adjoint_step: ensemble tape without scene slots");
+        return;
+    }
+    k_adj_xsph<<<B, kBS, 0, s>>>(Pf, t.Vs, t.segOf, st, dp, sc, t.sceneOfSlot, N, t.xsphGm, w.gVo, w.gVn, w.gPf,
+                                 w.partial, ens ? w.gcSlot : nullptr);
+    if (ens)
+        k_adj_scene_sum<<<sc.count, kBS, 0, s>>>(w.gcSlot, sc, gcAccum);
+    else
+        k_adj_sum<<<1, 1, 0, s>>>(w.partial, B, gcAccum);
     k_adj_velocity<<<B, kBS, 0, s>>>(t.idxS, t.finalClamp, w.gPf, w.gVn, t.dispU[I - 1], dp, 1.0f / p.dt, N, w.gP,
                                      w.a, gXout);
     for (int it = I - 1; it >= 0; --it) {

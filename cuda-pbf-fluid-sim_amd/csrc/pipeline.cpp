@@ -815,7 +815,7 @@ bool Simulator::densityStats(double* out5) {
 void Simulator::releaseAdjoint() {
     for (auto& t : m_tapes) {
         dfree(t.X); dfree(t.V); dfree(t.idxS); dfree(t.segOf); dfree(t.spans); dfree(t.Vs);
-        dfree(t.predClamp); dfree(t.finalClamp); dfree(t.xsphGm);
+        dfree(t.predClamp); dfree(t.finalClamp); dfree(t.xsphGm); dfree(t.sceneOfSlot);
         for (auto& q : t.P) dfree(q);
         for (auto& q : t.lamA) dfree(q);
         for (auto& q : t.lamB) dfree(q);
@@ -823,15 +823,15 @@ void Simulator::releaseAdjoint() {
     }
     m_tapes.clear();
     dfree(m_adj.gPf); dfree(m_adj.gVo); dfree(m_adj.gVn); dfree(m_adj.gP); dfree(m_adj.a);
-    dfree(m_adj.lamQ); dfree(m_adj.lamE); dfree(m_adj.partial);
+    dfree(m_adj.lamQ); dfree(m_adj.lamE); dfree(m_adj.partial); dfree(m_adj.gcSlot);
     m_adj = pbfx::AdjointScratch{};
 }
 
 bool Simulator::adjointInit(int slots) {
     releaseAdjoint();
-    if (!m_initialized || slots < 1 || m_opt.path != PathKind::Reordered || !m_scenes.empty() ||
+    if (!m_initialized || slots < 1 || m_opt.path != PathKind::Reordered ||
         m_params.solverIters < 1 || m_params.solverIters > pbfx::kMaxTapeIters) {
-        set_error(PBFX_E_INVALID_ARGUMENT, "adjointInit: needs a single-scene reordered simulator, 1..8 iterations");
+        set_error(PBFX_E_INVALID_ARGUMENT, "adjointInit: needs a reordered simulator, 1..8 iterations");
         return false;
     }
     const uint32_t cap = m_bufs.capacity;
@@ -842,6 +842,7 @@ bool Simulator::adjointInit(int slots) {
         ok = ok && dalloc(t.X, cap) && dalloc(t.V, cap) && dalloc(t.idxS, cap) && dalloc(t.segOf, cap) &&
              dalloc(t.spans, spanWords) && dalloc(t.Vs, cap) && dalloc(t.predClamp, cap) &&
              dalloc(t.finalClamp, cap) && dalloc(t.xsphGm, cap);
+        if (!m_scenes.empty()) ok = ok && dalloc(t.sceneOfSlot, cap);
         for (int it = 0; ok && it <= m_params.solverIters; ++it) ok = dalloc(t.P[it], cap);
         for (int it = 0; ok && it < m_params.solverIters; ++it)
             ok = dalloc(t.lamA[it], cap) && dalloc(t.lamB[it], cap) && dalloc(t.dispU[it], cap);
@@ -850,6 +851,7 @@ bool Simulator::adjointInit(int slots) {
     ok = ok && dalloc(m_adj.gPf, cap) && dalloc(m_adj.gVo, cap) && dalloc(m_adj.gVn, cap) && dalloc(m_adj.gP, cap) &&
          dalloc(m_adj.a, cap) && dalloc(m_adj.lamQ, cap) && dalloc(m_adj.lamE, cap) &&
          dalloc(m_adj.partial, m_adj.blocks);
+    if (!m_scenes.empty()) ok = ok && dalloc(m_adj.gcSlot, cap);
     if (!ok) {
         releaseAdjoint();
         return false;
@@ -875,6 +877,9 @@ bool Simulator::stepRecord(int slot) {
          hipMemcpyAsync(t.segOf, m_grid.d_segOf, b1, hipMemcpyDeviceToDevice, m_stream) == hipSuccess &&
          hipMemcpyAsync(t.spans, m_grid.spans.spans, sizeof(uint2) * m_grid.spans.rows * m_grid.spans.stride,
                         hipMemcpyDeviceToDevice, m_stream) == hipSuccess;
+    if (t.sceneOfSlot)
+        ok = ok && hipMemcpyAsync(t.sceneOfSlot, m_grid.scenes.sceneOfSlot, b1, hipMemcpyDeviceToDevice,
+                                  m_stream) == hipSuccess;
     if (!ok) {
         set_error(PBFX_E_INVALID_ARGUMENT, "stepRecord: tape copy failed");
         return false;
@@ -895,7 +900,8 @@ bool Simulator::backwardStep(int slot, const float4* gXin, const float4* gVin, f
         set_error(PBFX_E_ALIAS, "backwardStep: outputs alias inputs");
         return false;
     }
-    pbfx::adjoint_step(m_tapes[slot], m_grid.spans, m_params, gXin, gVin, gXout, gVout, gcAccum, m_adj, m_stream);
+    pbfx::adjoint_step(m_tapes[slot], m_grid.spans, m_params, m_grid.scenes, gXin, gVin, gXout, gVout, gcAccum,
+                       m_adj, m_stream);
     return true;
 }
 

@@ -40,16 +40,19 @@ def _as4(t):
 class Rollout:
     """T-step forward on the GPU, checkpointed every `checkpoint` steps, with a reverse pass."""
 
-    def __init__(self, params, steps, checkpoint=10, grid="dense", table_log2=None):
+    def __init__(self, params, steps, checkpoint=10, grid="dense", table_log2=None, sim=None, scenes=1):
         self.params = params
         self.steps = int(steps)
         self.ck = max(1, int(checkpoint))
-        tl = table_log2 or max(16, int(math.ceil(math.log2(max(params.numParticles, 1)))) + 1)
-        self.sim = Simulator(params, grid=grid, table_log2=tl, use_graph=False)
+        if sim is None:
+            tl = table_log2 or max(16, int(math.ceil(math.log2(max(params.numParticles, 1)))) + 1)
+            sim = Simulator(params, grid=grid, table_log2=tl, use_graph=False)
+        self.sim = sim
+        self.scenes = int(scenes)  # dL/dc has one entry per scene
         if _lib.lib().pbfSimAdjointInit(self.sim._h, self.ck) != 0:
             _lib.check("pbfSimAdjointInit")
             raise _lib.PbfError("adjoint init failed")
-        self.n = int(params.numParticles)
+        self.n = int(sim.n)
         self._ckpts = []
         self.timed = False    # when set, backward() synchronises between its phases and fills `timing`
         self.timing = {}
@@ -64,6 +67,8 @@ class Rollout:
         return time.perf_counter()
 
     def _set_c(self, c):
+        if c is None:
+            return
         if float(f32(c)) != self.params.xsph_c:
             self.params.xsph_c = float(c)
             if _lib.lib().pbfSimSetParams(self.sim._h, ctypes.byref(self.params)) != 0:
@@ -103,14 +108,15 @@ class Rollout:
         return self._get_state()
 
     def backward(self, gX, gV=None):
-        """dL/dX_0, dL/dV_0 (N, 4 float32) and dL/dc (float) from dL/dX_T and dL/dV_T."""
+        """dL/dX_0, dL/dV_0 (N, 4 float32) and dL/dc from dL/dX_T and dL/dV_T (a float, or
+        one float per scene for an ensemble rollout)."""
         L = _lib.lib()
         gX = _as4(gX).clone()
         gV = torch.zeros_like(gX) if gV is None else _as4(gV).clone()
         gX[:, 3] = 0
         gV[:, 3] = 0
         oX, oV = torch.empty_like(gX), torch.empty_like(gV)
-        gc = torch.zeros(1, dtype=torch.float64, device="cuda")
+        gc = torch.zeros(self.scenes, dtype=torch.float64, device="cuda")
         starts = list(range(0, self.steps, self.ck))
         t_rec = t_rev = 0.0
         for seg in reversed(range(len(starts))):
@@ -135,7 +141,25 @@ class Rollout:
             self.timing = {"record_s": t_rec, "reverse_s": t_rev}
         self.sim.synchronize()
         _lib.check("Rollout.backward")
-        return gX, gV, float(gc.item())
+        if self.scenes == 1:
+            return gX, gV, float(gc.item())
+        return gX, gV, gc.cpu().numpy()
+
+
+class EnsembleRollout(Rollout):
+    """Many rollouts advanced together as one batched ensemble (one launch per phase for
+    all scenes, forward and reverse); dL/dc comes back per scene.  The XSPH coefficient
+    of each scene is the one in its scene description."""
+
+    def __init__(self, scenes, steps, checkpoint=10, grid="dense"):
+        from .ensemble import Ensemble
+        ens = Ensemble(scenes, grid=grid, use_graph=False)
+        super().__init__(ens.base, steps, checkpoint, sim=ens, scenes=len(scenes))
+        self.ensemble = ens
+        self.firsts = [ens.scene_info(k)[0] for k in range(len(scenes))]
+
+    def forward(self, x0, v0, c=None):
+        return super().forward(x0, v0, None)
 
 
 class PBFRollout(torch.autograd.Function):

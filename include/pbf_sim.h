@@ -63,7 +63,7 @@ int pbfSimDensityStats(PbfSim* sim, double* out5);
 // 1 when a hash-table insert found no free slot since creation.
 int pbfSimHashOverflow(PbfSim* sim);
 
-// ---- differentiable rollout (single-scene reordered simulators) -------------
+// ---- differentiable rollout (reordered simulators and ensembles) ------------
 // Hand-written adjoint of the step.  A recorded step keeps the sort permutation,
 // neighbour spans, clamp bits and every solver iteration's positions and lambdas;
 // the reverse pass differentiates the step with those decisions frozen.  No
@@ -74,7 +74,8 @@ int pbfSimAdjointInit(PbfSim* sim, int tapeSlots);
 int pbfSimStepRecord(PbfSim* sim, int slot);
 // Reverse of the step recorded in `slot`.  gXin/gVin = dL/dX_{t+1}, dL/dV_{t+1};
 // gXout/gVout = dL/dX_t, dL/dV_t (device float4, particle order, w ignored; outputs
-// must not alias inputs).  Adds dL/d(xsph_c) of the step to *gcAccum (device double).
+// must not alias inputs).  Adds dL/d(xsph_c) of the step to gcAccum (device doubles:
+// one for a simulator, one per scene for an ensemble handle).
 int pbfSimBackwardStep(PbfSim* sim, int slot, const float4* gXin, const float4* gVin, float4* gXout, float4* gVout,
                        double* gcAccum);
 
