@@ -366,16 +366,23 @@ def main():
     if not a.no_cpu_baseline:
         from oracle import oracle as O
         Xc = P.seed_scene_reference(scene, O.seed_lattice)
-        import numpy as np
         Vc = np.zeros_like(Xc)
         Xc, Vc = O.run(Xc, Vc, sp, 1, compact=True)  # warm-up step
+        cores = O.num_threads()
         t0 = time.perf_counter()
         O.run(Xc, Vc, sp, a.cpu_steps, compact=True)
         cpu_s = time.perf_counter() - t0
+        O.set_num_threads(1)
+        t0 = time.perf_counter()
+        O.run(Xc, Vc, sp, 1, compact=True)
+        cpu1_s = time.perf_counter() - t0
+        O.set_num_threads(cores)
         out["cpu_baseline"] = {"value": round(N * a.cpu_steps / cpu_s, 1), "unit": "particle-steps/s",
-                               "cores": O.num_threads(), "kind": "port",
+                               "cores": cores, "kind": "port",
                                "sample": f"dam_break_1m, compact grid, steps 2..{1 + a.cpu_steps} after one "
-                                         f"warm-up step, {cpu_s:.2f} s, OpenMP over particles"}
+                                         f"warm-up step, {cpu_s:.2f} s, OpenMP over particles",
+                               "single_thread_value": round(N / cpu1_s, 1),
+                               "single_thread_sample": f"step 2 on one thread, {cpu1_s:.2f} s"}
     print(json.dumps(out))
     return 0
 

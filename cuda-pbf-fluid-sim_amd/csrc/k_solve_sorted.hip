@@ -57,7 +57,15 @@ __global__ __launch_bounds__(kBS) void k_predict_hash(const float4* __restrict__
     keys[i] = keyBase + cell_key(px, py, pz, grid);
 }
 
-__device__ __forceinline__ uint32_t table_home(uint32_t key, uint32_t mask) { return key & mask; }
+// Home slot of a cell key.  While the key space fits the table the home is the key
+// itself, so consecutive keys (the cells of an x-window) share cache lines.  Keys past
+// the table size are shifted by a multiplicative hash of their high bits, so key blocks
+// that wrap around the table do not all land on the same slots.
+__device__ __forceinline__ uint32_t table_home(uint32_t key, uint32_t mask) {
+    const int L = __popc(mask);  // mask = 2^L - 1
+    const uint32_t hi = L >= 32 ? 0u : (key >> L);
+    return (key + hi * 0x9E3779B1u) & mask;
+}
 
 __global__ __launch_bounds__(kBS) void k_hash_insert(HashTable t, const uint32_t* __restrict__ uniq,
                                                      const uint32_t* __restrict__ off,

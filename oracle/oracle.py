@@ -71,6 +71,7 @@ def _declare(L):
         "oracle_jitter_uniform": (_F, [ctypes.c_uint64, ctypes.c_uint64, _U32]),
         "oracle_pow_scorr": (_F, [_F, _F]),
         "oracle_num_threads": (ctypes.c_int, []),
+        "oracle_set_num_threads": (None, [ctypes.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -274,6 +275,10 @@ def pow_scorr(x, n):
 
 def num_threads():
     return lib().oracle_num_threads()
+
+
+def set_num_threads(n):
+    lib().oracle_set_num_threads(int(n))
 
 
 # ------------------------------------------------------------ scene generators

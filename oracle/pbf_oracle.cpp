@@ -656,6 +656,14 @@ void oracle_seed_lattice(of4* X, uint32_t nx, uint32_t ny, uint32_t nz, float ox
 
 float oracle_pow_scorr(float x, float n) { return pow_scorr(x, n); }
 
+void oracle_set_num_threads(int n) {
+#ifdef _OPENMP
+    if (n >= 1) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 int oracle_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

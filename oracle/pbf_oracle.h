@@ -119,6 +119,8 @@ float oracle_jitter_uniform(uint64_t seed, uint64_t particle, uint32_t axis);
 float oracle_pow_scorr(float x, float n);
 
 int oracle_num_threads(void);
+/* Threads the OpenMP loops use from now on (n >= 1). */
+void oracle_set_num_threads(int n);
 
 #ifdef __cplusplus
 }
