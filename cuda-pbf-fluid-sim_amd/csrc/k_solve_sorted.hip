@@ -301,18 +301,19 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, uint32_t m
             uint32_t k = lo;
             while (__any(k < hi)) {
                 const uint32_t cnt = k < hi ? min(kUnroll, hi - k) : 0u;
+                // loads past the span read its last slot (slot 0 for an empty lane), so
+                // every load is in bounds and none needs a branch
+                const uint32_t last = hi > 0u ? hi - 1u : 0u;
                 float4 pj[kUnroll];
 #pragma unroll
-                for (uint32_t u = 0; u < kUnroll; ++u) {
-                    pj[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-                    if (u < cnt) pj[u] = P[k + u];
-                }
+                for (uint32_t u = 0; u < kUnroll; ++u) pj[u] = P[min(k + u, last)];
                 uint32_t bits = 0u, skin = 0u;
 #pragma unroll
                 for (uint32_t u = 0; u < kUnroll; ++u) {
                     const float r2 = dist2(pj[u]);
-                    if (u < cnt && r2 <= h2 && k + u != s) bits |= 1u << u;
-                    if (kBuild && u < cnt && r2 <= mk.rs2) skin |= 1u << u;
+                    const uint32_t own = (uint32_t)(u < cnt) & (uint32_t)(k + u != s);
+                    bits |= (own & (uint32_t)(r2 <= h2)) << u;
+                    if (kBuild) skin |= ((uint32_t)(u < cnt) & (uint32_t)(r2 <= mk.rs2)) << u;
                 }
                 if (kBuild && cnt > 0) {
                     const uint32_t sh = cidx & 31u;
@@ -449,50 +450,55 @@ __device__ __forceinline__ void mask_walk(bool valid, uint32_t s, uint32_t m, co
     bool active = valid && total > 0;
     while (__any(active)) {
         while (__any(qn > kQD - (int)kUnroll)) pop_one();
+        // ks[u] for u >= cnt repeats the last taken slot (slot 0 when none was taken), so
+        // every load below is in bounds and branch-free
         uint32_t ks[kUnroll];
-        uint32_t cnt = 0;
-        if (active) {
+        uint32_t cnt = 0, prev = 0u;
+        bool more = active;
 #pragma unroll
-            for (uint32_t u = 0; u < kUnroll; ++u) {
+        for (uint32_t u = 0; u < kUnroll; ++u) {
+            if (more) {
                 while (word == 0u && w + 1 < nw) {
                     ++w;
                     word = next;
                     next = (w + 1 < nw) ? mk.words[(uint64_t)(w + 1) * mk.stride + s] : 0u;
                 }
-                uint32_t ci;
+                uint32_t ci = 0u;
                 if (word != 0u) {
                     ci = w * 32u + (uint32_t)__ffs((int)word) - 1u;
                     word &= word - 1u;
                 } else if (tail < total) {
                     ci = tail++;
                 } else {
-                    break;
+                    more = false;
                 }
-                ks[cnt++] = slot_of(ci);
+                if (more) {
+                    prev = slot_of(ci);
+                    ++cnt;
+                }
             }
-            if (cnt < kUnroll) active = false;  // every candidate taken
+            ks[u] = prev;
         }
+        if (cnt < kUnroll) active = false;  // every candidate taken
         float4 pj[kUnroll];
 #pragma unroll
-        for (uint32_t u = 0; u < kUnroll; ++u) {
-            pj[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (u < cnt) pj[u] = P[ks[u]];
-        }
+        for (uint32_t u = 0; u < kUnroll; ++u) pj[u] = P[ks[u]];
         uint32_t bits = 0u;
 #pragma unroll
         for (uint32_t u = 0; u < kUnroll; ++u)
-            if (u < cnt && ks[u] != s && dist2(pj[u]) <= h2) bits |= 1u << u;
+            bits |= ((uint32_t)(u < cnt) & (uint32_t)(ks[u] != s) & (uint32_t)(dist2(pj[u]) <= h2)) << u;
         if (cap > 0) {
             const int room = cap - found;
             while (__popc(bits) > room) bits &= ~(1u << (31 - __clz(bits)));
             found += __popc(bits);
             if (found >= cap) active = false;
         }
-        while (bits != 0u) {  // room for kUnroll pushes was made above
-            const uint32_t t = (uint32_t)__ffs((int)bits) - 1u;
-            bits &= bits - 1u;
-            q[((qh + qn) & (kQD - 1)) * kBS] = ks[t];
-            ++qn;
+#pragma unroll
+        for (uint32_t u = 0; u < kUnroll; ++u) {  // room for kUnroll pushes was made above
+            if ((bits >> u) & 1u) {
+                q[((qh + qn) & (kQD - 1)) * kBS] = ks[u];
+                ++qn;
+            }
         }
     }
     while (__any(qn > 0)) pop_one();
