@@ -199,15 +199,15 @@ struct HashRowsDev {
 // ------------------------------------------------------------------------
 // Neighbour walk of one wave (64 consecutive sorted slots, one particle per lane).
 //
-// Candidate stream.  For each neighbour row the wave takes the union of its
-// lanes' slot spans; when that union is not much longer than the longest lane
-// span (dense cells, many particles per cell), the wave streams it: 64
-// candidates at a time are loaded coalesced into a per-wave LDS buffer, then
-// every lane reads each candidate as an LDS broadcast and tests it against its
-// own span, itself and the distance h.  The trip count is wave-uniform, so the
-// scan does not diverge.  Rows whose union is much longer than the lane spans
-// (sparse cells) are walked per lane instead, reading candidates from global
-// memory.
+// Two walk modes per neighbour row, chosen wave-uniformly.  Stream: when the union
+// of the lanes' slot spans is at most SpanTable::streamRatio times the longest
+// lane span, the wave streams the union through a per-wave LDS buffer, 64
+// candidates at a time loaded coalesced, and every lane tests each candidate (an
+// LDS broadcast) against its own span, itself and the distance h; with chunkBits
+// it tests a whole chunk into a bit mask first.  Per lane: otherwise each lane
+// walks its own span from global memory, kUnroll candidates a round so their
+// loads are in flight together.  When a wave spans several cells the union carries
+// candidates most lanes do not need, and the per-lane walk is faster.
 //
 // Deferred pairs.  In-range slots go to a per-lane FIFO in LDS; the expensive
 // per-pair work (sqrt, IEEE divisions, the s_corr power) runs when some lane's

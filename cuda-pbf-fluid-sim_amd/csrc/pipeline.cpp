@@ -207,8 +207,9 @@ void SortedSolve(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream
 
 void SortedPost(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_t s) {
     const int fin = p.solverIters > 0 ? (p.solverIters & 1) : 0;
-    // XSPH's per-pair work is light, so its scan is cheaper as the coalesced LDS
-    // stream than as the per-lane walk of the candidate mask (measured)
+    // XSPH walks its candidates directly: each of its pairs also gathers a velocity,
+    // and the scattered loads of the candidate-mask walk cost more than the
+    // consecutive candidates the mask would skip
     pbfx::CandMask mk = g.mask;
     const bool useMask = false;
     const bool rec = b.rec.P[0] != nullptr;
