@@ -83,13 +83,15 @@ class _Snapshot:
             raise _lib.PbfError("restore failed")
 
 
-def autotune(sim, steps=3, sweeps=2, space=None, key=None, path=DEFAULT_CACHE, log=None):
+def autotune(sim, steps=3, sweeps=2, space=None, key=None, path=DEFAULT_CACHE, log=None, snapshot=None,
+             clock=time.perf_counter):
     """Time `steps` steps per setting from the current state and keep the fastest.
 
     Returns (best setting, ms per step of it, [(setting, ms per step), ...]).  The
-    simulator's state is the same before and after the call."""
+    simulator's state is the same before and after the call.  `snapshot(sim)` makes an
+    object whose restore() puts the state back (default: a device copy)."""
     space = space or SPACE
-    snap = _Snapshot(sim)
+    snap = (snapshot or _Snapshot)(sim)
     timed = {}
 
     def measure(cfg):
@@ -101,10 +103,10 @@ def autotune(sim, steps=3, sweeps=2, space=None, key=None, path=DEFAULT_CACHE, l
         sim.step(1)              # first step after a change also captures the graph
         sim.synchronize()
         snap.restore()
-        t0 = time.perf_counter()
+        t0 = clock()
         sim.step(steps)
         sim.synchronize()
-        ms = (time.perf_counter() - t0) * 1e3 / steps
+        ms = (clock() - t0) * 1e3 / steps
         timed[k] = ms
         if log:
             log(f"[tune] {cfg} {ms:.3f} ms/step")
