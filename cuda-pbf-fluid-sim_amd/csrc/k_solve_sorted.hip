@@ -220,6 +220,7 @@ struct HashRowsDev {
 // every sum is formed in the reference's order.
 constexpr int kQD = 16;          // FIFO depth per lane
 constexpr uint32_t kUnroll = 4;  // candidates per round of a lane's own walk
+static_assert(kSlotPad + 1 >= kUnroll, "a round may load kUnroll - 1 slots past the last particle");
 
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
 #pragma unroll
@@ -266,6 +267,14 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, uint32_t m
             apply(k);
         }
     };
+    // pops until no lane holds more than `keep` pairs (written as a do-while: the
+    // compiler then keeps the loop-carried sums in place instead of copying them
+    // on every pop)
+    auto pop_above = [&](int keep) {
+        if (__any(qn > keep)) do {
+                pop_one();
+            } while (__any(qn > keep));
+    };
     // one own candidate: record its skin bit, queue it when in range
     auto visit = [&](uint32_t k, float r2, uint32_t& hi, uint32_t lo) {
         if (kBuild) {
@@ -301,20 +310,22 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, uint32_t m
             uint32_t k = lo;
             while (__any(k < hi)) {
                 const uint32_t cnt = k < hi ? min(kUnroll, hi - k) : 0u;
-                // loads past the span read its last slot (slot 0 for an empty lane), so
-                // every load is in bounds and none needs a branch
-                const uint32_t last = hi > 0u ? hi - 1u : 0u;
+                // k <= hi <= N, so the loads reach at most slot N + kUnroll - 2, inside
+                // the kSlotPad slots past the last particle; tests past cnt are masked off
                 float4 pj[kUnroll];
 #pragma unroll
-                for (uint32_t u = 0; u < kUnroll; ++u) pj[u] = P[min(k + u, last)];
+                for (uint32_t u = 0; u < kUnroll; ++u) pj[u] = P[k + u];
+                const uint32_t inSpan = (1u << cnt) - 1u;
+                const uint32_t self = s - k < kUnroll ? 1u << (s - k) : 0u;
                 uint32_t bits = 0u, skin = 0u;
 #pragma unroll
                 for (uint32_t u = 0; u < kUnroll; ++u) {
                     const float r2 = dist2(pj[u]);
-                    const uint32_t own = (uint32_t)(u < cnt) & (uint32_t)(k + u != s);
-                    bits |= (own & (uint32_t)(r2 <= h2)) << u;
-                    if (kBuild) skin |= ((uint32_t)(u < cnt) & (uint32_t)(r2 <= mk.rs2)) << u;
+                    bits |= (r2 <= h2) ? 1u << u : 0u;
+                    if (kBuild) skin |= (r2 <= mk.rs2) ? 1u << u : 0u;
                 }
+                bits &= inSpan & ~self;
+                skin &= inSpan;
                 if (kBuild && cnt > 0) {
                     const uint32_t sh = cidx & 31u;
                     uint64_t w = (uint64_t)word | ((uint64_t)skin << sh);
@@ -332,16 +343,15 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, uint32_t m
                     found += __popc(bits);
                     if (found >= cap) hi = k;  // nothing more to take in this row
                 }
-                k += cnt;
-                while (__any(bits != 0u)) {
-                    if (__any(qn == kQD)) pop_one();
-                    if (bits != 0u && qn < kQD) {
-                        const uint32_t t = (uint32_t)__ffs((int)bits) - 1u;
-                        bits &= bits - 1u;
-                        q[((qh + qn) & (kQD - 1)) * kBS] = k - cnt + t;
+                pop_above(kQD - (int)kUnroll);
+#pragma unroll
+                for (uint32_t u = 0; u < kUnroll; ++u) {  // room for kUnroll pushes was made above
+                    if ((bits >> u) & 1u) {
+                        q[((qh + qn) & (kQD - 1)) * kBS] = k + u;
                         ++qn;
                     }
                 }
+                k += cnt;
             }
         } else if (kBuild || st.chunkBits == 0) {
             // dense row, candidate by candidate (and building the candidate mask):
@@ -401,7 +411,7 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, uint32_t m
         const uint32_t w = cidx >> 5;
         if (w < kMaskWords) mk.words[(uint64_t)w * mk.stride + s] = word;
     }
-    while (__any(qn > 0)) pop_one();
+    pop_above(0);
 }
 
 // Walk of the set mask bits (then the unmasked tail) in walk order, per lane.
@@ -419,6 +429,14 @@ __device__ __forceinline__ void mask_walk(bool valid, uint32_t s, uint32_t m, co
             --qn;
             apply(k);
         }
+    };
+    // pops until no lane holds more than `keep` pairs (written as a do-while: the
+    // compiler then keeps the loop-carried sums in place instead of copying them
+    // on every pop)
+    auto pop_above = [&](int keep) {
+        if (__any(qn > keep)) do {
+                pop_one();
+            } while (__any(qn > keep));
     };
     uint32_t total = 0;
     if (valid) {
@@ -449,7 +467,7 @@ __device__ __forceinline__ void mask_walk(bool valid, uint32_t s, uint32_t m, co
     uint32_t tail = covered;  // next unmasked candidate index past the mask
     bool active = valid && total > 0;
     while (__any(active)) {
-        while (__any(qn > kQD - (int)kUnroll)) pop_one();
+        pop_above(kQD - (int)kUnroll);
         // ks[u] for u >= cnt repeats the last taken slot (slot 0 when none was taken), so
         // every load below is in bounds and branch-free
         uint32_t ks[kUnroll];
@@ -501,7 +519,7 @@ __device__ __forceinline__ void mask_walk(bool valid, uint32_t s, uint32_t m, co
             }
         }
     }
-    while (__any(qn > 0)) pop_one();
+    pop_above(0);
 }
 
 // Lambda of a sorted slot: the full walk (first iteration) or the mask walk.

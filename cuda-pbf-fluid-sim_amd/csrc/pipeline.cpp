@@ -248,8 +248,9 @@ bool DeviceBuffers::allocate(uint32_t cap) {
     release();
     if (cap == 0) cap = 1;
     bool ok = dalloc(d_pos_curr, cap) && dalloc(d_pos_next, cap) && dalloc(d_vel, cap) && dalloc(d_lambda, cap) &&
-              dalloc(d_delta, cap) && dalloc(d_vel_snapshot, cap) && dalloc(d_pos_sorted[0], cap) &&
-              dalloc(d_pos_sorted[1], cap) && dalloc(d_vel_sorted, cap);
+              dalloc(d_delta, cap) && dalloc(d_vel_snapshot, cap) &&
+              dalloc(d_pos_sorted[0], (size_t)cap + pbfx::kSlotPad) &&
+              dalloc(d_pos_sorted[1], (size_t)cap + pbfx::kSlotPad) && dalloc(d_vel_sorted, cap);
     if (!ok) {
         release();
         return false;
@@ -260,6 +261,8 @@ bool DeviceBuffers::allocate(uint32_t cap) {
     (void)hipMemset(d_pos_curr, 0, sizeof(float4) * cap);
     (void)hipMemset(d_pos_next, 0, sizeof(float4) * cap);
     (void)hipMemset(d_vel, 0, sizeof(float4) * cap);
+    (void)hipMemset(d_pos_sorted[0], 0, sizeof(float4) * ((size_t)cap + pbfx::kSlotPad));
+    (void)hipMemset(d_pos_sorted[1], 0, sizeof(float4) * ((size_t)cap + pbfx::kSlotPad));
     return true;
 }
 
@@ -844,7 +847,10 @@ bool Simulator::adjointInit(int slots) {
              dalloc(t.spans, spanWords) && dalloc(t.Vs, cap) && dalloc(t.predClamp, cap) &&
              dalloc(t.finalClamp, cap) && dalloc(t.xsphGm, cap);
         if (!m_scenes.empty()) ok = ok && dalloc(t.sceneOfSlot, cap);
-        for (int it = 0; ok && it <= m_params.solverIters; ++it) ok = dalloc(t.P[it], cap);
+        for (int it = 0; ok && it <= m_params.solverIters; ++it) {
+            ok = dalloc(t.P[it], (size_t)cap + pbfx::kSlotPad);
+            if (ok) (void)hipMemset(t.P[it], 0, sizeof(float4) * ((size_t)cap + pbfx::kSlotPad));
+        }
         for (int it = 0; ok && it < m_params.solverIters; ++it)
             ok = dalloc(t.lamA[it], cap) && dalloc(t.lamB[it], cap) && dalloc(t.dispU[it], cap);
     }

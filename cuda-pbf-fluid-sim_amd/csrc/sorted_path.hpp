@@ -45,6 +45,11 @@ struct SpanTable {
     uint32_t chunkBits = 1;
 };
 
+// Sorted position arrays hold this many slots past the last particle, so that the
+// neighbour walks may load a few candidates past the end of a span without a bounds
+// select (the loaded values are never used).
+constexpr uint32_t kSlotPad = 4;
+
 // Candidate mask words per particle: the mask covers the first 32 * kCandMaskWords
 // candidates of a particle's walk; later candidates are tested directly.
 constexpr uint32_t kCandMaskWords = 64;
