@@ -630,7 +630,10 @@ __global__ __launch_bounds__(kBS) void k_finalize_sorted(float4* __restrict__ Ps
     Vs[s] = make_float4((px - x.x) * invDt, (py - x.y) * invDt, (pz - x.z) * invDt, 0.0f);
 }
 
-__global__ __launch_bounds__(kBS) void k_xsph_scatter(const float4* __restrict__ Ps, const float4* __restrict__ Vs,
+// Eight waves per SIMD (64 VGPRs): the walk is latency-bound on its candidate and
+// velocity loads, and the extra waves hide more of it than the few spilled registers
+// cost (measured on the collapsed 1M dam break).
+__global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(8))) void k_xsph_scatter(const float4* __restrict__ Ps, const float4* __restrict__ Vs,
                                                       const uint32_t* __restrict__ segOf, SpanTable st,
                                                       CandMask mk, int useMask, sim::DeviceParams dp, uint32_t N,
                                                       const uint32_t* __restrict__ idxS, float4* __restrict__ Xn,
