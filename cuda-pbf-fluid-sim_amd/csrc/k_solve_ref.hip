@@ -6,95 +6,120 @@
 // contiguous span of sorted slots (its cells have consecutive keys), found with
 // at most 2*(2*reach+1) range loads (dense) or one lower_bound (compact), and
 // walked in the reference's z -> y -> x, ascending-slot order, so sums are
-// formed in the same order as the reference's per-cell loops.
+// formed in the same order as the reference's per-cell loops.  The walk is the
+// cell-reordered kernels' wave walk (walk.hpp: unrolled per-lane candidate loads,
+// in-range pairs deferred through a per-lane FIFO), with each candidate's position
+// read through the sorted index array.
 // The headless pipeline uses the cell-reordered kernels in k_solve_sorted.hip;
 // these exist so the sim/ phase API can be served unchanged.
 #include "pbf_common.hpp"
 #include "pbf_hip.h"
 #include "pbf_math.hpp"
+#include "walk.hpp"
 
 namespace pbfx {
 namespace {
 
-constexpr int kBS = 256;
+using namespace walk;
 
-// Calls fn(lo, hi) for every non-empty neighbour row of the cell `key`, in
-// reference order; fn returns false to stop.
-template <class Rows, class Fn>
-__device__ __forceinline__ void for_each_row(uint32_t key, const sim::GridBounds& g, int reach, const Rows& rows,
-                                             Fn&& fn) {
-    int cx, cy, cz;
-    decode_key(key, g.dim, cx, cy, cz);
-    const int x0 = max(0, cx - reach), x1 = min(g.dim.x - 1, cx + reach);
-    if (x0 > x1) return;
-    for (int dz = -reach; dz <= reach; ++dz) {
-        const int z = cz + dz;
-        if (z < 0 || z >= g.dim.z) continue;
-        for (int dy = -reach; dy <= reach; ++dy) {
-            const int y = cy + dy;
-            if (y < 0 || y >= g.dim.y) continue;
-            const uint32_t rowBase = (uint32_t)((z * g.dim.y + y) * g.dim.x);
-            uint32_t lo, hi;
-            rows.span(rowBase + (uint32_t)x0, rowBase + (uint32_t)x1, lo, hi);
-            if (lo < hi && !fn(lo, hi)) return;
-        }
+// Neighbour rows of the cell `key`, computed from the cell lookup the launcher is
+// handed (dense cellStart/cellEnd or the compact unique keys): row r = (dz, dy) is one
+// contiguous span of sorted slots because its cells have consecutive keys.
+template <class Lookup>
+struct CellRows {
+    Lookup lookup;
+    sim::GridBounds g;
+    int reach, side, cx, cy, cz, x0, x1;
+    // the drop-in launchers take no tuning: lanes walk their own spans, and a row
+    // is streamed through LDS only when every lane of the wave shares it
+    uint32_t streamRatio = 1, chunkBits = 0;
+    __device__ __forceinline__ CellRows(const Lookup& l, const sim::GridBounds& grid, int R, uint32_t key)
+        : lookup(l), g(grid), reach(R), side(2 * R + 1) {
+        decode_key(key, g.dim, cx, cy, cz);
+        x0 = max(0, cx - R);
+        x1 = min(g.dim.x - 1, cx + R);
     }
-}
+    __device__ __forceinline__ int count() const { return side * side; }
+    __device__ __forceinline__ void span(int r, uint32_t& lo, uint32_t& hi) const {
+        const int z = cz + r / side - reach, y = cy + r % side - reach;
+        lo = hi = 0u;
+        if (z < 0 || z >= g.dim.z || y < 0 || y >= g.dim.y || x0 > x1) return;
+        const uint32_t rowBase = (uint32_t)((z * g.dim.y + y) * g.dim.x);
+        lookup.span(rowBase + (uint32_t)x0, rowBase + (uint32_t)x1, lo, hi);
+    }
+};
 
-template <class Rows>
+// Positions in original particle order, reached through the sorted index array
+// (which has no slots past the last particle); a particle's id is its original index.
+struct IndexedFetch {
+    static constexpr bool kPadded = false;
+    const float4* P;
+    const uint32_t* idxS;
+    __device__ __forceinline__ uint32_t id(uint32_t k) const { return idxS[k]; }
+    __device__ __forceinline__ float4 at(uint32_t j) const { return P[j]; }
+};
+
+// Every kernel below runs one lane per sorted slot and walks the slot's neighbours
+// with the shared wave walk; lanes past N take part in the walk's wave-level steps
+// with no rows.
+template <class Lookup>
 __global__ __launch_bounds__(kBS) void k_lambda_ref(float* __restrict__ lambda, const float4* __restrict__ P,
                                                     const uint32_t* __restrict__ idxS,
-                                                    const uint32_t* __restrict__ keyS, Rows rows,
+                                                    const uint32_t* __restrict__ keyS, Lookup lookup,
                                                     sim::DeviceParams dp, uint32_t N) {
+    __shared__ uint32_t sQ[kQD * kBS];
+    __shared__ float4 sBuf[kBS];
     const uint32_t s = blockIdx.x * kBS + threadIdx.x;
-    if (s >= N) return;
-    const uint32_t i = idxS[s];
+    const bool valid = s < N;
+    const uint32_t ss = valid ? s : N - 1;
+    const uint32_t i = idxS[ss];
     const float4 pi = P[i];
     const PairConsts c = make_pair_consts(dp);
-    const int cap = dp.maxNeighbors;
-    const bool hasCap = cap > 0;
     LambdaAcc a;
-    for_each_row(keyS[s], dp.grid, reach_of(c.h, dp.grid.cellSize), rows, [&](uint32_t lo, uint32_t hi) {
-        for (uint32_t k = lo; k < hi; ++k) {
-            if (hasCap && a.n >= cap) return false;
-            const uint32_t j = idxS[k];
-            if (j == i) continue;
-            const float4 pj = P[j];
-            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            const float r2 = rx * rx + ry * ry + rz * rz;
-            if (r2 > c.h2) continue;
-            lambda_pair(a, rx, ry, rz, r2, c);
-        }
-        return true;
-    });
-    lambda[i] = lambda_finish(a, dp);
+    auto dist2 = [&](const float4& pj) {
+        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+        return rx * rx + ry * ry + rz * rz;
+    };
+    auto apply = [&](uint32_t j) {
+        const float4 pj = P[j];
+        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+        lambda_pair(a, rx, ry, rz, rx * rx + ry * ry + rz * rz, c);
+    };
+    const CellRows<Lookup> rows(lookup, dp.grid, reach_of(c.h, dp.grid.cellSize), keyS[ss]);
+    neighbor_walk<false>(valid, s, rows, CandMask{}, IndexedFetch{P, idxS}, sBuf + (threadIdx.x & ~63u),
+                         sQ + threadIdx.x, dp.maxNeighbors, c.h2, dist2, apply);
+    if (valid) lambda[i] = lambda_finish(a, dp);
 }
 
-template <class Rows>
+template <class Lookup>
 __global__ __launch_bounds__(kBS) void k_delta_ref(float4* __restrict__ delta, const float4* __restrict__ P,
                                                    const float* __restrict__ lambda,
                                                    const uint32_t* __restrict__ idxS,
-                                                   const uint32_t* __restrict__ keyS, Rows rows,
+                                                   const uint32_t* __restrict__ keyS, Lookup lookup,
                                                    sim::DeviceParams dp, uint32_t N) {
+    __shared__ uint32_t sQ[kQD * kBS];
+    __shared__ float4 sBuf[kBS];
     const uint32_t s = blockIdx.x * kBS + threadIdx.x;
-    if (s >= N) return;
-    const uint32_t i = idxS[s];
+    const bool valid = s < N;
+    const uint32_t ss = valid ? s : N - 1;
+    const uint32_t i = idxS[ss];
     const float4 pi = P[i];
     const float lam_i = lambda[i];
     const PairConsts c = make_pair_consts(dp);
     float dx = 0.f, dy = 0.f, dz = 0.f;
-    for_each_row(keyS[s], dp.grid, reach_of(c.h, dp.grid.cellSize), rows, [&](uint32_t lo, uint32_t hi) {
-        for (uint32_t k = lo; k < hi; ++k) {
-            const uint32_t j = idxS[k];
-            if (j == i) continue;
-            const float4 pj = P[j];
-            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            const float r2 = rx * rx + ry * ry + rz * rz;
-            if (r2 > c.h2) continue;
-            delta_pair(dx, dy, dz, rx, ry, rz, r2, lam_i, lambda[j], c);
-        }
-        return true;
-    });
+    auto dist2 = [&](const float4& pj) {
+        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+        return rx * rx + ry * ry + rz * rz;
+    };
+    auto apply = [&](uint32_t j) {
+        const float4 pj = P[j];
+        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+        delta_pair(dx, dy, dz, rx, ry, rz, rx * rx + ry * ry + rz * rz, lam_i, lambda[j], c);
+    };
+    const CellRows<Lookup> rows(lookup, dp.grid, reach_of(c.h, dp.grid.cellSize), keyS[ss]);
+    neighbor_walk<false>(valid, s, rows, CandMask{}, IndexedFetch{P, idxS}, sBuf + (threadIdx.x & ~63u),
+                         sQ + threadIdx.x, 0, c.h2, dist2, apply);
+    if (!valid) return;
     const float3 d = delta_finish(dx, dy, dz, dp);
     delta[i] = make_float4(d.x, d.y, d.z, 0.0f);
 }
@@ -110,37 +135,41 @@ __global__ __launch_bounds__(kBS) void k_apply_delta(float4* __restrict__ P, con
     P[i] = make_float4(b.x + d.x, b.y + d.y, b.z + d.z, b.w);
 }
 
-template <class Rows>
+template <class Lookup>
 __global__ __launch_bounds__(kBS) void k_xsph_ref(float4* __restrict__ Vout, const float4* __restrict__ Vin,
                                                   const float4* __restrict__ P, const uint32_t* __restrict__ idxS,
-                                                  const uint32_t* __restrict__ keyS, Rows rows,
+                                                  const uint32_t* __restrict__ keyS, Lookup lookup,
                                                   sim::DeviceParams dp, uint32_t N) {
+    __shared__ uint32_t sQ[kQD * kBS];
+    __shared__ float4 sBuf[kBS];
     const uint32_t s = blockIdx.x * kBS + threadIdx.x;
-    if (s >= N) return;
-    const uint32_t i = idxS[s];
+    const bool valid = s < N;
+    const uint32_t ss = valid ? s : N - 1;
+    const uint32_t i = idxS[ss];
     const float4 pi = P[i];
     const float4 vi = Vin[i];
     const float h2 = dp.kernel.h2, poly6 = dp.kernel.poly6;
     float sx = 0.f, sy = 0.f, sz = 0.f;
     int n = 0;
-    for_each_row(keyS[s], dp.grid, reach_of(dp.kernel.h, dp.grid.cellSize), rows, [&](uint32_t lo, uint32_t hi) {
-        for (uint32_t k = lo; k < hi; ++k) {
-            const uint32_t j = idxS[k];
-            if (j == i) continue;
-            const float4 pj = P[j];
-            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            const float r2 = rx * rx + ry * ry + rz * rz;
-            if (r2 > h2) continue;
-            const float hr2 = h2 - r2;
-            const float w = poly6 * hr2 * hr2 * hr2;
-            const float4 vj = Vin[j];
-            sx += (vj.x - vi.x) * w;
-            sy += (vj.y - vi.y) * w;
-            sz += (vj.z - vi.z) * w;
-            ++n;
-        }
-        return true;
-    });
+    auto dist2 = [&](const float4& pj) {
+        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+        return rx * rx + ry * ry + rz * rz;
+    };
+    auto apply = [&](uint32_t j) {
+        const float4 pj = P[j];
+        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+        const float hr2 = h2 - (rx * rx + ry * ry + rz * rz);
+        const float w = poly6 * hr2 * hr2 * hr2;
+        const float4 vj = Vin[j];
+        sx += (vj.x - vi.x) * w;
+        sy += (vj.y - vi.y) * w;
+        sz += (vj.z - vi.z) * w;
+        ++n;
+    };
+    const CellRows<Lookup> rows(lookup, dp.grid, reach_of(dp.kernel.h, dp.grid.cellSize), keyS[ss]);
+    neighbor_walk<false>(valid, s, rows, CandMask{}, IndexedFetch{P, idxS}, sBuf + (threadIdx.x & ~63u),
+                         sQ + threadIdx.x, 0, h2, dist2, apply);
+    if (!valid) return;
     const float3 v = xsph_finish(vi.x, vi.y, vi.z, sx, sy, sz, n, dp, dp.xsph_c);
     Vout[i] = make_float4(v.x, v.y, v.z, 0.0f);
 }

@@ -17,11 +17,12 @@
 #include "pbf_hip.h"
 #include "pbf_math.hpp"
 #include "sorted_path.hpp"
+#include "walk.hpp"
 
 namespace pbfx {
 namespace {
 
-constexpr int kBS = 256;
+using namespace walk;
 
 __device__ __forceinline__ uint32_t slot_of_thread() {
     return xcd_tile(blockIdx.x, gridDim.x) * kBS + threadIdx.x;
@@ -196,223 +197,29 @@ struct HashRowsDev {
     }
 };
 
-// ------------------------------------------------------------------------
-// Neighbour walk of one wave (64 consecutive sorted slots, one particle per lane).
-//
-// Two walk modes per neighbour row, chosen wave-uniformly.  Stream: when the union
-// of the lanes' slot spans is at most SpanTable::streamRatio times the longest
-// lane span, the wave streams the union through a per-wave LDS buffer, 64
-// candidates at a time loaded coalesced, and every lane tests each candidate (an
-// LDS broadcast) against its own span, itself and the distance h; with chunkBits
-// it tests a whole chunk into a bit mask first.  Per lane: otherwise each lane
-// walks its own span from global memory, kUnroll candidates a round so their
-// loads are in flight together.  When a wave spans several cells the union carries
-// candidates most lanes do not need, and the per-lane walk is faster.
-//
-// Deferred pairs.  In-range slots go to a per-lane FIFO in LDS; the expensive
-// per-pair work (sqrt, IEEE divisions, the s_corr power) runs when some lane's
-// FIFO is full, for one queued pair on every lane that has one, and at the end.
-// Otherwise a wave would execute the per-pair path whenever any of its 64 lanes
-// finds a neighbour, at low utilisation.
-//
-// Order.  Each lane still sees its candidates in reference order (rows dz-major
-// then dy, slots ascending within a row) and consumes its FIFO in order, so
-// every sum is formed in the reference's order.
-constexpr int kQD = 16;          // FIFO depth per lane
-constexpr uint32_t kUnroll = 4;  // candidates per round of a lane's own walk
-static_assert(kSlotPad + 1 >= kUnroll, "a round may load kUnroll - 1 slots past the last particle");
-
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o));
-    return v;
-}
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
-    return v;
-}
-
-// Candidate mask: bit c of particle s is set when candidate c of its walk (the
-// concatenated neighbour-row spans, in walk order) was within the skin radius
-// h + 2 * I * maxDisp when the first delta pass scanned the predicted positions.
-// Every later pass of the step moves a particle by at most I * maxDisp (each
-// delta is clamped to maxDisp; the box clamp is a projection and cannot increase
-// distances), so a pair outside the skin cannot come within h later in the step.
-// Later passes visit only the set bits, in walk order, and still test r2 <= h2.
-// The mask covers the first 32 * kMaskWords candidates; candidates past that are
-// tested directly.
-constexpr uint32_t kMaskWords = kCandMaskWords;
-
-// Bits [0, n) of a 64-bit word (n <= 64).
-__device__ __forceinline__ uint64_t low_bits64(uint32_t n) { return n >= 64u ? ~0ull : ((1ull << n) - 1ull); }
-
-template <bool kBuild, class Dist2, class Apply>
-__device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, uint32_t m, const SpanTable& st,
-                                              const CandMask& mk, const float4* P, float4* buf, uint32_t* q,
-                                              int cap, float h2, Dist2&& dist2, Apply&& apply) {
-    const uint32_t lane = threadIdx.x & 63;
-    int qh = 0, qn = 0, found = 0;
-    uint32_t cidx = 0, word = 0;  // mask building: own-candidate index and pending word
-    auto push = [&](uint32_t k) {
-        q[((qh + qn) & (kQD - 1)) * kBS] = k;
-        ++qn;
-        ++found;
-    };
-    auto pop_one = [&]() {
-        if (qn > 0) {
-            const uint32_t k = q[qh * kBS];
-            qh = (qh + 1) & (kQD - 1);
-            --qn;
-            apply(k);
-        }
-    };
-    // pops until no lane holds more than `keep` pairs (written as a do-while: the
-    // compiler then keeps the loop-carried sums in place instead of copying them
-    // on every pop)
-    auto pop_above = [&](int keep) {
-        if (__any(qn > keep)) do {
-                pop_one();
-            } while (__any(qn > keep));
-    };
-    // one own candidate: record its skin bit, queue it when in range
-    auto visit = [&](uint32_t k, float r2, uint32_t& hi, uint32_t lo) {
-        if (kBuild) {
-            if (r2 <= mk.rs2) word |= 1u << (cidx & 31);
-            ++cidx;
-            if ((cidx & 31) == 0) {
-                const uint32_t w = (cidx >> 5) - 1;
-                if (w < kMaskWords) mk.words[(uint64_t)w * mk.stride + s] = word;
-                word = 0;
-            }
-        }
-        if (k != s && r2 <= h2) {
-            push(k);
-            if (cap > 0 && found >= cap) hi = lo;
-        }
-    };
-    for (int r = 0; r < st.rows; ++r) {
-        uint32_t lo = 0, hi = 0;
-        if (valid) {
-            const uint2 sp = st.spans[(uint64_t)r * st.stride + m];
-            lo = sp.x;
-            hi = sp.y;
-        }
-        if (cap > 0 && found >= cap) hi = lo;  // neighbour cap reached: nothing more to take
-        const bool has = lo < hi;
-        const uint32_t ulo = wave_min_u32(has ? lo : 0xFFFFFFFFu);
-        const uint32_t uhi = wave_max_u32(has ? hi : 0u);
-        if (uhi <= ulo) continue;
-        const uint32_t lmax = wave_max_u32(hi - lo);
-        if (uhi - ulo > st.streamRatio * lmax) {
-            // sparse row: each lane walks its own span, kUnroll candidates a round so
-            // that their loads are in flight together; in-range ones are queued in order
-            uint32_t k = lo;
-            while (__any(k < hi)) {
-                const uint32_t cnt = k < hi ? min(kUnroll, hi - k) : 0u;
-                // k <= hi <= N, so the loads reach at most slot N + kUnroll - 2, inside
-                // the kSlotPad slots past the last particle; tests past cnt are masked off
-                float4 pj[kUnroll];
-#pragma unroll
-                for (uint32_t u = 0; u < kUnroll; ++u) pj[u] = P[k + u];
-                const uint32_t inSpan = (1u << cnt) - 1u;
-                const uint32_t self = s - k < kUnroll ? 1u << (s - k) : 0u;
-                uint32_t bits = 0u, skin = 0u;
-#pragma unroll
-                for (uint32_t u = 0; u < kUnroll; ++u) {
-                    const float r2 = dist2(pj[u]);
-                    bits |= (r2 <= h2) ? 1u << u : 0u;
-                    if (kBuild) skin |= (r2 <= mk.rs2) ? 1u << u : 0u;
-                }
-                bits &= inSpan & ~self;
-                skin &= inSpan;
-                if (kBuild && cnt > 0) {
-                    const uint32_t sh = cidx & 31u;
-                    uint64_t w = (uint64_t)word | ((uint64_t)skin << sh);
-                    if (sh + cnt >= 32u) {
-                        const uint32_t wi = cidx >> 5;
-                        if (wi < kMaskWords) mk.words[(uint64_t)wi * mk.stride + s] = (uint32_t)w;
-                        w >>= 32;
-                    }
-                    word = (uint32_t)w;
-                    cidx += cnt;
-                }
-                if (cap > 0) {
-                    const int room = cap - found;
-                    while (__popc(bits) > room) bits &= ~(1u << (31 - __clz(bits)));
-                    found += __popc(bits);
-                    if (found >= cap) hi = k;  // nothing more to take in this row
-                }
-                pop_above(kQD - (int)kUnroll);
-#pragma unroll
-                for (uint32_t u = 0; u < kUnroll; ++u) {  // room for kUnroll pushes was made above
-                    if ((bits >> u) & 1u) {
-                        q[((qh + qn) & (kQD - 1)) * kBS] = k + u;
-                        ++qn;
-                    }
-                }
-                k += cnt;
-            }
-        } else if (kBuild || st.chunkBits == 0) {
-            // dense row, candidate by candidate (and building the candidate mask):
-            // stream the union through LDS
-            for (uint32_t base = ulo; base < uhi; base += 64) {
-                const uint32_t c = base + lane;
-                if (c < uhi) buf[lane] = P[c];
-                __builtin_amdgcn_wave_barrier();
-                const uint32_t n = min(64u, uhi - base);
-                for (uint32_t t = 0; t < n; ++t) {
-                    if (__any(qn == kQD)) pop_one();
-                    const uint32_t k = base + t;
-                    if (k >= lo && k < hi) visit(k, dist2(buf[t]), hi, lo);
-                }
-                __builtin_amdgcn_wave_barrier();
-            }
-        } else {
-            // dense row: stream the union through LDS, 64 candidates at a time.  Every
-            // lane tests every candidate of a chunk in a branch-free wave-uniform loop
-            // and keeps the in-range ones as bits; then only its own span's bits, minus
-            // its own slot and anything past the neighbour cap, enter its FIFO in order.
-            for (uint32_t base = ulo; base < uhi; base += 64) {
-                const uint32_t c = base + lane;
-                if (c < uhi) buf[lane] = P[c];
-                __builtin_amdgcn_wave_barrier();
-                const uint32_t n = min(64u, uhi - base);
-                const uint32_t n0 = min(n, 32u);
-                uint32_t m0 = 0u, m1 = 0u;
-                for (uint32_t t = 0; t < n0; ++t) m0 |= (dist2(buf[t]) <= h2) ? (1u << t) : 0u;
-                for (uint32_t t = 32; t < n; ++t) m1 |= (dist2(buf[t]) <= h2) ? (1u << (t - 32)) : 0u;
-                __builtin_amdgcn_wave_barrier();
-                const uint32_t a = lo > base ? min(lo - base, n) : 0u;
-                const uint32_t b = hi > base ? min(hi - base, n) : 0u;
-                uint64_t bits = (((uint64_t)m1 << 32) | m0) & low_bits64(b) & ~low_bits64(a);
-                if (s - base < n) bits &= ~(1ull << (s - base));
-                if (cap > 0) {
-                    const int room = cap - found;
-                    while (__popcll(bits) > room) bits &= ~(1ull << (63 - __clzll(bits)));
-                    found += __popcll(bits);
-                }
-                while (__any(bits != 0ull)) {
-                    if (__any(qn == kQD)) pop_one();
-                    if (bits != 0ull && qn < kQD) {
-                        const uint32_t t = (uint32_t)__ffsll((long long)bits) - 1u;
-                        bits &= bits - 1ull;
-                        q[((qh + qn) & (kQD - 1)) * kBS] = base + t;
-                        ++qn;
-                    }
-                }
-                // stop the row once no lane needs more of it
-                if (!__any((cap <= 0 || found < cap) && hi > base + 64)) break;
-            }
-            if (cap > 0 && found >= cap) hi = lo;
-        }
+// Rows of a sorted slot's neighbourhood from the per-step span table.
+struct SpanRows {
+    const SpanTable& st;
+    uint32_t m;  // occupied-cell index of the slot
+    uint32_t streamRatio, chunkBits;
+    __device__ __forceinline__ SpanRows(const SpanTable& t, uint32_t cell)
+        : st(t), m(cell), streamRatio(t.streamRatio), chunkBits(t.chunkBits) {}
+    __device__ __forceinline__ int count() const { return st.rows; }
+    __device__ __forceinline__ void span(int r, uint32_t& lo, uint32_t& hi) const {
+        const uint2 sp = st.spans[(uint64_t)r * st.stride + m];
+        lo = sp.x;
+        hi = sp.y;
     }
-    if (kBuild && valid && (cidx & 31) != 0) {
-        const uint32_t w = cidx >> 5;
-        if (w < kMaskWords) mk.words[(uint64_t)w * mk.stride + s] = word;
-    }
-    pop_above(0);
-}
+};
+
+// Sorted positions, padded by kSlotPad slots past the last particle; a particle's id
+// is its slot.
+struct SortedFetch {
+    static constexpr bool kPadded = true;
+    const float4* P;
+    __device__ __forceinline__ uint32_t id(uint32_t k) const { return k; }
+    __device__ __forceinline__ float4 at(uint32_t k) const { return P[k]; }
+};
 
 // Walk of the set mask bits (then the unmasked tail) in walk order, per lane.
 // Each round takes up to kUnroll candidates, so their position loads are in flight
@@ -550,7 +357,7 @@ __global__ __launch_bounds__(kBS) void k_lambda_sorted(float4* Ps, const uint32_
     if (useMask)
         mask_walk(valid, s, m, st, mk, Ps, sQ + threadIdx.x, dp.maxNeighbors, c.h2, dist2, apply);
     else
-        neighbor_walk<false>(valid, s, m, st, mk, Ps, sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x,
+        neighbor_walk<false>(valid, s, SpanRows(st, m), mk, SortedFetch{Ps}, sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x,
                              dp.maxNeighbors, c.h2, dist2, apply);
     if (!valid) return;
     reinterpret_cast<float*>(Ps)[4ull * s + 3] = lambda_finish(a, dp);
@@ -593,9 +400,11 @@ __global__ __launch_bounds__(kBS) void k_delta_sorted(const float4* __restrict__
     if (maskMode == 2)
         mask_walk(valid, s, m, st, mk, Pa, sQ + threadIdx.x, 0, c.h2, dist2, apply);
     else if (maskMode == 1)
-        neighbor_walk<true>(valid, s, m, st, mk, Pa, buf, sQ + threadIdx.x, 0, c.h2, dist2, apply);
+        neighbor_walk<true>(valid, s, SpanRows(st, m), mk, SortedFetch{Pa}, buf, sQ + threadIdx.x, 0, c.h2, dist2,
+                            apply);
     else
-        neighbor_walk<false>(valid, s, m, st, mk, Pa, buf, sQ + threadIdx.x, 0, c.h2, dist2, apply);
+        neighbor_walk<false>(valid, s, SpanRows(st, m), mk, SortedFetch{Pa}, buf, sQ + threadIdx.x, 0, c.h2, dist2,
+                             apply);
     if (!valid) return;
     const float3 e = delta_scaled(dx, dy, dz, dp);
     if (recU) recU[s] = make_float4(e.x, e.y, e.z, 0.f);
@@ -679,7 +488,7 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(8))) void k
         if (useMask)
             mask_walk(valid && walk, s, m, st, mk, Ps, sQ + threadIdx.x, 0, h2, dist2, apply);
         else
-            neighbor_walk<false>(valid && walk, s, m, st, mk, Ps, sBuf + (threadIdx.x & ~63u),
+            neighbor_walk<false>(valid && walk, s, SpanRows(st, m), mk, SortedFetch{Ps}, sBuf + (threadIdx.x & ~63u),
                                  sQ + threadIdx.x, 0, h2, dist2, apply);
         if (recGm && valid)
             recGm[s] = xsph_gate(n, dp) * ((dp.restDensity > 0.f) ? (dp.particleMass / dp.restDensity) : 0.f);

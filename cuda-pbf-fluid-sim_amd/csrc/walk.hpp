@@ -1,0 +1,243 @@
+// walk.hpp — the neighbour walk of one wave, shared by the cell-reordered kernels
+// (k_solve_sorted.hip) and the drop-in launchers over the reference layout
+// (k_solve_ref.hip).  Both visit a particle's candidates in the reference's order.
+#pragma once
+
+#include "pbf_common.hpp"
+#include "sorted_path.hpp"
+
+namespace pbfx {
+namespace walk {
+
+constexpr int kBS = 256;  // lanes per block of every kernel that walks
+
+// Neighbour walk of one wave (64 consecutive sorted slots, one particle per lane).
+//
+// Rows.  A row source gives the lane's particle its neighbour rows: count(), and
+// span(r, lo, hi), the [lo, hi) of sorted slots of row r (rows dz-major, then dy).
+// The cell-reordered kernels read them from the per-step span table; the drop-in
+// launchers compute them from the cell lookup they are handed.  fetch.id(k) names the
+// particle in sorted slot k (the slot itself, or the sorted index array's entry) and
+// fetch.at(id) returns its position; the FIFO carries ids, so apply(id) reads the
+// particle's data with one load.  When Fetch::kPadded, the slots past the last
+// particle may be read.
+//
+// Two walk modes per neighbour row, chosen wave-uniformly.  Stream: when the union
+// of the lanes' slot spans is at most rows.streamRatio times the longest
+// lane span, the wave streams the union through a per-wave LDS buffer, 64
+// candidates at a time loaded coalesced, and every lane tests each candidate (an
+// LDS broadcast) against its own span, itself and the distance h; with chunkBits
+// it tests a whole chunk into a bit mask first (rows.chunkBits).  Per lane: otherwise each lane
+// walks its own span from global memory, kUnroll candidates a round so their
+// loads are in flight together.  When a wave spans several cells the union carries
+// candidates most lanes do not need, and the per-lane walk is faster.
+//
+// Deferred pairs.  In-range slots go to a per-lane FIFO in LDS; the expensive
+// per-pair work (sqrt, IEEE divisions, the s_corr power) runs when some lane's
+// FIFO is full, for one queued pair on every lane that has one, and at the end.
+// Otherwise a wave would execute the per-pair path whenever any of its 64 lanes
+// finds a neighbour, at low utilisation.
+//
+// Order.  Each lane still sees its candidates in reference order (rows dz-major
+// then dy, slots ascending within a row) and consumes its FIFO in order, so
+// every sum is formed in the reference's order.
+constexpr int kQD = 16;          // FIFO depth per lane
+constexpr uint32_t kUnroll = 4;  // candidates per round of a lane's own walk
+static_assert(kSlotPad + 1 >= kUnroll, "a round may load kUnroll - 1 slots past the last particle");
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
+    return v;
+}
+
+// Candidate mask: bit c of particle s is set when candidate c of its walk (the
+// concatenated neighbour-row spans, in walk order) was within the skin radius
+// h + 2 * I * maxDisp when the first delta pass scanned the predicted positions.
+// Every later pass of the step moves a particle by at most I * maxDisp (each
+// delta is clamped to maxDisp; the box clamp is a projection and cannot increase
+// distances), so a pair outside the skin cannot come within h later in the step.
+// Later passes visit only the set bits, in walk order, and still test r2 <= h2.
+// The mask covers the first 32 * kMaskWords candidates; candidates past that are
+// tested directly.
+constexpr uint32_t kMaskWords = kCandMaskWords;
+
+// Bits [0, n) of a 64-bit word (n <= 64).
+__device__ __forceinline__ uint64_t low_bits64(uint32_t n) { return n >= 64u ? ~0ull : ((1ull << n) - 1ull); }
+
+template <bool kBuild, class Rows, class Fetch, class Dist2, class Apply>
+__device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, const Rows& rows, const CandMask& mk,
+                                              const Fetch& fetch, float4* buf, uint32_t* q, int cap, float h2,
+                                              Dist2&& dist2, Apply&& apply) {
+    const uint32_t lane = threadIdx.x & 63;
+    int qh = 0, qn = 0, found = 0;
+    uint32_t cidx = 0, word = 0;  // mask building: own-candidate index and pending word
+    auto push = [&](uint32_t k) {
+        q[((qh + qn) & (kQD - 1)) * kBS] = k;
+        ++qn;
+        ++found;
+    };
+    auto pop_one = [&]() {
+        if (qn > 0) {
+            const uint32_t k = q[qh * kBS];
+            qh = (qh + 1) & (kQD - 1);
+            --qn;
+            apply(k);
+        }
+    };
+    // pops until no lane holds more than `keep` pairs (written as a do-while: the
+    // compiler then keeps the loop-carried sums in place instead of copying them
+    // on every pop)
+    auto pop_above = [&](int keep) {
+        if (__any(qn > keep)) do {
+                pop_one();
+            } while (__any(qn > keep));
+    };
+    // one own candidate: record its skin bit, queue it when in range
+    auto visit = [&](uint32_t k, float r2, uint32_t& hi, uint32_t lo) {
+        if (kBuild) {
+            if (r2 <= mk.rs2) word |= 1u << (cidx & 31);
+            ++cidx;
+            if ((cidx & 31) == 0) {
+                const uint32_t w = (cidx >> 5) - 1;
+                if (w < kMaskWords) mk.words[(uint64_t)w * mk.stride + s] = word;
+                word = 0;
+            }
+        }
+        if (k != s && r2 <= h2) {
+            push(fetch.id(k));
+            if (cap > 0 && found >= cap) hi = lo;
+        }
+    };
+    const int nrows = rows.count();
+    for (int r = 0; r < nrows; ++r) {
+        uint32_t lo = 0, hi = 0;
+        if (valid) rows.span(r, lo, hi);
+        if (cap > 0 && found >= cap) hi = lo;  // neighbour cap reached: nothing more to take
+        const bool has = lo < hi;
+        const uint32_t ulo = wave_min_u32(has ? lo : 0xFFFFFFFFu);
+        const uint32_t uhi = wave_max_u32(has ? hi : 0u);
+        if (uhi <= ulo) continue;
+        const uint32_t lmax = wave_max_u32(hi - lo);
+        if (uhi - ulo > rows.streamRatio * lmax) {
+            // sparse row: each lane walks its own span, kUnroll candidates a round so
+            // that their loads are in flight together; in-range ones are queued in order
+            uint32_t k = lo;
+            while (__any(k < hi)) {
+                const uint32_t cnt = k < hi ? min(kUnroll, hi - k) : 0u;
+                // padded: k <= hi <= N, so the loads reach at most slot N + kUnroll - 2,
+                // inside the kSlotPad slots past the last particle; otherwise loads past
+                // the span read its last slot.  Tests past cnt are masked off.
+                const uint32_t last = hi > 0u ? hi - 1u : 0u;
+                uint32_t id[kUnroll];
+                float4 pj[kUnroll];
+#pragma unroll
+                for (uint32_t u = 0; u < kUnroll; ++u) id[u] = fetch.id(Fetch::kPadded ? k + u : min(k + u, last));
+#pragma unroll
+                for (uint32_t u = 0; u < kUnroll; ++u) pj[u] = fetch.at(id[u]);
+                const uint32_t inSpan = (1u << cnt) - 1u;
+                const uint32_t self = s - k < kUnroll ? 1u << (s - k) : 0u;
+                uint32_t bits = 0u, skin = 0u;
+#pragma unroll
+                for (uint32_t u = 0; u < kUnroll; ++u) {
+                    const float r2 = dist2(pj[u]);
+                    bits |= (r2 <= h2) ? 1u << u : 0u;
+                    if (kBuild) skin |= (r2 <= mk.rs2) ? 1u << u : 0u;
+                }
+                bits &= inSpan & ~self;
+                skin &= inSpan;
+                if (kBuild && cnt > 0) {
+                    const uint32_t sh = cidx & 31u;
+                    uint64_t w = (uint64_t)word | ((uint64_t)skin << sh);
+                    if (sh + cnt >= 32u) {
+                        const uint32_t wi = cidx >> 5;
+                        if (wi < kMaskWords) mk.words[(uint64_t)wi * mk.stride + s] = (uint32_t)w;
+                        w >>= 32;
+                    }
+                    word = (uint32_t)w;
+                    cidx += cnt;
+                }
+                if (cap > 0) {
+                    const int room = cap - found;
+                    while (__popc(bits) > room) bits &= ~(1u << (31 - __clz(bits)));
+                    found += __popc(bits);
+                    if (found >= cap) hi = k;  // nothing more to take in this row
+                }
+                pop_above(kQD - (int)kUnroll);
+#pragma unroll
+                for (uint32_t u = 0; u < kUnroll; ++u) {  // room for kUnroll pushes was made above
+                    if ((bits >> u) & 1u) {
+                        q[((qh + qn) & (kQD - 1)) * kBS] = id[u];
+                        ++qn;
+                    }
+                }
+                k += cnt;
+            }
+        } else if (kBuild || rows.chunkBits == 0) {
+            // dense row, candidate by candidate (and building the candidate mask):
+            // stream the union through LDS
+            for (uint32_t base = ulo; base < uhi; base += 64) {
+                const uint32_t c = base + lane;
+                if (c < uhi) buf[lane] = fetch.at(fetch.id(c));
+                __builtin_amdgcn_wave_barrier();
+                const uint32_t n = min(64u, uhi - base);
+                for (uint32_t t = 0; t < n; ++t) {
+                    if (__any(qn == kQD)) pop_one();
+                    const uint32_t k = base + t;
+                    if (k >= lo && k < hi) visit(k, dist2(buf[t]), hi, lo);
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        } else {
+            // dense row: stream the union through LDS, 64 candidates at a time.  Every
+            // lane tests every candidate of a chunk in a branch-free wave-uniform loop
+            // and keeps the in-range ones as bits; then only its own span's bits, minus
+            // its own slot and anything past the neighbour cap, enter its FIFO in order.
+            for (uint32_t base = ulo; base < uhi; base += 64) {
+                const uint32_t c = base + lane;
+                if (c < uhi) buf[lane] = fetch.at(fetch.id(c));
+                __builtin_amdgcn_wave_barrier();
+                const uint32_t n = min(64u, uhi - base);
+                const uint32_t n0 = min(n, 32u);
+                uint32_t m0 = 0u, m1 = 0u;
+                for (uint32_t t = 0; t < n0; ++t) m0 |= (dist2(buf[t]) <= h2) ? (1u << t) : 0u;
+                for (uint32_t t = 32; t < n; ++t) m1 |= (dist2(buf[t]) <= h2) ? (1u << (t - 32)) : 0u;
+                __builtin_amdgcn_wave_barrier();
+                const uint32_t a = lo > base ? min(lo - base, n) : 0u;
+                const uint32_t b = hi > base ? min(hi - base, n) : 0u;
+                uint64_t bits = (((uint64_t)m1 << 32) | m0) & low_bits64(b) & ~low_bits64(a);
+                if (s - base < n) bits &= ~(1ull << (s - base));
+                if (cap > 0) {
+                    const int room = cap - found;
+                    while (__popcll(bits) > room) bits &= ~(1ull << (63 - __clzll(bits)));
+                    found += __popcll(bits);
+                }
+                while (__any(bits != 0ull)) {
+                    if (__any(qn == kQD)) pop_one();
+                    if (bits != 0ull && qn < kQD) {
+                        const uint32_t t = (uint32_t)__ffsll((long long)bits) - 1u;
+                        bits &= bits - 1ull;
+                        q[((qh + qn) & (kQD - 1)) * kBS] = fetch.id(base + t);
+                        ++qn;
+                    }
+                }
+                // stop the row once no lane needs more of it
+                if (!__any((cap <= 0 || found < cap) && hi > base + 64)) break;
+            }
+            if (cap > 0 && found >= cap) hi = lo;
+        }
+    }
+    if (kBuild && valid && (cidx & 31) != 0) {
+        const uint32_t w = cidx >> 5;
+        if (w < kMaskWords) mk.words[(uint64_t)w * mk.stride + s] = word;
+    }
+    pop_above(0);
+}
+
+}  // namespace walk
+}  // namespace pbfx
