@@ -77,6 +77,28 @@ def sort_bytes(key_bits):
     return 16.0 * passes + 4.0
 
 
+def bench_drop_in(scene, warmup, steps):
+    """The same workload through the drop-in launchers (LaunchLambda / LaunchDeltaApply /
+    LaunchXSPH over the reference's layout, dense cell ranges), as the reference's own
+    phase pipeline would call them after swapping in this library: graph replay of the
+    same steps as the timed region."""
+    from pbfx.sim import Simulator
+    sim = Simulator.from_scene(scene, grid="dense", path="reference")
+    try:
+        sim.step(warmup)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.step(steps)
+        sim.synchronize()
+        dt_s = time.perf_counter() - t0
+        n = sim.n
+    finally:
+        sim.close()
+    print(f"[bench] drop-in launchers: {steps} steps in {dt_s:.4f} s", file=sys.stderr, flush=True)
+    return {"particle_steps_per_s": round(n * steps / dt_s, 1), "ms_per_step": round(dt_s * 1e3 / steps, 5),
+            "steps": f"{warmup}..{warmup + steps}", "grid": "dense", "path": "reference layout, drop-in launchers"}
+
+
 def bench_batched(count, warmup, steps, tune_steps=0, grid="dense"):
     """The batched ensemble (one launch per phase for all scenes, graph replay) against the
     same scenes as separate plain simulators, each replaying its own captured step:
@@ -250,6 +272,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--grid", default="hashed", choices=["hashed", "compact", "dense"])
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-drop-in", action="store_true", help="skip timing the drop-in launchers' path")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=30)
     ap.add_argument("--batch-warmup", type=int, default=20)
@@ -366,6 +389,9 @@ def main():
         "hash_overflow": overflow,
         "finite": bool((X == X).all() and (V == V).all()),
     }
+
+    if not a.no_drop_in:
+        out["drop_in"] = bench_drop_in(scene, a.warmup, a.steps)
 
     if not a.no_batched:
         out["batched"] = bench_batched(a.batch_scenes, a.batch_warmup, a.batch_steps,
