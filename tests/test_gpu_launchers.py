@@ -197,6 +197,48 @@ def test_lambda_delta_xsph(oracle, dev, L, grid, scene_name, steps, squeeze):
     np.testing.assert_array_equal(to_host(tVout), eV)
 
 
+@pytest.mark.parametrize("grid", ["dense", "compact"])
+def test_lambda_delta_xsph_reach_two(oracle, dev, L, grid):
+    # cells of 0.6 h: every neighbour row spans five cells and there are 25 rows
+    import torch
+    scene = P.small_block((12, 12, 12))
+    sp, X, V, Pp = predicted_state(oracle, scene, 0, (0.7, 0.15))
+    cell = np.float32(0.6) * np.float32(sp.kernel.h)
+    sp.grid.cellSize = float(cell)
+    sp.grid.dim = P.Int3(*P.compute_dims(scene.box_min, scene.box_max, cell))
+    g = grid_of(oracle, Pp, sp)
+    t = {k: to_dev(v, dev) for k, v in g.items()}
+    t["cnt"] = to_dev(np.array([len(g["uniq"])], np.uint32), dev)
+    dp = P.make_device_params(sp)
+    N = len(Pp)
+    tP = to_dev(Pp, dev)
+    lam = torch.zeros(N, dtype=torch.float32, device=dev)
+    delta = torch.zeros_like(tP)
+    rng = np.random.default_rng(5)
+    Vin = rng.normal(0, 0.5, (N, 4)).astype(np.float32)
+    tVin = to_dev(Vin, dev)
+    tVout = torch.zeros_like(tVin)
+    if grid == "dense":
+        look, elook = (t["start"], t["end"]), (g["start"], g["end"])
+        L.lambda_dense(lam, tP, t["idxS"], t["keyS"], *look, dp)
+        elam = oracle.lambda_dense(Pp, g["idxS"], g["keyS"], *elook, dp)
+        L.delta_apply_dense(tP, delta, lam, t["idxS"], t["keyS"], *look, dp)
+        eP, edelta = oracle.delta_apply_dense(Pp, elam, g["idxS"], g["keyS"], *elook, dp)
+        L.xsph_dense(tVout, tVin, tP, t["idxS"], t["keyS"], *look, dp)
+        eV = oracle.xsph_dense(Vin, eP, g["idxS"], g["keyS"], *elook, dp)
+    else:
+        L.lambda_compact(lam, tP, t["idxS"], t["keyS"], t["uniq"], t["off"], t["cnt"], dp)
+        elam = oracle.lambda_compact(Pp, g["idxS"], g["keyS"], g["uniq"], g["off"], dp)
+        L.delta_apply_compact(tP, delta, lam, t["idxS"], t["keyS"], t["uniq"], t["off"], t["cnt"], dp)
+        eP, edelta = oracle.delta_apply_compact(Pp, elam, g["idxS"], g["keyS"], g["uniq"], g["off"], dp)
+        L.xsph_compact(tVout, tVin, tP, t["idxS"], t["keyS"], t["uniq"], t["off"], t["cnt"], dp)
+        eV = oracle.xsph_compact(Vin, eP, g["idxS"], g["keyS"], g["uniq"], g["off"], dp)
+    np.testing.assert_array_equal(to_host(lam), elam)
+    np.testing.assert_array_equal(to_host(tP), eP)
+    np.testing.assert_array_equal(to_host(delta), edelta)
+    np.testing.assert_array_equal(to_host(tVout), eV)
+
+
 def test_lambda_neighbor_cap(oracle, dev, L):
     import torch
     scene = P.small_block((12, 12, 12))

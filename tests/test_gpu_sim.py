@@ -67,6 +67,33 @@ def test_ten_steps_small_block(oracle, dev, grid):
     np.testing.assert_array_equal(V, eV)
 
 
+def _fine_cells(sp, scene, frac=0.6):
+    """Cells of frac * h: the neighbourhood then reaches two cells each way (25 rows)."""
+    cell = np.float32(frac) * np.float32(sp.kernel.h)
+    sp.grid.cellSize = float(cell)
+    sp.grid.dim = P.Int3(*P.compute_dims(scene.box_min, scene.box_max, cell))
+    return sp
+
+
+@pytest.mark.parametrize("path,grid", [("reordered", "dense"), ("reordered", "hashed"), ("reference", "dense"),
+                                       ("reference", "compact")])
+def test_cells_smaller_than_h(oracle, dev, path, grid):
+    from pbfx.sim import Simulator
+    scene = P.small_block((12, 12, 12), table_log2=16)
+    sp = _fine_cells(P.build_sim_params(scene), scene)
+    sim = Simulator(sp, grid=grid, table_log2=scene.table_log2, path=path)
+    try:
+        sim.seed_scene(scene)
+        sim.step(3)
+        X, V = sim.state()
+    finally:
+        sim.close()
+    X0 = P.seed_scene_reference(scene, oracle.seed_lattice)
+    eX, eV = oracle.run(X0, np.zeros_like(X0), sp, 3, compact=(grid != "dense"))
+    np.testing.assert_array_equal(X, eX)
+    np.testing.assert_array_equal(V, eV)
+
+
 def test_graph_replay_equals_eager(dev):
     scene = P.small_block((16, 16, 16), table_log2=16)
     a = _sim(scene, "hashed", use_graph=True)
