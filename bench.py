@@ -32,6 +32,11 @@ sys.path.insert(0, REPO)
 from pbfx import params as P  # noqa: E402
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+# VALU issue rate of the whole chip in wave64 instructions per second, from the
+# published 157.3 TFLOP/s FP32 vector peak: that figure counts packed FP32 FMAs
+# (2 lanes x 2 flops per lane-op), so one wave64 instruction a SIMD every 4 cycles is
+# 157.3e12 / (64 lanes x 2 x 2).
+VALU_PEAK = 157.3e12 / (64 * 2 * 2)
 
 # Algorithmic bytes per particle of each launch label (FP32 xyz = 12 B, each datum
 # counted once per kernel): predict+clamp 36 + hash 20; sort 16 per pass + 4;
@@ -56,6 +61,7 @@ KERNELS = {
 }
 
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "hbm_traffic_dam_break_1m.json")
+INSTR_FILE = os.path.join(REPO, "profiles", "instructions_dam_break_1m.json")
 
 
 def measured_traffic(kernel):
@@ -70,6 +76,34 @@ def measured_traffic(kernel):
     if not k:
         return None, "kernel not in the counter summary"
     return k["bytes_per_launch"], f"{os.path.relpath(TRAFFIC_FILE, REPO)} ({rec.get('command', '')})"
+
+
+# Launch knobs that change what the neighbour kernels execute (the sort's keys per
+# thread does not).
+WALK_KNOBS = ("candidate_mask", "stream_ratio", "chunk_bits")
+
+
+def measured_valu(kernel, warmup, steps, knobs):
+    """VALU wave-instructions per launch of `kernel` from the profiler's SQ_INSTS_VALU
+    summary kept next to the other profiles, or (None, reason).  The counts hold only
+    for the same timed steps and the same walk knobs as the profiled run."""
+    try:
+        with open(INSTR_FILE) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None, "no counter summary"
+    if (rec.get("warmup"), rec.get("steps")) != (warmup, steps):
+        return None, (f"instruction counts were profiled for steps {rec.get('warmup')}.."
+                      f"{(rec.get('warmup') or 0) + (rec.get('steps') or 0)}; this run times other steps")
+    for run in rec.get("runs", []):
+        prof = run.get("tuning") or {}
+        if all(prof.get(n) == knobs.get(n) for n in WALK_KNOBS):
+            k = run.get("kernels", {}).get(kernel)
+            if not k:
+                return None, "kernel not in the counter summary"
+            cmd = rec.get("command", "").replace("<TUNING>", json.dumps(prof))
+            return k["valu_per_launch"], f"{os.path.relpath(INSTR_FILE, REPO)} ({cmd})"
+    return None, "no instruction counts profiled for this run's launch knobs"
 
 
 def sort_bytes(key_bits):
@@ -338,6 +372,7 @@ def main():
     graph_value = N * gsteps / (time.perf_counter() - t0)
     per_step = {k: v / a.steps for k, v in agg.items()}
     overflow = sim.hash_overflow()
+    sim_knobs = sim.tuning()
     X, V = sim.state()
     sim.close()
 
@@ -355,10 +390,23 @@ def main():
                          "avg_launch_ms": round(ms / launches, 5), "ms_per_step": round(ms / a.steps, 5),
                          "bytes_per_launch": nbytes / launches, "GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1),
                          "frac": round(nbytes / (ms * 1e-3) / HBM_PEAK, 5)}
+    knobs = sim_knobs
+    for name, kr in kernels.items():
+        valu, _ = measured_valu(name, a.warmup, a.steps, knobs)
+        if valu is not None:
+            kr["valu_issue_frac"] = round(valu / (kr["avg_launch_ms"] * 1e-3) / VALU_PEAK, 4)
     dominant = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
     dk = kernels[dominant]
     achieved = dk["bytes_per_launch"] / (dk["avg_launch_ms"] * 1e-3)
     traffic, traffic_note = measured_traffic(dominant)
+    valu, valu_note = measured_valu(dominant, a.warmup, a.steps, knobs)
+    valu_roofline = {"bound": "valu", "kernel": dominant, "achieved": None, "frac": None, "note": valu_note}
+    if valu is not None:
+        rate = valu / (dk["avg_launch_ms"] * 1e-3)
+        valu_roofline = {"bound": "valu", "kernel": dominant, "achieved": round(rate, 1), "peak": round(VALU_PEAK, 1),
+                         "unit": "wave64 VALU instructions/s", "frac": round(rate / VALU_PEAK, 4),
+                         "instructions_per_launch": valu, "avg_launch_ms": dk["avg_launch_ms"],
+                         "source": valu_note + "; launch time measured live"}
 
     out = {
         "metric": "particle-steps/s @1M (2 iters); HBM GB/s vs peak; batched & adjoint/fwd ratio",
@@ -381,6 +429,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5), "traffic": traffic,
                      "traffic_note": traffic_note, "bytes_per_launch": dk["bytes_per_launch"],
                      "avg_launch_ms": dk["avg_launch_ms"]},
+        "valu_roofline": valu_roofline,
         "graph_replay_particle_steps_per_s": round(graph_value, 1),
         "graph_replay_steps": f"{a.warmup + a.steps}..{a.warmup + a.steps + gsteps}",
         "tuning": tuning,
