@@ -23,18 +23,19 @@ constexpr int kBS = 256;  // lanes per block of every kernel that walks
 // particle may be read.
 //
 // Two walk modes per neighbour row, chosen wave-uniformly.  Stream: when the union
-// of the lanes' slot spans is at most rows.streamRatio times the longest
-// lane span, the wave streams the union through a per-wave LDS buffer, 64
-// candidates at a time loaded coalesced, and every lane tests each candidate (an
-// LDS broadcast) against its own span, itself and the distance h; with chunkBits
-// it tests a whole chunk into a bit mask first (rows.chunkBits).  Per lane: otherwise each lane
-// walks its own span from global memory, kUnroll candidates a round so their
-// loads are in flight together.  When a wave spans several cells the union carries
-// candidates most lanes do not need, and the per-lane walk is faster.
+// of the lanes' slot spans is at most rows.streamRatio times the longest lane span,
+// the wave streams the union through a per-wave LDS buffer, 64 candidates at a time
+// loaded coalesced, and every lane tests each candidate (an LDS broadcast) against
+// its own span, itself and the distance h; with rows.chunkBits it tests a whole
+// chunk into a bit mask first.  Per lane: otherwise each lane walks its own span
+// from global memory, kUnroll candidates a round so their loads are in flight
+// together.  When a wave spans several cells the union carries candidates most
+// lanes do not need, and the per-lane walk is faster.
 //
-// Deferred pairs.  In-range slots go to a per-lane FIFO in LDS; the expensive
+// Deferred pairs.  In-range candidates go to a per-lane FIFO in LDS; the expensive
 // per-pair work (sqrt, IEEE divisions, the s_corr power) runs when some lane's
-// FIFO is full, for one queued pair on every lane that has one, and at the end.
+// FIFO is full (in the per-lane walk: when it has no room for a round's kUnroll
+// pushes), for one queued pair on every lane that has one, and at the end.
 // Otherwise a wave would execute the per-pair path whenever any of its 64 lanes
 // finds a neighbour, at low utilisation.
 //
