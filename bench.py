@@ -106,6 +106,18 @@ def measured_valu(kernel, warmup, steps, knobs):
     return None, "no instruction counts profiled for this run's launch knobs"
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def sort_bytes(key_bits):
     passes = max(1, -(-key_bits // 8))
     return 16.0 * passes + 4.0
@@ -468,7 +480,8 @@ def main():
                                "sample": f"dam_break_1m, compact grid, steps 2..{1 + a.cpu_steps} after one "
                                          f"warm-up step, {cpu_s:.2f} s, OpenMP over particles",
                                "single_thread_value": round(N / cpu1_s, 1),
-                               "single_thread_sample": f"step 2 on one thread, {cpu1_s:.2f} s"}
+                               "single_thread_sample": f"step 2 on one thread, {cpu1_s:.2f} s",
+                               "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
     print(json.dumps(out))
     return 0
 
