@@ -78,8 +78,8 @@ def measured_traffic(kernel):
     return k["bytes_per_launch"], f"{os.path.relpath(TRAFFIC_FILE, REPO)} ({rec.get('command', '')})"
 
 
-# Launch knobs that change what the neighbour kernels execute (the sort's keys per
-# thread does not).
+# Launch knobs that change what the neighbour kernels execute per wave (the sort's
+# knobs do not, nor does the block size, which only groups the same waves).
 WALK_KNOBS = ("candidate_mask", "stream_ratio", "chunk_bits")
 
 

@@ -64,19 +64,20 @@ extern "C" int pbfSimPublishVelocity(PbfSim* sim, uint32_t count) {
 
 extern "C" int pbfSimSetTuning(PbfSim* sim, const PbfSimTuning* t) {
     if (!sim || !t) return -1;
-    return sim->s.setTuning(t->candidateMask != 0, t->streamRatio, t->chunkBits != 0, t->sortItems) ? 0 : -1;
+    const sim::Simulator::Tuning k{t->candidateMask != 0, t->streamRatio, t->chunkBits != 0, t->sortItems,
+                              t->sortDigitBits,      t->walkBlock};
+    return sim->s.setTuning(k) ? 0 : -1;
 }
 
 extern "C" int pbfSimGetTuning(PbfSim* sim, PbfSimTuning* t) {
     if (!sim || !t) return -1;
-    bool cm = false, cb = false;
-    uint32_t ratio = 0;
-    int items = 0;
-    sim->s.getTuning(cm, ratio, cb, items);
-    t->sortItems = items;
-    t->candidateMask = cm ? 1 : 0;
-    t->streamRatio = ratio;
-    t->chunkBits = cb ? 1 : 0;
+    const sim::Simulator::Tuning k = sim->s.tuning();
+    t->candidateMask = k.candidateMask ? 1 : 0;
+    t->streamRatio = k.streamRatio;
+    t->chunkBits = k.chunkBits ? 1 : 0;
+    t->sortItems = k.sortItems;
+    t->sortDigitBits = k.sortDigitBits;
+    t->walkBlock = k.walkBlock;
     return 0;
 }
 

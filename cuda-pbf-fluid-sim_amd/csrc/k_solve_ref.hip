@@ -86,7 +86,7 @@ __global__ __launch_bounds__(kBS) void k_lambda_ref(float* __restrict__ lambda, 
         lambda_pair(a, rx, ry, rz, rx * rx + ry * ry + rz * rz, c);
     };
     const CellRows<Lookup> rows(lookup, dp.grid, reach_of(c.h, dp.grid.cellSize), keyS[ss]);
-    neighbor_walk<false>(valid, s, rows, CandMask{}, IndexedFetch{P, idxS}, sBuf + (threadIdx.x & ~63u),
+    neighbor_walk<false, kBS>(valid, s, rows, CandMask{}, IndexedFetch{P, idxS}, sBuf + (threadIdx.x & ~63u),
                          sQ + threadIdx.x, dp.maxNeighbors, c.h2, dist2, apply);
     if (valid) lambda[i] = lambda_finish(a, dp);
 }
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(kBS) void k_delta_ref(float4* __restrict__ delta, c
         delta_pair(dx, dy, dz, rx, ry, rz, rx * rx + ry * ry + rz * rz, lam_i, lambda[j], c);
     };
     const CellRows<Lookup> rows(lookup, dp.grid, reach_of(c.h, dp.grid.cellSize), keyS[ss]);
-    neighbor_walk<false>(valid, s, rows, CandMask{}, IndexedFetch{P, idxS}, sBuf + (threadIdx.x & ~63u),
+    neighbor_walk<false, kBS>(valid, s, rows, CandMask{}, IndexedFetch{P, idxS}, sBuf + (threadIdx.x & ~63u),
                          sQ + threadIdx.x, 0, c.h2, dist2, apply);
     if (!valid) return;
     const float3 d = delta_finish(dx, dy, dz, dp);
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(kBS) void k_xsph_ref(float4* __restrict__ Vout, con
         ++n;
     };
     const CellRows<Lookup> rows(lookup, dp.grid, reach_of(dp.kernel.h, dp.grid.cellSize), keyS[ss]);
-    neighbor_walk<false>(valid, s, rows, CandMask{}, IndexedFetch{P, idxS}, sBuf + (threadIdx.x & ~63u),
+    neighbor_walk<false, kBS>(valid, s, rows, CandMask{}, IndexedFetch{P, idxS}, sBuf + (threadIdx.x & ~63u),
                          sQ + threadIdx.x, 0, h2, dist2, apply);
     if (!valid) return;
     const float3 v = xsph_finish(vi.x, vi.y, vi.z, sx, sy, sz, n, dp, dp.xsph_c);

@@ -24,8 +24,9 @@ namespace {
 
 using namespace walk;
 
+template <int BS>
 __device__ __forceinline__ uint32_t slot_of_thread() {
-    return xcd_tile(blockIdx.x, gridDim.x) * kBS + threadIdx.x;
+    return xcd_tile(blockIdx.x, gridDim.x) * BS + threadIdx.x;
 }
 
 __global__ __launch_bounds__(kBS) void k_predict_hash(const float4* __restrict__ X, const float4* __restrict__ V,
@@ -224,14 +225,14 @@ struct SortedFetch {
 // Walk of the set mask bits (then the unmasked tail) in walk order, per lane.
 // Each round takes up to kUnroll candidates, so their position loads are in flight
 // together; the next mask word is loaded while the current one is consumed.
-template <class Dist2, class Apply>
+template <int kLanes, class Dist2, class Apply>
 __device__ __forceinline__ void mask_walk(bool valid, uint32_t s, uint32_t m, const SpanTable& st,
                                           const CandMask& mk, const float4* P, uint32_t* q, int cap, float h2,
                                           Dist2&& dist2, Apply&& apply) {
     int qh = 0, qn = 0, found = 0;
     auto pop_one = [&]() {
         if (qn > 0) {
-            const uint32_t k = q[qh * kBS];
+            const uint32_t k = q[qh * kLanes];
             qh = (qh + 1) & (kQD - 1);
             --qn;
             apply(k);
@@ -321,7 +322,7 @@ __device__ __forceinline__ void mask_walk(bool valid, uint32_t s, uint32_t m, co
 #pragma unroll
         for (uint32_t u = 0; u < kUnroll; ++u) {  // room for kUnroll pushes was made above
             if ((bits >> u) & 1u) {
-                q[((qh + qn) & (kQD - 1)) * kBS] = ks[u];
+                q[((qh + qn) & (kQD - 1)) * kLanes] = ks[u];
                 ++qn;
             }
         }
@@ -330,12 +331,13 @@ __device__ __forceinline__ void mask_walk(bool valid, uint32_t s, uint32_t m, co
 }
 
 // Lambda of a sorted slot: the full walk (first iteration) or the mask walk.
-__global__ __launch_bounds__(kBS) void k_lambda_sorted(float4* Ps, const uint32_t* __restrict__ segOf, SpanTable st,
+template <int BS>
+__global__ __launch_bounds__(BS) void k_lambda_sorted(float4* Ps, const uint32_t* __restrict__ segOf, SpanTable st,
                                                        CandMask mk, int useMask, sim::DeviceParams dp, uint32_t N,
                                                        float4* __restrict__ recA, uint2* __restrict__ recB) {
-    __shared__ uint32_t sQ[kQD * kBS];
-    __shared__ float4 sBuf[kBS];
-    const uint32_t s = slot_of_thread();
+    __shared__ uint32_t sQ[kQD * BS];
+    __shared__ float4 sBuf[BS];
+    const uint32_t s = slot_of_thread<BS>();
     const bool valid = s < N;
     const uint32_t ss = valid ? s : N - 1;
     const float4 pi = Ps[ss];
@@ -355,9 +357,9 @@ __global__ __launch_bounds__(kBS) void k_lambda_sorted(float4* Ps, const uint32_
         if (recB && a.n == cap) cut = k;
     };
     if (useMask)
-        mask_walk(valid, s, m, st, mk, Ps, sQ + threadIdx.x, dp.maxNeighbors, c.h2, dist2, apply);
+        mask_walk<BS>(valid, s, m, st, mk, Ps, sQ + threadIdx.x, dp.maxNeighbors, c.h2, dist2, apply);
     else
-        neighbor_walk<false>(valid, s, SpanRows(st, m), mk, SortedFetch{Ps}, sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x,
+        neighbor_walk<false, BS>(valid, s, SpanRows(st, m), mk, SortedFetch{Ps}, sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x,
                              dp.maxNeighbors, c.h2, dist2, apply);
     if (!valid) return;
     reinterpret_cast<float*>(Ps)[4ull * s + 3] = lambda_finish(a, dp);
@@ -371,16 +373,17 @@ __global__ __launch_bounds__(kBS) void k_lambda_sorted(float4* Ps, const uint32_
 
 // Delta of a sorted slot.  maskMode: 0 full walk, 1 full walk that builds the
 // candidate mask, 2 mask walk.
-__global__ __launch_bounds__(kBS) void k_delta_sorted(const float4* __restrict__ Pa, float4* __restrict__ Pb,
+template <int BS>
+__global__ __launch_bounds__(BS) void k_delta_sorted(const float4* __restrict__ Pa, float4* __restrict__ Pb,
                                                       const uint32_t* __restrict__ segOf, SpanTable st,
                                                       CandMask mk, int maskMode, sim::DeviceParams dp, uint32_t N,
                                                       int last, const float4* __restrict__ X,
                                                       const uint32_t* __restrict__ idxS, float4* __restrict__ Vs,
                                                       float invDt, Scenes sc, uint32_t* __restrict__ clampBits,
                                                       float4* __restrict__ recU) {
-    __shared__ uint32_t sQ[kQD * kBS];
-    __shared__ float4 sBuf[kBS];
-    const uint32_t s = slot_of_thread();
+    __shared__ uint32_t sQ[kQD * BS];
+    __shared__ float4 sBuf[BS];
+    const uint32_t s = slot_of_thread<BS>();
     const bool valid = s < N;
     const uint32_t ss = valid ? s : N - 1;
     const float4 pi = Pa[ss];
@@ -398,12 +401,12 @@ __global__ __launch_bounds__(kBS) void k_delta_sorted(const float4* __restrict__
     };
     float4* buf = sBuf + (threadIdx.x & ~63u);
     if (maskMode == 2)
-        mask_walk(valid, s, m, st, mk, Pa, sQ + threadIdx.x, 0, c.h2, dist2, apply);
+        mask_walk<BS>(valid, s, m, st, mk, Pa, sQ + threadIdx.x, 0, c.h2, dist2, apply);
     else if (maskMode == 1)
-        neighbor_walk<true>(valid, s, SpanRows(st, m), mk, SortedFetch{Pa}, buf, sQ + threadIdx.x, 0, c.h2, dist2,
+        neighbor_walk<true, BS>(valid, s, SpanRows(st, m), mk, SortedFetch{Pa}, buf, sQ + threadIdx.x, 0, c.h2, dist2,
                             apply);
     else
-        neighbor_walk<false>(valid, s, SpanRows(st, m), mk, SortedFetch{Pa}, buf, sQ + threadIdx.x, 0, c.h2, dist2,
+        neighbor_walk<false, BS>(valid, s, SpanRows(st, m), mk, SortedFetch{Pa}, buf, sQ + threadIdx.x, 0, c.h2, dist2,
                              apply);
     if (!valid) return;
     const float3 e = delta_scaled(dx, dy, dz, dp);
@@ -442,14 +445,15 @@ __global__ __launch_bounds__(kBS) void k_finalize_sorted(float4* __restrict__ Ps
 // Eight waves per SIMD (64 VGPRs): the walk is latency-bound on its candidate and
 // velocity loads, and the extra waves hide more of it than the few spilled registers
 // cost (measured on the collapsed 1M dam break).
-__global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(8))) void k_xsph_scatter(const float4* __restrict__ Ps, const float4* __restrict__ Vs,
+template <int BS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8))) void k_xsph_scatter(const float4* __restrict__ Ps, const float4* __restrict__ Vs,
                                                       const uint32_t* __restrict__ segOf, SpanTable st,
                                                       CandMask mk, int useMask, sim::DeviceParams dp, uint32_t N,
                                                       const uint32_t* __restrict__ idxS, float4* __restrict__ Xn,
                                                       float4* __restrict__ V, Scenes sc, float* __restrict__ recGm) {
-    __shared__ uint32_t sQ[kQD * kBS];
-    __shared__ float4 sBuf[kBS];
-    const uint32_t s = slot_of_thread();
+    __shared__ uint32_t sQ[kQD * BS];
+    __shared__ float4 sBuf[BS];
+    const uint32_t s = slot_of_thread<BS>();
     const bool valid = s < N;
     const uint32_t ss = valid ? s : N - 1;
     const float4 pi = Ps[ss];
@@ -486,9 +490,9 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(8))) void k
             ++n;
         };
         if (useMask)
-            mask_walk(valid && walk, s, m, st, mk, Ps, sQ + threadIdx.x, 0, h2, dist2, apply);
+            mask_walk<BS>(valid && walk, s, m, st, mk, Ps, sQ + threadIdx.x, 0, h2, dist2, apply);
         else
-            neighbor_walk<false>(valid && walk, s, SpanRows(st, m), mk, SortedFetch{Ps}, sBuf + (threadIdx.x & ~63u),
+            neighbor_walk<false, BS>(valid && walk, s, SpanRows(st, m), mk, SortedFetch{Ps}, sBuf + (threadIdx.x & ~63u),
                                  sQ + threadIdx.x, 0, h2, dist2, apply);
         if (recGm && valid)
             recGm[s] = xsph_gate(n, dp) * ((dp.restDensity > 0.f) ? (dp.particleMass / dp.restDensity) : 0.f);
@@ -563,7 +567,11 @@ void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st
         set_error(PBFX_E_INVALID_ARGUMENT, "launch_lambda_sorted: recA and recB go together");
         return;
     }
-    k_lambda_sorted<<<blocks_for(N), kBS, 0, s>>>(Ps, segOf, st, mk, useMask ? 1 : 0, dp, N, recA, recB);
+    const int um = useMask ? 1 : 0;
+    if (st.block == 128)
+        k_lambda_sorted<128><<<ceil_div(N, 128u), 128, 0, s>>>(Ps, segOf, st, mk, um, dp, N, recA, recB);
+    else
+        k_lambda_sorted<256><<<ceil_div(N, 256u), 256, 0, s>>>(Ps, segOf, st, mk, um, dp, N, recA, recB);
     check_launch("lambda_sorted");
 }
 
@@ -572,8 +580,13 @@ void launch_delta_sorted(const float4* Pa, float4* Pb, const uint32_t* segOf, co
                          const float4* X, const uint32_t* idxS, float4* Vs, float invDt, const Scenes& sc,
                          hipStream_t s, uint32_t* clampBits, float4* recU) {
     if (N == 0) return;
-    k_delta_sorted<<<blocks_for(N), kBS, 0, s>>>(Pa, Pb, segOf, st, mk, maskMode, dp, N, last ? 1 : 0, X, idxS, Vs,
-                                                 invDt, sc, clampBits, recU);
+    const int lastPass = last ? 1 : 0;
+    if (st.block == 128)
+        k_delta_sorted<128><<<ceil_div(N, 128u), 128, 0, s>>>(Pa, Pb, segOf, st, mk, maskMode, dp, N, lastPass, X,
+                                                              idxS, Vs, invDt, sc, clampBits, recU);
+    else
+        k_delta_sorted<256><<<ceil_div(N, 256u), 256, 0, s>>>(Pa, Pb, segOf, st, mk, maskMode, dp, N, lastPass, X,
+                                                              idxS, Vs, invDt, sc, clampBits, recU);
     check_launch("delta_sorted");
 }
 
@@ -590,8 +603,13 @@ void launch_xsph_scatter(const float4* Ps, const float4* Vs, const uint32_t* seg
                          const uint32_t* idxS, float4* Xn, float4* V, const Scenes& sc, hipStream_t s,
                          float* recGm) {
     if (N == 0) return;
-    k_xsph_scatter<<<blocks_for(N), kBS, 0, s>>>(Ps, Vs, segOf, st, mk, useMask ? 1 : 0, dp, N, idxS, Xn, V, sc,
-                                                 recGm);
+    const int um = useMask ? 1 : 0;
+    if (st.block == 128)
+        k_xsph_scatter<128><<<ceil_div(N, 128u), 128, 0, s>>>(Ps, Vs, segOf, st, mk, um, dp, N, idxS, Xn, V, sc,
+                                                              recGm);
+    else
+        k_xsph_scatter<256><<<ceil_div(N, 256u), 256, 0, s>>>(Ps, Vs, segOf, st, mk, um, dp, N, idxS, Xn, V, sc,
+                                                              recGm);
     check_launch("xsph_scatter");
 }
 

@@ -1,6 +1,7 @@
 // k_sort.hip — stable LSD radix sort of (key, value) pairs for wave64.
 //
-// Per 8-bit digit pass, three kernels:
+// Per digit pass (digits of at most 8 bits; the bits to sort are spread evenly
+// over ceil(bits / maxDigitBits) passes), three kernels:
 //   hist     each tile counts its digits (wave64 match via 8 ballots, one LDS
 //            atomic per distinct digit per wave), writes counts[digit][tile];
 //   scanrows one block per digit turns its row into exclusive tile offsets and
@@ -30,11 +31,10 @@ __device__ __forceinline__ uint32_t popc_below(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// Lanes of this wave (among `valid`) holding the same 8-bit digit.
-__device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
+// Lanes of this wave (among `valid`) holding the same digit of `bits` bits (<= 8).
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid, int bits) {
     uint64_t m = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < kRadixBits; ++b) {
+    for (int b = 0; b < bits; ++b) {
         const bool bit = (d >> b) & 1u;
         const uint64_t bb = __ballot(bit);
         m &= bit ? bb : ~bb;
@@ -67,7 +67,7 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* sh, ui
 
 template <int ITEMS>
 __global__ __launch_bounds__(kBS) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t N, int shift,
-                                                    uint32_t mask, uint32_t* __restrict__ counts,
+                                                    uint32_t mask, int bits, uint32_t* __restrict__ counts,
                                                     uint32_t numTiles) {
     constexpr uint32_t T = kBS * ITEMS;
     __shared__ uint32_t hist[kR];
@@ -80,7 +80,7 @@ __global__ __launch_bounds__(kBS) void k_radix_hist(const uint32_t* __restrict__
         const uint64_t idx = base + (uint64_t)k * kBS + threadIdx.x;
         const bool valid = idx < N;
         const uint32_t d = valid ? ((keys[idx] >> shift) & mask) : 0u;
-        const uint64_t peers = match_digit(d, valid);
+        const uint64_t peers = match_digit(d, valid, bits);
         if (valid && popc_below(peers) == 0) atomicAdd(&hist[d], (uint32_t)__popcll(peers));
     }
     __syncthreads();
@@ -111,7 +111,7 @@ template <int ITEMS>
 __global__ __launch_bounds__(kBS) void k_radix_scatter(const uint32_t* __restrict__ kin,
                                                        const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
                                                        uint32_t* __restrict__ vout, uint32_t N, int shift,
-                                                       uint32_t mask, const uint32_t* __restrict__ counts,
+                                                       uint32_t mask, int bits, const uint32_t* __restrict__ counts,
                                                        const uint32_t* __restrict__ totals, uint32_t numTiles) {
     constexpr uint32_t T = kBS * ITEMS;
     constexpr uint32_t TW = T / kWaves;  // elements per wave sub-tile
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kBS) void k_radix_scatter(const uint32_t* __restric
         const uint64_t idx = waveBase + (uint64_t)k * 64 + lane;
         const bool valid = idx < N;
         const uint32_t d = (key[k] >> shift) & mask;
-        const uint64_t peers = match_digit(d, valid);
+        const uint64_t peers = match_digit(d, valid, bits);
         const uint32_t pre = sWaveHist[w][d];
         rank[k] = pre + popc_below(peers);
         const uint32_t leader = 63u - (uint32_t)__clzll(peers);
@@ -228,12 +228,17 @@ Layout layout_for(uint32_t N, int items = 0) {
     return L;
 }
 
+// One pass over the digit (key >> shift) & mask of `bits` bits.  Digits above the
+// mask never occur: their count rows stay zero from the histogram and are not
+// scanned, and their totals (possibly left by an earlier, wider pass) only shift the
+// bases of those absent digits.
 template <int ITEMS>
 void launch_pass(const Layout& L, const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
-                 uint32_t N, int shift, uint32_t mask, uint32_t* counts, uint32_t* totals, hipStream_t s) {
-    k_radix_hist<ITEMS><<<L.numTiles, kBS, 0, s>>>(kin, N, shift, mask, counts, L.numTiles);
-    k_radix_scan_rows<<<kR, kBS, 0, s>>>(counts, L.numTiles, totals);
-    k_radix_scatter<ITEMS><<<L.numTiles, kBS, 0, s>>>(kin, vin, kout, vout, N, shift, mask, counts, totals,
+                 uint32_t N, int shift, uint32_t mask, int bits, uint32_t* counts, uint32_t* totals,
+                 hipStream_t s) {
+    k_radix_hist<ITEMS><<<L.numTiles, kBS, 0, s>>>(kin, N, shift, mask, bits, counts, L.numTiles);
+    k_radix_scan_rows<<<(int)mask + 1, kBS, 0, s>>>(counts, L.numTiles, totals);
+    k_radix_scatter<ITEMS><<<L.numTiles, kBS, 0, s>>>(kin, vin, kout, vout, N, shift, mask, bits, counts, totals,
                                                       L.numTiles);
 }
 
@@ -244,8 +249,14 @@ void launch_pass(const Layout& L, const uint32_t* kin, const uint32_t* vin, uint
 size_t radix_sort_temp_bytes(uint32_t N, int, int) { return layout_for(N, 4).bytes; }
 
 bool radix_sort_pairs(void* temp, size_t tempBytes, const uint32_t* kin, uint32_t* kout, const uint32_t* vin,
-                      uint32_t* vout, uint32_t N, int beginBit, int endBit, hipStream_t s, int items) {
+                      uint32_t* vout, uint32_t N, int beginBit, int endBit, hipStream_t s, int items,
+                      int digitBits) {
     if (N == 0) return true;
+    if (digitBits == 0) digitBits = kRadixBits;
+    if (digitBits < 1 || digitBits > kRadixBits) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "radix sort: digit width must be 1..8 bits");
+        return false;
+    }
     const Layout L = layout_for(N, items);
     if (!temp || tempBytes < L.bytes) {
         set_error(PBFX_E_CAPACITY, "radix sort: temp storage too small");
@@ -268,8 +279,8 @@ bool radix_sort_pairs(void* temp, size_t tempBytes, const uint32_t* kin, uint32_
         check_launch("radix sort copy");
         return true;
     }
-    const int passes = (bits + kRadixBits - 1) / kRadixBits;
-    // spread the bits evenly over the passes (e.g. 21 bits -> 7, 7, 7)
+    const int passes = (bits + digitBits - 1) / digitBits;
+    // spread the bits evenly over the passes (e.g. 21 bits at 8 -> 7, 7, 7)
     const int width = (bits + passes - 1) / passes;
     const uint32_t* srcK = kin;
     const uint32_t* srcV = vin;
@@ -281,9 +292,9 @@ bool radix_sort_pairs(void* temp, size_t tempBytes, const uint32_t* kin, uint32_
         uint32_t* dK = toOut ? kout : altK;
         uint32_t* dV = toOut ? vout : altV;
         switch (L.items) {
-            case 4: launch_pass<4>(L, srcK, srcV, dK, dV, N, shift, mask, counts, totals, s); break;
-            case 8: launch_pass<8>(L, srcK, srcV, dK, dV, N, shift, mask, counts, totals, s); break;
-            default: launch_pass<16>(L, srcK, srcV, dK, dV, N, shift, mask, counts, totals, s); break;
+            case 4: launch_pass<4>(L, srcK, srcV, dK, dV, N, shift, mask, wbits, counts, totals, s); break;
+            case 8: launch_pass<8>(L, srcK, srcV, dK, dV, N, shift, mask, wbits, counts, totals, s); break;
+            default: launch_pass<16>(L, srcK, srcV, dK, dV, N, shift, mask, wbits, counts, totals, s); break;
         }
         srcK = dK;
         srcV = dV;

@@ -152,7 +152,7 @@ void SortedPredict(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStre
 void SortedGridBuild(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_t s) {
     const uint32_t N = p.numParticles;
     pbfx::radix_sort_pairs(g.d_sortTemp, g.sortTempBytes, g.d_cellKeys, g.d_cellKeys_sorted, nullptr,
-                           g.d_indices_sorted, N, 0, g.keyBits, s, g.sortItems);
+                           g.d_indices_sorted, N, 0, g.keyBits, s, g.sortItems, g.sortDigitBits);
     mark("sort");
     pbfx::compact_segments(g.d_cellUniqueKeys, g.d_cellOffsets, g.d_compactCount, g.d_segOf, g.d_cellKeys_sorted, N,
                            g.d_blockCounts, g.d_blockOffsets, s);
@@ -721,15 +721,22 @@ bool Simulator::bindExternalPositions(float4* a, float4* b, uint32_t capacity) {
     return true;
 }
 
-bool Simulator::setTuning(bool candidateMask, uint32_t streamRatio, bool chunkBits, int sortItems) {
-    if (!m_initialized || streamRatio < 1 || !(sortItems == 0 || sortItems == 4 || sortItems == 8 || sortItems == 16)) {
-        set_error(PBFX_E_INVALID_ARGUMENT, "setTuning: streamRatio >= 1, sortItems in {0, 4, 8, 16}");
+bool Simulator::setTuning(const Tuning& t) {
+    const bool itemsOk = t.sortItems == 0 || t.sortItems == 4 || t.sortItems == 8 || t.sortItems == 16;
+    const bool digitOk = t.sortDigitBits == 0 || (t.sortDigitBits >= 4 && t.sortDigitBits <= 8);
+    const bool blockOk = t.walkBlock == 0 || t.walkBlock == 128 || t.walkBlock == 256;
+    if (!m_initialized || t.streamRatio < 1 || !itemsOk || !digitOk || !blockOk) {
+        set_error(PBFX_E_INVALID_ARGUMENT,
+                  "setTuning: streamRatio >= 1, sortItems in {0, 4, 8, 16}, sortDigitBits 0 or 4..8, "
+                  "walkBlock in {0, 128, 256}");
         return false;
     }
-    m_grid.sortItems = sortItems;
-    m_grid.candidateMask = candidateMask;
-    m_grid.spans.streamRatio = streamRatio;
-    m_grid.spans.chunkBits = chunkBits ? 1u : 0u;
+    m_grid.sortItems = t.sortItems;
+    m_grid.sortDigitBits = t.sortDigitBits == 0 ? 8 : t.sortDigitBits;
+    m_grid.candidateMask = t.candidateMask;
+    m_grid.spans.streamRatio = t.streamRatio;
+    m_grid.spans.chunkBits = t.chunkBits ? 1u : 0u;
+    m_grid.spans.block = t.walkBlock == 0 ? 256u : (uint32_t)t.walkBlock;
     m_graphDirty = true;
     return true;
 }

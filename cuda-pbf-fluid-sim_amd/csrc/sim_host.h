@@ -82,6 +82,7 @@ struct GridBuffers {
     pbfx::CandMask mask{};  // per-step candidate mask (reordered path)
     bool candidateMask = true;  // later passes walk the mask (tuning knob)
     int sortItems = 0;          // radix sort keys per thread (0: by N; tuning knob)
+    int sortDigitBits = 8;      // radix sort widest digit of a pass (4..8 bits; tuning knob)
     pbfx::HashTable table{};
     pbfx::Scenes scenes{};  // ensemble view (empty for one scene)
     uint32_t tableLog2 = 0;
@@ -208,13 +209,19 @@ public:
     bool bindExternalPositions(float4* a, float4* b, uint32_t capacity);
     bool bindExternalVelocity(void* dst, size_t bytes, uint32_t strideBytes);
     bool publishVelocity(uint32_t count);
-    // Launch-shape knobs of the reordered solver (see PbfSimTuning).
-    bool setTuning(bool candidateMask, uint32_t streamRatio, bool chunkBits, int sortItems);
-    void getTuning(bool& candidateMask, uint32_t& streamRatio, bool& chunkBits, int& sortItems) const {
-        candidateMask = m_grid.candidateMask;
-        streamRatio = m_grid.spans.streamRatio;
-        chunkBits = m_grid.spans.chunkBits != 0;
-        sortItems = m_grid.sortItems;
+    // Launch-shape knobs of the reordered solver (the fields of PbfSimTuning).
+    struct Tuning {
+        bool candidateMask;
+        uint32_t streamRatio;
+        bool chunkBits;
+        int sortItems;
+        int sortDigitBits;
+        int walkBlock;
+    };
+    bool setTuning(const Tuning& t);
+    Tuning tuning() const {
+        return Tuning{m_grid.candidateMask, m_grid.spans.streamRatio, m_grid.spans.chunkBits != 0,
+                      m_grid.sortItems,     m_grid.sortDigitBits,     (int)m_grid.spans.block};
     }
     uint32_t sceneCount() const { return (uint32_t)m_scenes.size(); }
     const pbfx::SceneParams* sceneParams(uint32_t scene) const {

@@ -43,6 +43,8 @@ struct SpanTable {
     // streamed rows: 1 = each lane tests a whole 64-candidate chunk into a bit mask,
     // then queues its set bits; 0 = each candidate is tested and queued in turn
     uint32_t chunkBits = 1;
+    // lanes per block of the neighbour kernels (128 or 256)
+    uint32_t block = 256;
 };
 
 // Sorted position arrays hold this many slots past the last particle, so that the

@@ -9,7 +9,7 @@
 namespace pbfx {
 namespace walk {
 
-constexpr int kBS = 256;  // lanes per block of every kernel that walks
+constexpr int kBS = 256;  // lanes per block of the kernels that walk (default)
 
 // Neighbour walk of one wave (64 consecutive sorted slots, one particle per lane).
 //
@@ -71,7 +71,8 @@ constexpr uint32_t kMaskWords = kCandMaskWords;
 // Bits [0, n) of a 64-bit word (n <= 64).
 __device__ __forceinline__ uint64_t low_bits64(uint32_t n) { return n >= 64u ? ~0ull : ((1ull << n) - 1ull); }
 
-template <bool kBuild, class Rows, class Fetch, class Dist2, class Apply>
+// kLanes: lanes per block of the calling kernel (the FIFO's stride in LDS).
+template <bool kBuild, int kLanes, class Rows, class Fetch, class Dist2, class Apply>
 __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, const Rows& rows, const CandMask& mk,
                                               const Fetch& fetch, float4* buf, uint32_t* q, int cap, float h2,
                                               Dist2&& dist2, Apply&& apply) {
@@ -79,13 +80,13 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, const Rows
     int qh = 0, qn = 0, found = 0;
     uint32_t cidx = 0, word = 0;  // mask building: own-candidate index and pending word
     auto push = [&](uint32_t k) {
-        q[((qh + qn) & (kQD - 1)) * kBS] = k;
+        q[((qh + qn) & (kQD - 1)) * kLanes] = k;
         ++qn;
         ++found;
     };
     auto pop_one = [&]() {
         if (qn > 0) {
-            const uint32_t k = q[qh * kBS];
+            const uint32_t k = q[qh * kLanes];
             qh = (qh + 1) & (kQD - 1);
             --qn;
             apply(k);
@@ -173,7 +174,7 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, const Rows
 #pragma unroll
                 for (uint32_t u = 0; u < kUnroll; ++u) {  // room for kUnroll pushes was made above
                     if ((bits >> u) & 1u) {
-                        q[((qh + qn) & (kQD - 1)) * kBS] = id[u];
+                        q[((qh + qn) & (kQD - 1)) * kLanes] = id[u];
                         ++qn;
                     }
                 }
@@ -223,7 +224,7 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, const Rows
                     if (bits != 0ull && qn < kQD) {
                         const uint32_t t = (uint32_t)__ffsll((long long)bits) - 1u;
                         bits &= bits - 1ull;
-                        q[((qh + qn) & (kQD - 1)) * kBS] = fetch.id(base + t);
+                        q[((qh + qn) & (kQD - 1)) * kLanes] = fetch.id(base + t);
                         ++qn;
                     }
                 }

@@ -20,6 +20,8 @@ SPACE = {
     "stream_ratio": (1, 2, 3, 6),
     "chunk_bits": (True, False),
     "sort_items": (0, 4, 8, 16),
+    "sort_digit_bits": (8, 6, 4),
+    "walk_block": (256, 128),
 }
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "pbfx", "tune.json")

@@ -115,6 +115,10 @@ typedef struct {
     int chunkBits;         // streamed rows: 1 = test a 64-candidate chunk into a bit mask, then
                            // queue the set bits; 0 = test and queue candidate by candidate
     int sortItems;         // radix sort keys per thread: 4, 8 or 16 (0 = chosen by N)
+    int sortDigitBits;     // radix sort: widest digit of a pass, 4..8 bits (0 = 8); the key bits
+                           // are spread evenly over ceil(bits / sortDigitBits) passes
+    int walkBlock;         // lanes per block of the neighbour kernels: 128 or 256 (0 = 256)
+                           // (pbfSimGetTuning reports the effective values: 8 and 256 for 0)
 } PbfSimTuning;
 int pbfSimSetTuning(PbfSim* sim, const PbfSimTuning* tuning);
 int pbfSimGetTuning(PbfSim* sim, PbfSimTuning* tuning);

@@ -24,9 +24,10 @@ def test_every_setting_gives_the_same_bits(dev):
     try:
         X0, V0 = sim.state()
         ref = None
-        for cm, ratio, cb, items in itertools.product(*tune.SPACE.values()):
-            print(f"setting mask={cm} ratio={ratio} chunk={cb} items={items}", flush=True)
-            sim.set_tuning(candidate_mask=cm, stream_ratio=ratio, chunk_bits=cb, sort_items=items)
+        for setting in itertools.product(*tune.SPACE.values()):
+            knobs = dict(zip(tune.SPACE, setting))
+            print(f"setting {knobs}", flush=True)
+            sim.set_tuning(**knobs)
             sim.set_state(X0, V0)
             sim.step(3)
             X, V = sim.state()
@@ -67,5 +68,9 @@ def test_bad_knobs_are_rejected(dev):
             sim.set_tuning(stream_ratio=0)
         with pytest.raises(Exception):
             sim.set_tuning(sort_items=5)
+        with pytest.raises(Exception):
+            sim.set_tuning(sort_digit_bits=3)
+        with pytest.raises(Exception):
+            sim.set_tuning(walk_block=64)
     finally:
         sim.close()
