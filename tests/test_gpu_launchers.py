@@ -239,6 +239,44 @@ def test_lambda_delta_xsph_reach_two(oracle, dev, L, grid):
     np.testing.assert_array_equal(to_host(tVout), eV)
 
 
+@pytest.mark.parametrize("grid", ["dense", "compact"])
+def test_lambda_delta_xsph_partial_wave(oracle, dev, L, grid):
+    # 1001 particles (13 x 11 x 7), squeezed: the last wave has 41 lanes
+    import torch
+    scene = P.small_block((13, 11, 7))
+    sp, X, V, Pp = predicted_state(oracle, scene, 0, (0.7, 0.15))
+    g = grid_of(oracle, Pp, sp)
+    t = {k: to_dev(v, dev) for k, v in g.items()}
+    t["cnt"] = to_dev(np.array([len(g["uniq"])], np.uint32), dev)
+    dp = P.make_device_params(sp)
+    N = len(Pp)
+    assert N % 64 != 0
+    tP = to_dev(Pp, dev)
+    lam = torch.zeros(N, dtype=torch.float32, device=dev)
+    delta = torch.zeros_like(tP)
+    Vin = np.random.default_rng(7).normal(0, 0.5, (N, 4)).astype(np.float32)
+    tVout = torch.zeros_like(tP)
+    if grid == "dense":
+        look, elook = (t["start"], t["end"]), (g["start"], g["end"])
+        L.lambda_dense(lam, tP, t["idxS"], t["keyS"], *look, dp)
+        elam = oracle.lambda_dense(Pp, g["idxS"], g["keyS"], *elook, dp)
+        L.delta_apply_dense(tP, delta, lam, t["idxS"], t["keyS"], *look, dp)
+        eP, edelta = oracle.delta_apply_dense(Pp, elam, g["idxS"], g["keyS"], *elook, dp)
+        L.xsph_dense(tVout, to_dev(Vin, dev), tP, t["idxS"], t["keyS"], *look, dp)
+        eV = oracle.xsph_dense(Vin, eP, g["idxS"], g["keyS"], *elook, dp)
+    else:
+        L.lambda_compact(lam, tP, t["idxS"], t["keyS"], t["uniq"], t["off"], t["cnt"], dp)
+        elam = oracle.lambda_compact(Pp, g["idxS"], g["keyS"], g["uniq"], g["off"], dp)
+        L.delta_apply_compact(tP, delta, lam, t["idxS"], t["keyS"], t["uniq"], t["off"], t["cnt"], dp)
+        eP, edelta = oracle.delta_apply_compact(Pp, elam, g["idxS"], g["keyS"], g["uniq"], g["off"], dp)
+        L.xsph_compact(tVout, to_dev(Vin, dev), tP, t["idxS"], t["keyS"], t["uniq"], t["off"], t["cnt"], dp)
+        eV = oracle.xsph_compact(Vin, eP, g["idxS"], g["keyS"], g["uniq"], g["off"], dp)
+    np.testing.assert_array_equal(to_host(lam), elam)
+    np.testing.assert_array_equal(to_host(tP), eP)
+    np.testing.assert_array_equal(to_host(delta), edelta)
+    np.testing.assert_array_equal(to_host(tVout), eV)
+
+
 def test_lambda_neighbor_cap(oracle, dev, L):
     import torch
     scene = P.small_block((12, 12, 12))

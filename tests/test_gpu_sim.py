@@ -67,6 +67,22 @@ def test_ten_steps_small_block(oracle, dev, grid):
     np.testing.assert_array_equal(V, eV)
 
 
+@pytest.mark.parametrize("dims", [(1, 1, 1), (2, 1, 1), (7, 3, 3), (13, 5, 1), (13, 11, 7)])
+@pytest.mark.parametrize("path,grid", [("reordered", "hashed"), ("reordered", "dense"), ("reference", "compact")])
+def test_particle_counts_off_the_wave_size(oracle, dev, dims, path, grid):
+    # 1, 2, 63, 65 and 1001 particles: the last wave (and block) of every kernel is partly empty
+    scene = P.small_block(dims, table_log2=14)
+    sim = _sim(scene, grid, path)
+    try:
+        sim.step(4)
+        X, V = sim.state()
+    finally:
+        sim.close()
+    eX, eV = _oracle_state(oracle, scene, 4, compact=(grid != "dense"))
+    np.testing.assert_array_equal(X, eX)
+    np.testing.assert_array_equal(V, eV)
+
+
 def _fine_cells(sp, scene, frac=0.6):
     """Cells of frac * h: the neighbourhood then reaches two cells each way (25 rows)."""
     cell = np.float32(frac) * np.float32(sp.kernel.h)
