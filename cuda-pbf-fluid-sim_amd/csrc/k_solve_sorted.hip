@@ -517,7 +517,9 @@ inline uint32_t blocks_for(uint32_t n) { return ceil_div(n > 0 ? n : 1, kBS); }
 float mask_skin_radius2(const sim::SimParams& p) {
     if (!p.pbf.enable_disp_clamp || p.solverIters < 1 || !(p.pbf.disp_clamp_max_h >= 0.f)) return 0.f;
     const double h = p.kernel.h;
-    const double rs = h * (1.0 + 2.0 * p.solverIters * (double)p.pbf.disp_clamp_max_h);
+    // the mask serves iterations 1 .. I-1, whose positions are at most I-1 clamped
+    // corrections away from the ones it was built on (XSPH walks without it)
+    const double rs = h * (1.0 + 2.0 * (p.solverIters - 1) * (double)p.pbf.disp_clamp_max_h);
     if (rs > 2.0 * h) return 0.f;
     const double r = rs * (1.0 + 1e-3);  // margin for float rounding of positions and r^2
     return (float)(r * r);

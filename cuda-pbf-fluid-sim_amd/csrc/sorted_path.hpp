@@ -64,8 +64,8 @@ struct CandMask {
     float rs2 = 0.f;
 };
 
-// Squared skin radius for `iters` clamped iterations, or 0 when no mask applies
-// (no displacement clamp, or a skin wider than 2h).
+// Squared skin radius of the candidate mask for p's solver iterations, or 0 when no
+// mask applies (no displacement clamp, or a skin wider than 2h).
 float mask_skin_radius2(const sim::SimParams& p);
 
 // Per-scene parameters of a batched ensemble.  All scenes share h, rest density,
