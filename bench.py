@@ -310,7 +310,7 @@ def main():
                     help="step at which the autotuner times the launch knobs (the dam break has collapsed "
                          "by then); the state is then restored to the end of the warm-up")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch knobs")
-    ap.add_argument("--tune-steps", type=int, default=3)
+    ap.add_argument("--tune-steps", type=int, default=5)
     ap.add_argument("--tuning", default=None,
                     help='fixed launch knobs as JSON (e.g. {"candidate_mask": true, "stream_ratio": 1}); '
                          'skips the autotuner, as profiler runs do to time one setting')
