@@ -205,14 +205,17 @@ def bench_batched(count, warmup, steps, tune_steps=0, grid="dense"):
             "speedup_vs_concurrent_streams": round(t_streams / t_ens, 3), "tuning": best}
 
 
-def bench_adjoint_batched(steps, checkpoint, rollouts=64):
+def bench_adjoint_batched(steps, checkpoint, rollouts=64, knobs=None):
     """64 differentiable rollouts (each the 64x32x32 configuration, own jitter seed and
-    target velocity field) advanced and differentiated together as one ensemble."""
+    target velocity field) advanced and differentiated together as one ensemble.
+    knobs: launch knobs to apply (the neighbour walk's), or None for the defaults."""
     import torch
     from pbfx.diff import EnsembleRollout, differentiable_scene, occupancy_grid, seeded_velocity
     scenes = [differentiable_scene(seed=7 + k) for k in range(rollouts)]
     roll = EnsembleRollout(scenes, steps, checkpoint=checkpoint)
     try:
+        if knobs:
+            roll.sim.set_tuning(**knobs)
         X0, _ = roll.ensemble.state()
         n = len(X0)
         h = float(roll.params.kernel.h)
@@ -248,11 +251,12 @@ def bench_adjoint_batched(steps, checkpoint, rollouts=64):
             "rollout_steps_per_s": round(rollouts * steps / (t_fwd + t_bwd), 1)}
 
 
-def bench_adjoint(steps=60, checkpoint=10, rollouts=64):
+def bench_adjoint(steps=60, checkpoint=10, rollouts=64, knobs=None):
     """The differentiable rollout: 65,536 particles (64x32x32 lattice), 60 steps, checkpoint every
     10.  Loss: squared L2 distance between the 32^3 Gaussian occupancy grid (sigma = h) of X_T and
     that of a target rollout from a seeded perturbed initial velocity.  Times the checkpointed
-    forward, then the reverse pass split into re-recording the segments and the reverse kernels."""
+    forward, then the reverse pass split into re-recording the segments and the reverse kernels.
+    knobs: the neighbour walk's launch knobs to apply to forward and reverse, or None."""
     import torch
     from pbfx.diff import Rollout, differentiable_scene, initial_state, occupancy_grid, seeded_velocity
     scene = differentiable_scene()
@@ -263,6 +267,8 @@ def bench_adjoint(steps=60, checkpoint=10, rollouts=64):
     x0 = torch.from_numpy(X0).cuda()
     roll = Rollout(sp, steps, checkpoint=checkpoint)
     try:
+        if knobs:
+            roll.sim.set_tuning(**knobs)
         vt = torch.from_numpy(seeded_velocity(n, 77, 0.5)).cuda()
         XT, _ = roll.forward(x0, vt, sp.xsph_c)
         target = occupancy_grid(XT[:, :3].double(), scene.box_min, scene.box_max, cells=32, sigma=h)
@@ -288,8 +294,9 @@ def bench_adjoint(steps=60, checkpoint=10, rollouts=64):
         roll.close()
     print(f"[bench] adjoint rollout {n} particles x {steps} steps: forward {t_fwd:.4f} s, backward {t_bwd:.4f} s",
           file=sys.stderr, flush=True)
-    batched = bench_adjoint_batched(steps, checkpoint) if rollouts > 1 else None
+    batched = bench_adjoint_batched(steps, checkpoint, rollouts, knobs) if rollouts > 1 else None
     return {"particles": n, "steps": steps, "checkpoint": checkpoint, "loss": float(loss.detach()), "batched": batched,
+            "knobs": knobs or "defaults",
             "forward_ms_per_step": round(t_fwd * 1e3 / steps, 4),
             "record_ms_per_step": round(tm["record_s"] * 1e3 / steps, 4),
             "reverse_ms_per_step": round(tm["reverse_s"] * 1e3 / steps, 4),
@@ -459,7 +466,12 @@ def main():
                                        0 if a.no_tune else a.tune_steps)
 
     if not a.no_adjoint:
-        out["adjoint"] = bench_adjoint()
+        # the dam break's tuned walk knobs (the rollout's states are dam breaks too)
+        walk = None
+        if tuning and "chosen" in tuning:
+            walk = {k: tuning["chosen"][k] for k in ("candidate_mask", "stream_ratio", "chunk_bits", "walk_block")
+                    if k in tuning["chosen"]}
+        out["adjoint"] = bench_adjoint(knobs=walk)
 
     if not a.no_cpu_baseline:
         from oracle import oracle as O

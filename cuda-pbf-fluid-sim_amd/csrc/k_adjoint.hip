@@ -42,11 +42,12 @@
 #include "pbf_common.hpp"
 #include "pbf_hip.h"
 #include "pbf_math.hpp"
+#include "walk.hpp"
 
 namespace pbfx {
 namespace {
 
-constexpr int kBS = 256;
+using namespace walk;
 
 __device__ __forceinline__ float3 f3(const float4& v) { return make_float3(v.x, v.y, v.z); }
 __device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
@@ -83,17 +84,11 @@ __device__ __forceinline__ float3 hess_apply(const Spiky& s, float3 r, float3 v)
     return add3(mul3(v, s.f), mul3(r, s.dfr * dot3(r, v)));
 }
 
-// Walk every neighbour slot k != s in walk order; fn(row, k) returns false to stop.
-template <class Fn>
-__device__ __forceinline__ void walk_all(uint32_t s, uint32_t m, const SpanTable& st, Fn&& fn) {
-    for (int r = 0; r < st.rows; ++r) {
-        const uint2 sp = st.spans[(uint64_t)r * st.stride + m];
-        for (uint32_t k = sp.x; k < sp.y; ++k) {
-            if (k == s) continue;
-            if (!fn(r, k)) return;
-        }
-    }
-}
+// The neighbour kernels below gather over every in-range neighbour j != k of their
+// slot with the forward kernels' wave walk (walk.hpp): the same candidates, the same
+// r^2 <= h^2 test and the same ascending-slot order, with the pair work deferred
+// through the per-lane FIFO so a wave runs it with most lanes busy.  Lanes past N
+// take part in the walk's wave-level steps with no rows.
 
 __global__ __launch_bounds__(kBS) void k_adj_gather(const uint32_t* __restrict__ idxS, const float4* __restrict__ gX,
                                                     const float4* __restrict__ gV, float4* __restrict__ gPf,
@@ -137,21 +132,28 @@ __global__ __launch_bounds__(kBS) void k_adj_xsph(const float4* __restrict__ Pf,
                                                   float4* __restrict__ gVn, float4* __restrict__ gPf,
                                                   double* __restrict__ partial, double* __restrict__ gcSlot) {
     __shared__ double sh[kBS];
+    __shared__ uint32_t sQ[kQD * kBS];
+    __shared__ float4 sBuf[kBS];
     const uint32_t s = blockIdx.x * kBS + threadIdx.x;
+    const bool valid = s < N;
+    const uint32_t ss = valid ? s : N - 1;
     double gcl = 0.0;
-    if (s < N) {
+    {
         const float h2 = dp.kernel.h2, poly6 = dp.kernel.poly6;
         // neighbours share the slot's scene: rows never leave a scene's key range
-        const float c = xsph_c_of(s, dp, sc, sceneOfSlot);
-        const float3 pk = f3(Pf[s]), vk = f3(Vn[s]), gk = f3(gVo[s]);
-        const float gmk = gm[s];
+        const float c = xsph_c_of(ss, dp, sc, sceneOfSlot);
+        const float3 pk = f3(Pf[ss]), vk = f3(Vn[ss]), gk = f3(gVo[ss]);
+        const float gmk = gm[ss];
         const float kk = c * gmk;
         float sumW = 0.f;
         float3 accV = make_float3(0.f, 0.f, 0.f), accP = accV, dv = accV;
-        walk_all(s, segOf[s], st, [&](int, uint32_t j) {
+        auto dist2 = [&](const float4& pj) {
+            const float3 r = sub3(pk, f3(pj));
+            return dot3(r, r);
+        };
+        auto apply = [&](uint32_t j) {
             const float3 r = sub3(pk, f3(Pf[j]));
             const float r2 = dot3(r, r);
-            if (r2 > h2) return true;
             const float hr2 = h2 - r2;
             const float W = poly6 * hr2 * hr2 * hr2;
             const float3 vj = f3(Vn[j]);
@@ -165,12 +167,15 @@ __global__ __launch_bounds__(kBS) void k_adj_xsph(const float4* __restrict__ Pf,
                 accV = add3(accV, mul3(gj, kj * W));
                 accP = add3(accP, mul3(dW, dot3(sub3(mul3(gk, kk), mul3(gj, kj)), dvj)));
             }
-            return true;
-        });
-        gVn[s] = mk4(add3(mul3(gk, 1.0f - kk * sumW), accV), 0.f);
-        gPf[s] = mk4(add3(f3(gPf[s]), accP), 0.f);
-        gcl = (double)gmk * (double)dot3(gk, dv);
-        if (gcSlot) gcSlot[s] = gcl;
+        };
+        neighbor_walk<false, kBS>(valid, s, SpanRows(st, segOf[ss]), CandMask{}, SortedFetch{Pf},
+                                  sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x, 0, h2, dist2, apply);
+        if (valid) {
+            gVn[s] = mk4(add3(mul3(gk, 1.0f - kk * sumW), accV), 0.f);
+            gPf[s] = mk4(add3(f3(gPf[s]), accP), 0.f);
+            gcl = (double)gmk * (double)dot3(gk, dv);
+            if (gcSlot) gcSlot[s] = gcl;
+        }
     }
     if (gcSlot) return;  // ensembles reduce per scene (k_adj_scene_sum)
     sh[threadIdx.x] = gcl;
@@ -225,18 +230,24 @@ __global__ __launch_bounds__(kBS) void k_adj_delta(const float4* __restrict__ P,
                                                    const float4* __restrict__ A, const float4* __restrict__ lamA,
                                                    const uint2* __restrict__ lamB, float4* __restrict__ gP,
                                                    float4* __restrict__ lamQ, uint2* __restrict__ lamE) {
+    __shared__ uint32_t sQ[kQD * kBS];
+    __shared__ float4 sBuf[kBS];
     const uint32_t s = blockIdx.x * kBS + threadIdx.x;
-    if (s >= N) return;
+    const bool valid = s < N;
+    const uint32_t ss = valid ? s : N - 1;
     const PairConsts c = make_pair_consts(dp);
-    const float4 pk4 = P[s];
-    const float3 pk = f3(pk4), ak = f3(A[s]);
+    const float4 pk4 = P[ss];
+    const float3 pk = f3(pk4), ak = f3(A[ss]);
     float gl = 0.f;
     float3 acc = make_float3(0.f, 0.f, 0.f);
-    walk_all(s, segOf[s], st, [&](int, uint32_t j) {
+    auto dist2 = [&](const float4& pj) {
+        const float3 r = sub3(pk, f3(pj));
+        return dot3(r, r);
+    };
+    auto apply = [&](uint32_t j) {
         const float4 pj4 = P[j];
         const float3 r = sub3(pk, f3(pj4));
         const float r2 = dot3(r, r);
-        if (r2 > c.h2) return true;
         const Spiky sp = spiky_of(r2, c);
         const float3 g = mul3(r, sp.f);
         const float3 da = sub3(ak, f3(A[j]));
@@ -259,8 +270,10 @@ __global__ __launch_bounds__(kBS) void k_adj_delta(const float4* __restrict__ P,
         }
         const float ckj = pk4.w + pj4.w + scorr;
         acc = add3(acc, add3(mul3(hess_apply(sp, r, da), ckj), mul3(g, dsr * dot3(r, da))));
-        return true;
-    });
+    };
+    neighbor_walk<false, kBS>(valid, s, SpanRows(st, segOf[ss]), CandMask{}, SortedFetch{P},
+                              sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x, 0, c.h2, dist2, apply);
+    if (!valid) return;
     gP[s] = mk4(add3(f3(gP[s]), acc), 0.f);
     const float4 la = lamA[s];
     const uint2 lb = lamB[s];
@@ -275,36 +288,44 @@ __global__ __launch_bounds__(kBS) void k_adj_lambda(const float4* __restrict__ P
                                                     const float4* __restrict__ lamQ, const uint2* __restrict__ lamE,
                                                     const float4* __restrict__ U, float4* __restrict__ gP,
                                                     float4* __restrict__ A) {
+    __shared__ uint32_t sQ[kQD * kBS];
+    __shared__ float4 sBuf[kBS];
     const uint32_t s = blockIdx.x * kBS + threadIdx.x;
-    if (s >= N) return;
+    const bool valid = s < N;
+    const uint32_t ss = valid ? s : N - 1;
     const PairConsts c = make_pair_consts(dp);
-    const float3 pk = f3(P[s]);
-    const float4 qk = lamQ[s];
-    const uint2 ek = lamE[s];
+    const float3 pk = f3(P[ss]);
+    const float4 qk = lamQ[ss];
+    const uint2 ek = lamE[ss];
     const float3 Gk = f3(qk);
     const float bk = qk.w, epk = __uint_as_float(ek.x);
     const uint32_t cutK = ek.y;
     const float poly6 = dp.kernel.poly6;
     const bool own = bk != 0.f || epk != 0.f;
     float3 acc = make_float3(0.f, 0.f, 0.f);
-    walk_all(s, segOf[s], st, [&](int, uint32_t j) {
+    auto dist2 = [&](const float4& pj) {
+        const float3 r = sub3(pk, f3(pj));
+        return dot3(r, r);
+    };
+    auto apply = [&](uint32_t j) {
         const float3 r = sub3(pk, f3(P[j]));
         const float r2 = dot3(r, r);
-        if (r2 > c.h2) return true;
         const float4 qj = lamQ[j];
         const uint2 ej = lamE[j];
         const float epj = __uint_as_float(ej.x);
         const bool inK = own && j <= cutK;                       // j in Ncap(k)
         const bool inJ = (qj.w != 0.f || epj != 0.f) && s <= ej.y;  // k in Ncap(j)
-        if (!inK && !inJ) return true;
+        if (!inK && !inJ) return;
         const float hr2 = c.h2 - r2;
         const float3 dW6 = mul3(r, -6.0f * poly6 * hr2 * hr2);
         const Spiky sp = spiky_of(r2, c);
         const float3 g = mul3(r, sp.f);
         if (inK) acc = add3(acc, add3(mul3(dW6, bk), mul3(hess_apply(sp, r, add3(g, Gk)), 2.0f * epk)));
         if (inJ) acc = add3(acc, add3(mul3(dW6, qj.w), mul3(hess_apply(sp, r, sub3(g, f3(qj))), 2.0f * epj)));
-        return true;
-    });
+    };
+    neighbor_walk<false, kBS>(valid, s, SpanRows(st, segOf[ss]), CandMask{}, SortedFetch{P},
+                              sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x, 0, c.h2, dist2, apply);
+    if (!valid) return;
     const float3 g = add3(f3(gP[s]), acc);
     gP[s] = mk4(g, 0.f);
     if (U) A[s] = mk4(disp_adjoint(g, U[s], dp), 0.f);

@@ -242,5 +242,29 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, const Rows
     pop_above(0);
 }
 
+// Rows of a sorted slot's neighbourhood from the per-step span table.
+struct SpanRows {
+    const SpanTable& st;
+    uint32_t m;  // occupied-cell index of the slot
+    uint32_t streamRatio, chunkBits;
+    __device__ __forceinline__ SpanRows(const SpanTable& t, uint32_t cell)
+        : st(t), m(cell), streamRatio(t.streamRatio), chunkBits(t.chunkBits) {}
+    __device__ __forceinline__ int count() const { return st.rows; }
+    __device__ __forceinline__ void span(int r, uint32_t& lo, uint32_t& hi) const {
+        const uint2 sp = st.spans[(uint64_t)r * st.stride + m];
+        lo = sp.x;
+        hi = sp.y;
+    }
+};
+
+// Sorted positions, padded by kSlotPad slots past the last particle; a particle's id
+// is its slot.
+struct SortedFetch {
+    static constexpr bool kPadded = true;
+    const float4* P;
+    __device__ __forceinline__ uint32_t id(uint32_t k) const { return k; }
+    __device__ __forceinline__ float4 at(uint32_t k) const { return P[k]; }
+};
+
 }  // namespace walk
 }  // namespace pbfx
