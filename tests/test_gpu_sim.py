@@ -3,9 +3,12 @@
 One step: positions within 1e-5*h and velocities within 1e-5*h/dt of the oracle.
 Many steps: the dam break is chaotic (SI-unit PBF with a saturating displacement
 clamp), so after 200 steps the density-constraint statistics must agree within 1%.
-At 1M particles the oracle is too slow, so the three grid kinds are checked against
-each other (they must give identical bits) and against size-independent invariants.
+At 1M particles the default run checks the three grid kinds against each other (they
+must give identical bits) and against size-independent invariants; with PBF_LONG_PARITY=1
+a 200-step run is also compared with the oracle bit for bit (minutes of oracle time).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -173,6 +176,29 @@ def test_1m_grid_kinds_identical_and_invariants(dev):
     hi = np.array(list(sp.grid.maxs), np.float32)
     assert np.all(np.isfinite(X)) and np.all(X[:, :3] >= lo) and np.all(X[:, :3] <= hi)
     assert np.all(X[:, 3] == 1.0) and np.all(V[:, 3] == 0.0)
+
+
+@pytest.mark.timeout(1200)
+@pytest.mark.skipif(os.environ.get("PBF_LONG_PARITY") != "1",
+                    reason="about 3 minutes of multi-threaded oracle time; set PBF_LONG_PARITY=1")
+def test_1m_200_steps_bit_identical_to_oracle(oracle, dev):
+    """dam_break_1m at full size, 200 steps on the hashed grid with the candidate mask (the
+    benchmark's setting): bit-identical to the oracle after every 20 steps."""
+    scene = P.dam_break_1m()
+    sp = P.build_sim_params(scene)
+    sim = _sim(scene, "hashed")
+    sim.set_tuning(candidate_mask=True, stream_ratio=1, chunk_bits=False)
+    X = P.seed_scene_reference(scene, oracle.seed_lattice)
+    V = np.zeros_like(X)
+    for done in range(20, 201, 20):
+        sim.step(20)
+        X, V = oracle.run(X, V, sp, 20, compact=True)
+        gX, gV = sim.state()
+        print(f"step {done}: bit-identical {np.array_equal(gX, X) and np.array_equal(gV, V)}", flush=True)
+        np.testing.assert_array_equal(gX, X)
+        np.testing.assert_array_equal(gV, V)
+    assert not sim.hash_overflow()
+    sim.close()
 
 
 def test_8m_splash_hashed_equals_compact(dev):
