@@ -54,13 +54,90 @@ __device__ __forceinline__ int scorr_int_exponent(float n) {
     return (n >= 1.0f && n <= 16.0f && (float)(int)n == n) ? (int)n : 0;
 }
 
+// ------------------------------------------------ unscaled IEEE sequences
+// The compiler's IEEE sqrtf and division on gfx950 wrap a core sequence in scaling
+// steps for operands near the ends of the exponent range.  Where the pair arithmetic
+// can prove its operands stay clear of those ends (pair_unscaled_ok), the core
+// sequence alone returns the same bits in fewer instructions.
+
+// Square root: the raw v_sqrt_f32 (within 1 ulp), then the one-ulp neighbour whose
+// residual x - s*s has the right sign.  Correctly rounded for 2^-96 <= x < inf (the
+// compiler scales smaller inputs by 2^32 first); for 0 <= x < 2^-96 the result is at
+// most 2^-47.
+__device__ __forceinline__ float sqrt_unscaled(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sdn = __uint_as_float(__float_as_uint(s) - 1u);
+    const float sup = __uint_as_float(__float_as_uint(s) + 1u);
+    float r = (__builtin_fmaf(-sdn, s, x) <= 0.f) ? sdn : s;
+    r = (__builtin_fmaf(-sup, s, x) > 0.f) ? sup : r;
+    return r;
+}
+
+// Reciprocal of b refined by one Newton step: the first half of the division below,
+// which depends on b alone (so a constant divisor pays for it once).
+__device__ __forceinline__ float rcp_refined(float b) {
+    const float y = __builtin_amdgcn_rcpf(b);
+    return __builtin_fmaf(__builtin_fmaf(-b, y, 1.0f), y, y);
+}
+
+// a / b from y = rcp_refined(b): a product and two Newton corrections of the quotient
+// (v_div_scale and v_div_fixup dropped, v_div_fmas as a plain fma).  Those steps are
+// the identity when b and a / b are normal, 1 / b is normal, |a| >= 2^-103 and a's
+// exponent exceeds b's by less than 96; there this is the correctly rounded quotient.
+// a = +-0 gives +0 (IEEE: a zero with the sign of a / b), which no sum in the pair
+// arithmetic can tell apart: every accumulator starts at +0.
+__device__ __forceinline__ float div_unscaled(float a, float b, float y) {
+    float q = a * y;
+    float e = __builtin_fmaf(-b, q, a);
+    q = __builtin_fmaf(e, y, q);
+    e = __builtin_fmaf(-b, q, a);
+    return __builtin_fmaf(e, y, q);
+}
+
+// max(a, b) for a, b >= +0 and not NaN, whose bit patterns order as their values
+// (equal to fmaxf there, without its NaN-quieting step).
+__device__ __forceinline__ float max_nonneg(float a, float b) {
+    return __uint_as_float(max(__float_as_uint(a), __float_as_uint(b)));
+}
+
 // Constants hoisted out of the neighbour loops (each is the first product the
 // reference's left-to-right expression forms, so hoisting does not change bits).
 struct PairConsts {
     float h, h2, poly6, m_poly6, neg3_spiky, grad_r_eps;
     float w_q, neg_scorr_k, scorr_n, scorr_min;
     int scorr_enable, scorr_ni;
+    float rcp_w_q;  // rcp_refined(w_q)
+    bool unscaled;  // pair_unscaled_ok: the pair arithmetic uses the unscaled sequences
 };
+
+// Quotients a / b with a in {0} or [amin, amax] and b in [bmin, bmax] (all positive)
+// meet div_unscaled's conditions, with margins of 2^6 and more on every bound.
+__device__ __forceinline__ bool div_unscaled_range(double amin, double amax, double bmin, double bmax) {
+    return amin >= 0x1p-100 && amax <= 0x1p100 && bmin >= 0x1p-100 && bmax <= 0x1p100 && amax / bmin <= 0x1p90 &&
+           amin / bmax >= 0x1p-100;
+}
+
+// Whether every square root and division of lambda_pair and delta_pair is exact in
+// its unscaled form, from bounds on their operands:
+//  - r = sqrt(r2), r2 in [0, h2]: below 2^-96 the unscaled result is at most 2^-47,
+//    and r enters the sums only through t = h - r (== h for r < ulp(h) / 2) and
+//    max(r, eps) (== eps for r < eps), so h >= 2^-16 and eps >= 2^-40 make it moot;
+//  - coeff = (neg3_spiky * t^2) / max(r, eps): t = h - r is 0 or at least h * 2^-25
+//    in magnitude (h - r is exact when r >= h / 2, a multiple of half an ulp of h),
+//    and r <= h (1 + 2^-23);
+//  - ratio = poly6 * hr2^3 / w_q: hr2 = h2 - r2 is 0 or at least h2 * 2^-25 for
+//    the same reason.
+__device__ __forceinline__ bool pair_unscaled_ok(const PairConsts& c) {
+    const double h = c.h, eps = c.grad_r_eps, k = fabs((double)c.neg3_spiky);
+    if (!(h >= 0x1p-16 && h <= 0x1p16 && eps >= 0x1p-40 && eps <= h)) return false;
+    const double tmin = h * 0x1p-26, tmax = h * (1.0 + 0x1p-20);
+    if (!div_unscaled_range(k * tmin * tmin, k * tmax * tmax, eps, tmax)) return false;
+    if (c.scorr_enable && c.w_q > 0.f) {
+        const double p = c.poly6, lo = (double)c.h2 * 0x1p-26, hi = (double)c.h2 * (1.0 + 0x1p-20);
+        if (!div_unscaled_range(p * lo * lo * lo, p * hi * hi * hi, c.w_q, c.w_q)) return false;
+    }
+    return true;
+}
 
 __device__ __forceinline__ PairConsts make_pair_consts(const sim::DeviceParams& dp) {
     PairConsts c;
@@ -78,6 +155,8 @@ __device__ __forceinline__ PairConsts make_pair_consts(const sim::DeviceParams& 
     c.scorr_min = dp.pbf.scorr_min;
     c.scorr_enable = dp.pbf.scorr_enable;
     c.scorr_ni = scorr_int_exponent(dp.pbf.scorr_n);
+    c.rcp_w_q = rcp_refined(c.w_q);
+    c.unscaled = pair_unscaled_ok(c);
     return c;
 }
 
@@ -92,10 +171,18 @@ __device__ __forceinline__ void lambda_pair(LambdaAcc& a, float rx, float ry, fl
     a.n++;
     const float hr2 = c.h2 - r2;
     a.density += c.m_poly6 * hr2 * hr2 * hr2;
-    const float r = sqrtf(r2);
-    const float r_safe = fmaxf(r, c.grad_r_eps);
-    const float t = c.h - r;
-    const float coeff = c.neg3_spiky * (t * t) / r_safe;
+    float coeff;
+    if (c.unscaled) {
+        const float r = sqrt_unscaled(r2);  // +0 or positive: r2 is a sum of squares
+        const float r_safe = max_nonneg(r, c.grad_r_eps);
+        const float t = c.h - r;
+        coeff = div_unscaled(c.neg3_spiky * (t * t), r_safe, rcp_refined(r_safe));
+    } else {
+        const float r = sqrtf(r2);
+        const float r_safe = fmaxf(r, c.grad_r_eps);
+        const float t = c.h - r;
+        coeff = c.neg3_spiky * (t * t) / r_safe;
+    }
     const float ax = coeff * rx, ay = coeff * ry, az = coeff * rz;
     a.gx += ax;
     a.gy += ay;
@@ -148,16 +235,24 @@ __device__ __forceinline__ void lambda_adjoint_coefs(const LambdaAcc& a, const s
 // One in-range neighbour of the position-correction sum.
 __device__ __forceinline__ void delta_pair(float& dx, float& dy, float& dz, float rx, float ry, float rz, float r2,
                                            float lam_i, float lam_j, const PairConsts& c) {
-    const float r = sqrtf(r2);
-    const float t = c.h - r;
-    const float r_safe = fmaxf(r, c.grad_r_eps);
-    const float coeff = c.neg3_spiky * (t * t) / r_safe;
+    float coeff;
+    if (c.unscaled) {
+        const float r = sqrt_unscaled(r2);  // +0 or positive: r2 is a sum of squares
+        const float t = c.h - r;
+        const float r_safe = max_nonneg(r, c.grad_r_eps);
+        coeff = div_unscaled(c.neg3_spiky * (t * t), r_safe, rcp_refined(r_safe));
+    } else {
+        const float r = sqrtf(r2);
+        const float t = c.h - r;
+        const float r_safe = fmaxf(r, c.grad_r_eps);
+        coeff = c.neg3_spiky * (t * t) / r_safe;
+    }
     const float ax = coeff * rx, ay = coeff * ry, az = coeff * rz;
     float scorr = 0.0f;
     if (c.scorr_enable) {
         const float hr2 = c.h2 - r2;
         const float w = c.poly6 * hr2 * hr2 * hr2;
-        const float ratio = (c.w_q > 0.f) ? (w / c.w_q) : 0.0f;
+        const float ratio = (c.w_q > 0.f) ? (c.unscaled ? div_unscaled(w, c.w_q, c.rcp_w_q) : (w / c.w_q)) : 0.0f;
         scorr = c.neg_scorr_k * pow_scorr(ratio, c.scorr_n, c.scorr_ni);
         if (c.scorr_min < 0.f) scorr = fmaxf(scorr, c.scorr_min);
     }

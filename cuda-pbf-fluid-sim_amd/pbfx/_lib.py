@@ -77,6 +77,8 @@ SIGNATURES = {
     "pbfDebugEchoDeviceParams": (None, [DeviceParams, ctypes.POINTER(DeviceParams)]),
     "pbfDebugEchoGridBounds": (None, [GridBounds, ctypes.POINTER(GridBounds)]),
     "pbfDebugEchoFloat3": (None, [Float3, ctypes.POINTER(ctypes.c_float)]),
+    "pbfDebugPairUnscaled": (_I, [DeviceParams]),
+    "pbfDebugCheckUnscaledMath": (_I, [ctypes.c_uint64, ctypes.c_uint64, _P]),
 }
 
 _lib = None

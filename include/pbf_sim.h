@@ -176,6 +176,20 @@ void pbfDebugEchoDeviceParams(sim::DeviceParams dp, sim::DeviceParams* out);
 void pbfDebugEchoGridBounds(sim::GridBounds g, sim::GridBounds* out);
 void pbfDebugEchoFloat3(float3 v, float* out3);
 
+// Arithmetic self-checks (device; no reference counterpart).  The neighbour kernels
+// evaluate the pair square roots and divisions without the compiler's range-scaling
+// steps when the parameters bound every operand away from the ends of the exponent
+// range, which returns the same bits as IEEE sqrtf and division.
+// pbfDebugPairUnscaled: 1 when the kernels take that form for dp, 0 when they take
+// the general one, -1 on error.
+int pbfDebugPairUnscaled(sim::DeviceParams dp);
+// pbfDebugCheckUnscaledMath: counts where the unscaled sequences differ from IEEE,
+// out5 = {square roots over every finite float >= 2^-96, roots of smaller inputs
+// that are not in [+0, 2^-47], quotients over divSamples pseudo-random pairs (seeded)
+// in the admitted operand range, zero numerators whose quotient is not +0, integer
+// max results that differ from fmaxf}.  Synchronous; 0 on success, -1 on error.
+int pbfDebugCheckUnscaledMath(uint64_t divSamples, uint64_t seed, unsigned long long* out5);
+
 #ifdef __cplusplus
 }
 #endif

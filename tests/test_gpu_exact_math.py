@@ -1,0 +1,73 @@
+"""The pair arithmetic's unscaled IEEE sequences (csrc/pbf_math.hpp): the same bits as
+sqrtf and division wherever the kernels use them, checked on the device; the scenes of
+the parity tests take that form, and parameters outside its range take the general
+one and still match the oracle bit for bit."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from pbfx import _lib
+from pbfx import params as P
+
+pytestmark = pytest.mark.gpu
+
+SCENES = {
+    "dam_break_1m": P.dam_break_1m,
+    "dam_break_32k": P.dam_break_32k,
+    "small_block": P.small_block,
+    "splash_8m": P.splash_8m,
+}
+
+
+def _unscaled(sp):
+    return _lib.lib().pbfDebugPairUnscaled(P.make_device_params(sp))
+
+
+def test_unscaled_sequences_match_ieee(dev):
+    out = (ctypes.c_ulonglong * 5)()
+    samples = 1 << 30
+    rc = _lib.lib().pbfDebugCheckUnscaledMath(samples, 0x9E3779B97F4A7C15, ctypes.addressof(out))
+    assert rc == 0, _lib.lib().pbfGetLastError()
+    names = ("sqrt mismatches (all floats >= 2^-96)", "tiny-input roots out of [0, 2^-47]",
+             f"division mismatches ({samples} pairs)", "zero numerators not giving +0", "integer max mismatches")
+    print(dict(zip(names, list(out))))
+    assert list(out) == [0, 0, 0, 0, 0], dict(zip(names, list(out)))
+
+
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_scenes_take_the_unscaled_form(dev, name):
+    assert _unscaled(P.build_sim_params(SCENES[name]())) == 1
+
+
+@pytest.mark.parametrize("field,value", [("grad_r_eps", 0.0), ("grad_r_eps", 1e-30)])
+def test_out_of_range_parameters_take_the_general_form(dev, field, value):
+    sp = P.build_sim_params(P.small_block())
+    setattr(sp.pbf, field, value)
+    assert _unscaled(sp) == 0
+
+
+def test_general_form_matches_oracle(oracle, dev):
+    """grad_r_eps = 1e-30 leaves the lattice's physics unchanged but takes the kernels off
+    the unscaled form (max(r, eps) no longer hides tiny roots): both forms agree with the
+    oracle, and with each other, bit for bit."""
+    from pbfx.sim import Simulator
+    states = []
+    for eps in (1e-6, 1e-30):
+        scene = P.small_block((14, 14, 14), table_log2=16)
+        sp = P.build_sim_params(scene)
+        sp.pbf.grad_r_eps = eps
+        assert _unscaled(sp) == (1 if eps == 1e-6 else 0)
+        sim = Simulator(sp, grid="dense", table_log2=scene.table_log2)
+        try:
+            sim.seed_scene(scene)
+            sim.step(4)
+            X, V = sim.state()
+        finally:
+            sim.close()
+        X0 = P.seed_scene_reference(scene, oracle.seed_lattice)
+        eX, eV = oracle.run(X0, np.zeros_like(X0), sp, 4)
+        np.testing.assert_array_equal(X, eX)
+        np.testing.assert_array_equal(V, eV)
+        states.append((X, V))
+    np.testing.assert_array_equal(states[0][0], states[1][0])
