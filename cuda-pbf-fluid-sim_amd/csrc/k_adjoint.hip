@@ -65,14 +65,18 @@ __device__ __forceinline__ float3 mask3(float3 g, uint32_t bits) {
 struct Spiky {
     float f, dfr;  // f(r) and (df/dr)/|r|
 };
+// One division per pair: 1 / r, correctly rounded, then products.
 __device__ __forceinline__ Spiky spiky_of(float r2, const PairConsts& c) {
-    const float r = sqrtf(r2);
+    const float r = c.unscaled ? sqrt_unscaled(r2) : sqrtf(r2);
     const float t = c.h - r;
     Spiky s;
     if (r > c.grad_r_eps) {
-        s.f = c.neg3_spiky * (t * t) / r;
+        // r in (eps, h]: inside div_unscaled's range whenever the pair arithmetic is
+        const float ir = c.unscaled ? div_unscaled(1.0f, r, rcp_refined(r)) : 1.0f / r;
+        const float t2 = t * t;
+        s.f = c.neg3_spiky * t2 * ir;
         // f = -3 spiky t^2 / r  ->  df/dr = 3 spiky (2 t / r + t^2 / r^2)
-        s.dfr = (-c.neg3_spiky) * (2.0f * t / r + (t * t) / (r * r)) / r;
+        s.dfr = (-c.neg3_spiky) * (2.0f * t * ir + t2 * ir * ir) * ir;
     } else {
         s.f = c.neg3_spiky * (t * t) / c.grad_r_eps;
         // r below the clamp: f depends on r only through t^2
@@ -80,6 +84,14 @@ __device__ __forceinline__ Spiky spiky_of(float r2, const PairConsts& c) {
     }
     return s;
 }
+// (W / Wq)^(n - 1), the power in s_corr's derivative: for integral n the product in
+// double that pow_scorr forms, otherwise powf.
+__device__ __forceinline__ float pow_scorr_deriv(float x, const PairConsts& c) {
+    if (c.scorr_ni == 1) return 1.0f;
+    if (c.scorr_ni > 1) return pow_scorr(x, c.scorr_n - 1.0f, c.scorr_ni - 1);
+    return powf(x, c.scorr_n - 1.0f);
+}
+
 __device__ __forceinline__ float3 hess_apply(const Spiky& s, float3 r, float3 v) {
     return add3(mul3(v, s.f), mul3(r, s.dfr * dot3(r, v)));
 }
@@ -236,6 +248,7 @@ __global__ __launch_bounds__(kBS) void k_adj_delta(const float4* __restrict__ P,
     const bool valid = s < N;
     const uint32_t ss = valid ? s : N - 1;
     const PairConsts c = make_pair_consts(dp);
+    const float inv_w_q = (c.w_q > 0.f) ? 1.0f / c.w_q : 0.f;
     const float4 pk4 = P[ss];
     const float3 pk = f3(pk4), ak = f3(A[ss]);
     float gl = 0.f;
@@ -257,15 +270,16 @@ __global__ __launch_bounds__(kBS) void k_adj_delta(const float4* __restrict__ P,
         if (c.scorr_enable) {
             const float hr2 = c.h2 - r2;
             const float W = c.poly6 * hr2 * hr2 * hr2;
-            const float ratio = (c.w_q > 0.f) ? (W / c.w_q) : 0.0f;
+            // the forward's ratio and s_corr, bit for bit
+            const float ratio = (c.w_q > 0.f) ? (c.unscaled ? div_unscaled(W, c.w_q, c.rcp_w_q) : (W / c.w_q)) : 0.0f;
             const float raw = c.neg_scorr_k * pow_scorr(ratio, c.scorr_n, c.scorr_ni);
             scorr = raw;
             if (c.scorr_min < 0.f && raw < c.scorr_min) {
                 scorr = c.scorr_min;
             } else if (c.w_q > 0.f && ratio > 0.f) {
                 // d/dr of -k (W/Wq)^n: -k n (W/Wq)^(n-1) / Wq * dW6, dW6 = -6 poly6 hr2^2 r
-                const float pn1 = powf(ratio, c.scorr_n - 1.0f);
-                dsr = c.neg_scorr_k * c.scorr_n * pn1 / c.w_q * (-6.0f * c.poly6 * hr2 * hr2);
+                const float pn1 = pow_scorr_deriv(ratio, c);
+                dsr = c.neg_scorr_k * c.scorr_n * pn1 * inv_w_q * (-6.0f * c.poly6 * hr2 * hr2);
             }
         }
         const float ckj = pk4.w + pj4.w + scorr;
