@@ -198,114 +198,6 @@ struct HashRowsDev {
     }
 };
 
-// Walk of the set mask bits (then the unmasked tail) in walk order, per lane.
-// Each round takes up to kUnroll candidates, so their position loads are in flight
-// together; the next mask word is loaded while the current one is consumed.
-template <int kLanes, class Dist2, class Apply>
-__device__ __forceinline__ void mask_walk(bool valid, uint32_t s, uint32_t m, const SpanTable& st,
-                                          const CandMask& mk, const float4* P, uint32_t* q, int cap, float h2,
-                                          Dist2&& dist2, Apply&& apply) {
-    int qh = 0, qn = 0, found = 0;
-    auto pop_one = [&]() {
-        if (qn > 0) {
-            const uint32_t k = q[qh * kLanes];
-            qh = (qh + 1) & (kQD - 1);
-            --qn;
-            apply(k);
-        }
-    };
-    // pops until no lane holds more than `keep` pairs (written as a do-while: the
-    // compiler then keeps the loop-carried sums in place instead of copying them
-    // on every pop)
-    auto pop_above = [&](int keep) {
-        if (__any(qn > keep)) do {
-                pop_one();
-            } while (__any(qn > keep));
-    };
-    uint32_t total = 0;
-    if (valid) {
-        for (int r = 0; r < st.rows; ++r) {
-            const uint2 sp = st.spans[(uint64_t)r * st.stride + m];
-            total += sp.y - sp.x;
-        }
-    }
-    // row cursor: candidate indices [rowStart, rowStart + rowLen) are slots rowLo + (ci - rowStart);
-    // candidate indices only grow, so the cursor only moves forward
-    int r = -1;
-    uint32_t rowStart = 0, rowLen = 0, rowLo = 0;
-    auto slot_of = [&](uint32_t ci) {
-        while (ci >= rowStart + rowLen) {
-            rowStart += rowLen;
-            ++r;
-            const uint2 sp = st.spans[(uint64_t)r * st.stride + m];
-            rowLo = sp.x;
-            rowLen = sp.y - sp.x;
-        }
-        return rowLo + (ci - rowStart);
-    };
-    const uint32_t nw = valid ? min((total + 31u) >> 5, kMaskWords) : 0u;
-    const uint32_t covered = kMaskWords * 32u;
-    uint32_t w = 0;
-    uint32_t word = nw > 0 ? mk.words[s] : 0u;
-    uint32_t next = nw > 1 ? mk.words[(uint64_t)mk.stride + s] : 0u;
-    uint32_t tail = covered;  // next unmasked candidate index past the mask
-    bool active = valid && total > 0;
-    while (__any(active)) {
-        pop_above(kQD - (int)kUnroll);
-        // ks[u] for u >= cnt repeats the last taken slot (slot 0 when none was taken), so
-        // every load below is in bounds and branch-free
-        uint32_t ks[kUnroll];
-        uint32_t cnt = 0, prev = 0u;
-        bool more = active;
-#pragma unroll
-        for (uint32_t u = 0; u < kUnroll; ++u) {
-            if (more) {
-                while (word == 0u && w + 1 < nw) {
-                    ++w;
-                    word = next;
-                    next = (w + 1 < nw) ? mk.words[(uint64_t)(w + 1) * mk.stride + s] : 0u;
-                }
-                uint32_t ci = 0u;
-                if (word != 0u) {
-                    ci = w * 32u + (uint32_t)__ffs((int)word) - 1u;
-                    word &= word - 1u;
-                } else if (tail < total) {
-                    ci = tail++;
-                } else {
-                    more = false;
-                }
-                if (more) {
-                    prev = slot_of(ci);
-                    ++cnt;
-                }
-            }
-            ks[u] = prev;
-        }
-        if (cnt < kUnroll) active = false;  // every candidate taken
-        float4 pj[kUnroll];
-#pragma unroll
-        for (uint32_t u = 0; u < kUnroll; ++u) pj[u] = P[ks[u]];
-        uint32_t bits = 0u;
-#pragma unroll
-        for (uint32_t u = 0; u < kUnroll; ++u)
-            bits |= ((uint32_t)(u < cnt) & (uint32_t)(ks[u] != s) & (uint32_t)(dist2(pj[u]) <= h2)) << u;
-        if (cap > 0) {
-            const int room = cap - found;
-            while (__popc(bits) > room) bits &= ~(1u << (31 - __clz(bits)));
-            found += __popc(bits);
-            if (found >= cap) active = false;
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < kUnroll; ++u) {  // room for kUnroll pushes was made above
-            if ((bits >> u) & 1u) {
-                q[((qh + qn) & (kQD - 1)) * kLanes] = ks[u];
-                ++qn;
-            }
-        }
-    }
-    pop_above(0);
-}
-
 // Lambda of a sorted slot: the full walk (first iteration) or the mask walk.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_lambda_sorted(float4* Ps, const uint32_t* __restrict__ segOf, SpanTable st,

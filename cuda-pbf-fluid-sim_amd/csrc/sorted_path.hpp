@@ -56,7 +56,7 @@ constexpr uint32_t kSlotPad = 4;
 // candidates of a particle's walk; later candidates are tested directly.
 constexpr uint32_t kCandMaskWords = 64;
 
-// Per-step candidate mask (see k_solve_sorted.hip): words[w * stride + s] holds
+// Per-step candidate mask (built and walked in walk.hpp): words[w * stride + s] holds
 // bits 32w..32w+31 of slot s's walk; rs2 is the squared skin radius.
 struct CandMask {
     uint32_t* words = nullptr;
