@@ -40,7 +40,7 @@ struct StepTape {
 // Scratch of the reverse pass (sorted order unless noted).
 struct AdjointScratch {
     float4* gPf = nullptr;    // dL/d(final position)
-    float4* gVo = nullptr;    // dL/d(velocity after XSPH)
+    float4* gVo = nullptr;    // dL/d(velocity after XSPH); w: the recorded XSPH factor
     float4* gVn = nullptr;    // dL/d(velocity before XSPH)
     float4* gP = nullptr;     // dL/d(positions of the current iteration)
     float4* a = nullptr;      // delta adjoint vectors

@@ -102,14 +102,16 @@ __device__ __forceinline__ float3 hess_apply(const Spiky& s, float3 r, float3 v)
 // through the per-lane FIFO so a wave runs it with most lanes busy.  Lanes past N
 // take part in the walk's wave-level steps with no rows.
 
+// gVo carries the slot's recorded XSPH factor gm = gate m / rho0 in w, so the XSPH
+// adjoint gathers a neighbour's gradient and factor with one load.
 __global__ __launch_bounds__(kBS) void k_adj_gather(const uint32_t* __restrict__ idxS, const float4* __restrict__ gX,
-                                                    const float4* __restrict__ gV, float4* __restrict__ gPf,
-                                                    float4* __restrict__ gVo, uint32_t N) {
+                                                    const float4* __restrict__ gV, const float* __restrict__ gm,
+                                                    float4* __restrict__ gPf, float4* __restrict__ gVo, uint32_t N) {
     const uint32_t s = blockIdx.x * kBS + threadIdx.x;
     if (s >= N) return;
     const uint32_t i = idxS[s];
     gPf[s] = gX[i];
-    gVo[s] = gV[i];
+    gVo[s] = mk4(f3(gV[i]), gm[s]);
 }
 
 // a = s J^T g for the recorded scaled correction e (s = omega / rho0, or 1 / rho0
@@ -140,7 +142,7 @@ __global__ __launch_bounds__(kBS) void k_adj_xsph(const float4* __restrict__ Pf,
                                                   const uint32_t* __restrict__ segOf, SpanTable st,
                                                   sim::DeviceParams dp, Scenes sc,
                                                   const uint32_t* __restrict__ sceneOfSlot, uint32_t N,
-                                                  const float* __restrict__ gm, const float4* __restrict__ gVo,
+                                                  const float4* __restrict__ gVo,
                                                   float4* __restrict__ gVn, float4* __restrict__ gPf,
                                                   double* __restrict__ partial, double* __restrict__ gcSlot) {
     __shared__ double sh[kBS];
@@ -154,8 +156,9 @@ __global__ __launch_bounds__(kBS) void k_adj_xsph(const float4* __restrict__ Pf,
         const float h2 = dp.kernel.h2, poly6 = dp.kernel.poly6;
         // neighbours share the slot's scene: rows never leave a scene's key range
         const float c = xsph_c_of(ss, dp, sc, sceneOfSlot);
-        const float3 pk = f3(Pf[ss]), vk = f3(Vn[ss]), gk = f3(gVo[ss]);
-        const float gmk = gm[ss];
+        const float4 gk4 = gVo[ss];
+        const float3 pk = f3(Pf[ss]), vk = f3(Vn[ss]), gk = f3(gk4);
+        const float gmk = gk4.w;
         const float kk = c * gmk;
         float sumW = 0.f;
         float3 accV = make_float3(0.f, 0.f, 0.f), accP = accV, dv = accV;
@@ -173,8 +176,9 @@ __global__ __launch_bounds__(kBS) void k_adj_xsph(const float4* __restrict__ Pf,
             dv = add3(dv, mul3(dvj, W));
             if (c != 0.f) {
                 const float3 dW = mul3(r, -6.0f * poly6 * hr2 * hr2);
-                const float3 gj = f3(gVo[j]);
-                const float kj = c * gm[j];
+                const float4 gj4 = gVo[j];
+                const float3 gj = f3(gj4);
+                const float kj = c * gj4.w;
                 sumW += W;
                 accV = add3(accV, mul3(gj, kj * W));
                 accP = add3(accP, mul3(dW, dot3(sub3(mul3(gk, kk), mul3(gj, kj)), dvj)));
@@ -378,13 +382,13 @@ void adjoint_step(const StepTape& t, const SpanTable& stIn, const sim::SimParams
     SpanTable st = stIn;
     st.spans = t.spans;
     const float4* Pf = t.P[I];
-    k_adj_gather<<<B, kBS, 0, s>>>(t.idxS, gXin, gVin, w.gPf, w.gVo, N);
+    k_adj_gather<<<B, kBS, 0, s>>>(t.idxS, gXin, gVin, t.xsphGm, w.gPf, w.gVo, N);
     const bool ens = sc.params != nullptr;
     if (ens && (!t.sceneOfSlot || !w.gcSlot)) {
         set_error(PBFX_E_INVALID_ARGUMENT, "adjoint_step: ensemble tape without scene slots");
         return;
     }
-    k_adj_xsph<<<B, kBS, 0, s>>>(Pf, t.Vs, t.segOf, st, dp, sc, t.sceneOfSlot, N, t.xsphGm, w.gVo, w.gVn, w.gPf,
+    k_adj_xsph<<<B, kBS, 0, s>>>(Pf, t.Vs, t.segOf, st, dp, sc, t.sceneOfSlot, N, w.gVo, w.gVn, w.gPf,
                                  w.partial, ens ? w.gcSlot : nullptr);
     if (ens)
         k_adj_scene_sum<<<sc.count, kBS, 0, s>>>(w.gcSlot, sc, gcAccum);
