@@ -80,7 +80,7 @@ def measured_traffic(kernel):
 
 # Launch knobs that change what the neighbour kernels execute per wave (the sort's
 # knobs do not, nor does the block size, which only groups the same waves).
-WALK_KNOBS = ("candidate_mask", "stream_ratio", "chunk_bits")
+WALK_KNOBS = ("candidate_mask", "stream_ratio", "chunk_bits", "walk_block")
 
 
 def measured_valu(kernel, warmup, steps, knobs):
