@@ -80,14 +80,16 @@ __global__ __launch_bounds__(kBS) void k_lambda_ref(float* __restrict__ lambda, 
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         return rx * rx + ry * ry + rz * rz;
     };
-    auto apply = [&](uint32_t j) {
-        const float4 pj = P[j];
-        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-        lambda_pair(a, rx, ry, rz, rx * rx + ry * ry + rz * rz, c);
-    };
     const CellRows<Lookup> rows(lookup, dp.grid, reach_of(c.h, dp.grid.cellSize), keyS[ss]);
-    neighbor_walk<false, kBS>(valid, s, rows, CandMask{}, IndexedFetch{P, idxS}, sBuf + (threadIdx.x & ~63u),
-                         sQ + threadIdx.x, dp.maxNeighbors, c.h2, dist2, apply);
+    with_pair_form(c, [&](auto form) {
+        auto apply = [&](uint32_t j) {
+            const float4 pj = P[j];
+            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+            lambda_pair<decltype(form)::value>(a, rx, ry, rz, rx * rx + ry * ry + rz * rz, c);
+        };
+        neighbor_walk<false, kBS>(valid, s, rows, CandMask{}, IndexedFetch{P, idxS}, sBuf + (threadIdx.x & ~63u),
+                                  sQ + threadIdx.x, dp.maxNeighbors, c.h2, dist2, apply);
+    });
     if (valid) lambda[i] = lambda_finish(a, dp);
 }
 
@@ -111,14 +113,17 @@ __global__ __launch_bounds__(kBS) void k_delta_ref(float4* __restrict__ delta, c
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         return rx * rx + ry * ry + rz * rz;
     };
-    auto apply = [&](uint32_t j) {
-        const float4 pj = P[j];
-        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-        delta_pair(dx, dy, dz, rx, ry, rz, rx * rx + ry * ry + rz * rz, lam_i, lambda[j], c);
-    };
     const CellRows<Lookup> rows(lookup, dp.grid, reach_of(c.h, dp.grid.cellSize), keyS[ss]);
-    neighbor_walk<false, kBS>(valid, s, rows, CandMask{}, IndexedFetch{P, idxS}, sBuf + (threadIdx.x & ~63u),
-                         sQ + threadIdx.x, 0, c.h2, dist2, apply);
+    with_pair_form(c, [&](auto form) {
+        auto apply = [&](uint32_t j) {
+            const float4 pj = P[j];
+            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+            delta_pair<decltype(form)::value>(dx, dy, dz, rx, ry, rz, rx * rx + ry * ry + rz * rz, lam_i, lambda[j],
+                                              c);
+        };
+        neighbor_walk<false, kBS>(valid, s, rows, CandMask{}, IndexedFetch{P, idxS}, sBuf + (threadIdx.x & ~63u),
+                                  sQ + threadIdx.x, 0, c.h2, dist2, apply);
+    });
     if (!valid) return;
     const float3 d = delta_finish(dx, dy, dz, dp);
     delta[i] = make_float4(d.x, d.y, d.z, 0.0f);

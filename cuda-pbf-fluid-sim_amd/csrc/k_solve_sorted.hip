@@ -218,17 +218,19 @@ __global__ __launch_bounds__(BS) void k_lambda_sorted(float4* Ps, const uint32_t
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         return rx * rx + ry * ry + rz * rz;
     };
-    auto apply = [&](uint32_t k) {
-        const float4 pj = Ps[k];
-        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-        lambda_pair(a, rx, ry, rz, rx * rx + ry * ry + rz * rz, c);
-        if (recB && a.n == cap) cut = k;
-    };
-    if (useMask)
-        mask_walk<BS>(valid, s, m, st, mk, Ps, sQ + threadIdx.x, dp.maxNeighbors, c.h2, dist2, apply);
-    else
-        neighbor_walk<false, BS>(valid, s, SpanRows(st, m), mk, SortedFetch{Ps}, sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x,
-                             dp.maxNeighbors, c.h2, dist2, apply);
+    with_pair_form(c, [&](auto form) {
+        auto apply = [&](uint32_t k) {
+            const float4 pj = Ps[k];
+            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+            lambda_pair<decltype(form)::value>(a, rx, ry, rz, rx * rx + ry * ry + rz * rz, c);
+            if (recB && a.n == cap) cut = k;
+        };
+        if (useMask)
+            mask_walk<BS>(valid, s, m, st, mk, Ps, sQ + threadIdx.x, dp.maxNeighbors, c.h2, dist2, apply);
+        else
+            neighbor_walk<false, BS>(valid, s, SpanRows(st, m), mk, SortedFetch{Ps}, sBuf + (threadIdx.x & ~63u),
+                                     sQ + threadIdx.x, dp.maxNeighbors, c.h2, dist2, apply);
+    });
     if (!valid) return;
     reinterpret_cast<float*>(Ps)[4ull * s + 3] = lambda_finish(a, dp);
     if (recA) {
@@ -262,20 +264,22 @@ __global__ __launch_bounds__(BS) void k_delta_sorted(const float4* __restrict__ 
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         return rx * rx + ry * ry + rz * rz;
     };
-    auto apply = [&](uint32_t k) {
-        const float4 pj = Pa[k];
-        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-        delta_pair(dx, dy, dz, rx, ry, rz, rx * rx + ry * ry + rz * rz, pi.w, pj.w, c);
-    };
     float4* buf = sBuf + (threadIdx.x & ~63u);
-    if (maskMode == 2)
-        mask_walk<BS>(valid, s, m, st, mk, Pa, sQ + threadIdx.x, 0, c.h2, dist2, apply);
-    else if (maskMode == 1)
-        neighbor_walk<true, BS>(valid, s, SpanRows(st, m), mk, SortedFetch{Pa}, buf, sQ + threadIdx.x, 0, c.h2, dist2,
-                            apply);
-    else
-        neighbor_walk<false, BS>(valid, s, SpanRows(st, m), mk, SortedFetch{Pa}, buf, sQ + threadIdx.x, 0, c.h2, dist2,
-                             apply);
+    with_pair_form(c, [&](auto form) {
+        auto apply = [&](uint32_t k) {
+            const float4 pj = Pa[k];
+            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+            delta_pair<decltype(form)::value>(dx, dy, dz, rx, ry, rz, rx * rx + ry * ry + rz * rz, pi.w, pj.w, c);
+        };
+        if (maskMode == 2)
+            mask_walk<BS>(valid, s, m, st, mk, Pa, sQ + threadIdx.x, 0, c.h2, dist2, apply);
+        else if (maskMode == 1)
+            neighbor_walk<true, BS>(valid, s, SpanRows(st, m), mk, SortedFetch{Pa}, buf, sQ + threadIdx.x, 0, c.h2,
+                                    dist2, apply);
+        else
+            neighbor_walk<false, BS>(valid, s, SpanRows(st, m), mk, SortedFetch{Pa}, buf, sQ + threadIdx.x, 0, c.h2,
+                                     dist2, apply);
+    });
     if (!valid) return;
     const float3 e = delta_scaled(dx, dy, dz, dp);
     if (recU) recU[s] = make_float4(e.x, e.y, e.z, 0.f);

@@ -11,6 +11,8 @@
 //   dv      += (v_j - v_i) * W                         xsph.cu:66-72
 #pragma once
 
+#include <type_traits>
+
 #include "pbf_common.hpp"
 
 namespace pbfx {
@@ -139,6 +141,24 @@ __device__ __forceinline__ bool pair_unscaled_ok(const PairConsts& c) {
     return true;
 }
 
+// The common setting: unscaled sequences, s_corr on with W(dq) > 0, exponent 4 and a
+// floor below zero.  In it the pair functions' uniform branches are fixed at compile
+// time (their kFixed form): the same expressions, without a dozen scalar branches per
+// pair in the kernels' hot loops.
+__device__ __forceinline__ bool pair_fixed(const PairConsts& c) {
+    return c.unscaled && c.scorr_enable && c.w_q > 0.f && c.scorr_ni == 4 && c.scorr_min < 0.f;
+}
+
+// f(std::true_type{}) in the common setting, f(std::false_type{}) otherwise: a kernel
+// instantiates its walk for both forms and takes one per launch.
+template <class F>
+__device__ __forceinline__ void with_pair_form(const PairConsts& c, F&& f) {
+    if (pair_fixed(c))
+        f(std::true_type{});
+    else
+        f(std::false_type{});
+}
+
 __device__ __forceinline__ PairConsts make_pair_consts(const sim::DeviceParams& dp) {
     PairConsts c;
     c.h = dp.kernel.h;
@@ -165,14 +185,16 @@ struct LambdaAcc {
     int n = 0;
 };
 
-// One in-range neighbour (r2 <= h2 already tested) of the lambda sum.
+// One in-range neighbour (r2 <= h2 already tested) of the lambda sum.  kFixed: the
+// caller has established pair_fixed(c).
+template <bool kFixed = false>
 __device__ __forceinline__ void lambda_pair(LambdaAcc& a, float rx, float ry, float rz, float r2,
                                             const PairConsts& c) {
     a.n++;
     const float hr2 = c.h2 - r2;
     a.density += c.m_poly6 * hr2 * hr2 * hr2;
     float coeff;
-    if (c.unscaled) {
+    if (kFixed || c.unscaled) {
         const float r = sqrt_unscaled(r2);  // +0 or positive: r2 is a sum of squares
         const float r_safe = max_nonneg(r, c.grad_r_eps);
         const float t = c.h - r;
@@ -232,11 +254,13 @@ __device__ __forceinline__ void lambda_adjoint_coefs(const LambdaAcc& a, const s
     e0 = floored ? 0.f : C * (scale * scale) / (den * den);
 }
 
-// One in-range neighbour of the position-correction sum.
+// One in-range neighbour of the position-correction sum.  kFixed: the caller has
+// established pair_fixed(c).
+template <bool kFixed = false>
 __device__ __forceinline__ void delta_pair(float& dx, float& dy, float& dz, float rx, float ry, float rz, float r2,
                                            float lam_i, float lam_j, const PairConsts& c) {
     float coeff;
-    if (c.unscaled) {
+    if (kFixed || c.unscaled) {
         const float r = sqrt_unscaled(r2);  // +0 or positive: r2 is a sum of squares
         const float t = c.h - r;
         const float r_safe = max_nonneg(r, c.grad_r_eps);
@@ -249,7 +273,14 @@ __device__ __forceinline__ void delta_pair(float& dx, float& dy, float& dz, floa
     }
     const float ax = coeff * rx, ay = coeff * ry, az = coeff * rz;
     float scorr = 0.0f;
-    if (c.scorr_enable) {
+    if (kFixed) {
+        const float hr2 = c.h2 - r2;
+        const float w = c.poly6 * hr2 * hr2 * hr2;
+        const float ratio = div_unscaled(w, c.w_q, c.rcp_w_q);
+        const double x = (double)ratio;  // pow_scorr's n = 4 product
+        const double x2 = x * x;
+        scorr = fmaxf(c.neg_scorr_k * (float)(x2 * x2), c.scorr_min);
+    } else if (c.scorr_enable) {
         const float hr2 = c.h2 - r2;
         const float w = c.poly6 * hr2 * hr2 * hr2;
         const float ratio = (c.w_q > 0.f) ? (c.unscaled ? div_unscaled(w, c.w_q, c.rcp_w_q) : (w / c.w_q)) : 0.0f;

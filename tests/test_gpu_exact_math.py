@@ -71,3 +71,30 @@ def test_general_form_matches_oracle(oracle, dev):
         np.testing.assert_array_equal(V, eV)
         states.append((X, V))
     np.testing.assert_array_equal(states[0][0], states[1][0])
+
+
+# s_corr settings outside the common one (exponent 4 with a floor): the kernels take the
+# general pair form, whose branches are decided per pair
+SCORR_VARIANTS = {"scorr_off": {"scorr_enable": 0}, "n3": {"scorr_n": 3.0}, "n1": {"scorr_n": 1.0},
+                  "no_floor": {"scorr_min": 0.0}}
+
+
+@pytest.mark.parametrize("path", ["reordered", "reference"])
+@pytest.mark.parametrize("variant", sorted(SCORR_VARIANTS))
+def test_general_pair_form_matches_oracle(oracle, dev, variant, path):
+    from pbfx.sim import Simulator
+    scene = P.small_block((12, 12, 12), table_log2=16)
+    sp = P.build_sim_params(scene)
+    for k, v in SCORR_VARIANTS[variant].items():
+        setattr(sp.pbf, k, v)
+    sim = Simulator(sp, grid="dense", table_log2=scene.table_log2, path=path)
+    try:
+        sim.seed_scene(scene)
+        sim.step(4)
+        X, V = sim.state()
+    finally:
+        sim.close()
+    X0 = P.seed_scene_reference(scene, oracle.seed_lattice)
+    eX, eV = oracle.run(X0, np.zeros_like(X0), sp, 4)
+    np.testing.assert_array_equal(X, eX)
+    np.testing.assert_array_equal(V, eV)
