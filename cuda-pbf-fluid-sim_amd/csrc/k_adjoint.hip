@@ -65,14 +65,17 @@ __device__ __forceinline__ float3 mask3(float3 g, uint32_t bits) {
 struct Spiky {
     float f, dfr;  // f(r) and (df/dr)/|r|
 };
-// One division per pair: 1 / r, correctly rounded, then products.
+// One division per pair: 1 / r, correctly rounded, then products.  kFixed: the
+// caller has established pair_fixed(c) (so the unscaled sequences apply).
+template <bool kFixed>
 __device__ __forceinline__ Spiky spiky_of(float r2, const PairConsts& c) {
-    const float r = c.unscaled ? sqrt_unscaled(r2) : sqrtf(r2);
+    const bool unscaled = kFixed || c.unscaled;
+    const float r = unscaled ? sqrt_unscaled(r2) : sqrtf(r2);
     const float t = c.h - r;
     Spiky s;
     if (r > c.grad_r_eps) {
         // r in (eps, h]: inside div_unscaled's range whenever the pair arithmetic is
-        const float ir = c.unscaled ? div_unscaled(1.0f, r, rcp_refined(r)) : 1.0f / r;
+        const float ir = unscaled ? div_unscaled(1.0f, r, rcp_refined(r)) : 1.0f / r;
         const float t2 = t * t;
         s.f = c.neg3_spiky * t2 * ir;
         // f = -3 spiky t^2 / r  ->  df/dr = 3 spiky (2 t / r + t^2 / r^2)
@@ -261,36 +264,55 @@ __global__ __launch_bounds__(kBS) void k_adj_delta(const float4* __restrict__ P,
         const float3 r = sub3(pk, f3(pj));
         return dot3(r, r);
     };
-    auto apply = [&](uint32_t j) {
-        const float4 pj4 = P[j];
-        const float3 r = sub3(pk, f3(pj4));
-        const float r2 = dot3(r, r);
-        const Spiky sp = spiky_of(r2, c);
-        const float3 g = mul3(r, sp.f);
-        const float3 da = sub3(ak, f3(A[j]));
-        gl += dot3(g, da);
-        // c_kj and its gradient (s_corr, reference: pbf_delta_apply.cu:77-84)
-        float scorr = 0.f, dsr = 0.f;  // dscorr = dsr * r
-        if (c.scorr_enable) {
-            const float hr2 = c.h2 - r2;
-            const float W = c.poly6 * hr2 * hr2 * hr2;
-            // the forward's ratio and s_corr, bit for bit
-            const float ratio = (c.w_q > 0.f) ? (c.unscaled ? div_unscaled(W, c.w_q, c.rcp_w_q) : (W / c.w_q)) : 0.0f;
-            const float raw = c.neg_scorr_k * pow_scorr(ratio, c.scorr_n, c.scorr_ni);
-            scorr = raw;
-            if (c.scorr_min < 0.f && raw < c.scorr_min) {
-                scorr = c.scorr_min;
-            } else if (c.w_q > 0.f && ratio > 0.f) {
-                // d/dr of -k (W/Wq)^n: -k n (W/Wq)^(n-1) / Wq * dW6, dW6 = -6 poly6 hr2^2 r
-                const float pn1 = pow_scorr_deriv(ratio, c);
-                dsr = c.neg_scorr_k * c.scorr_n * pn1 * inv_w_q * (-6.0f * c.poly6 * hr2 * hr2);
+    with_pair_form(c, [&](auto form) {
+        constexpr bool kFixed = decltype(form)::value;
+        auto apply = [&](uint32_t j) {
+            const float4 pj4 = P[j];
+            const float3 r = sub3(pk, f3(pj4));
+            const float r2 = dot3(r, r);
+            const Spiky sp = spiky_of<kFixed>(r2, c);
+            const float3 g = mul3(r, sp.f);
+            const float3 da = sub3(ak, f3(A[j]));
+            gl += dot3(g, da);
+            // c_kj and its gradient (s_corr, reference: pbf_delta_apply.cu:77-84)
+            float scorr = 0.f, dsr = 0.f;  // dscorr = dsr * r
+            if (kFixed) {
+                // the general branch below in the common setting (exponent 4, a floor)
+                const float hr2 = c.h2 - r2;
+                const float W = c.poly6 * hr2 * hr2 * hr2;
+                const float ratio = div_unscaled(W, c.w_q, c.rcp_w_q);
+                const double x = (double)ratio;
+                const double x2 = x * x;
+                const float raw = c.neg_scorr_k * (float)(x2 * x2);
+                scorr = raw;
+                if (raw < c.scorr_min) {
+                    scorr = c.scorr_min;
+                } else if (ratio > 0.f) {
+                    const float pn1 = (float)(x * (x * x));
+                    dsr = c.neg_scorr_k * c.scorr_n * pn1 * inv_w_q * (-6.0f * c.poly6 * hr2 * hr2);
+                }
+            } else if (c.scorr_enable) {
+                const float hr2 = c.h2 - r2;
+                const float W = c.poly6 * hr2 * hr2 * hr2;
+                // the forward's ratio and s_corr, bit for bit
+                const float ratio =
+                    (c.w_q > 0.f) ? (c.unscaled ? div_unscaled(W, c.w_q, c.rcp_w_q) : (W / c.w_q)) : 0.0f;
+                const float raw = c.neg_scorr_k * pow_scorr(ratio, c.scorr_n, c.scorr_ni);
+                scorr = raw;
+                if (c.scorr_min < 0.f && raw < c.scorr_min) {
+                    scorr = c.scorr_min;
+                } else if (c.w_q > 0.f && ratio > 0.f) {
+                    // d/dr of -k (W/Wq)^n: -k n (W/Wq)^(n-1) / Wq * dW6, dW6 = -6 poly6 hr2^2 r
+                    const float pn1 = pow_scorr_deriv(ratio, c);
+                    dsr = c.neg_scorr_k * c.scorr_n * pn1 * inv_w_q * (-6.0f * c.poly6 * hr2 * hr2);
+                }
             }
-        }
-        const float ckj = pk4.w + pj4.w + scorr;
-        acc = add3(acc, add3(mul3(hess_apply(sp, r, da), ckj), mul3(g, dsr * dot3(r, da))));
-    };
-    neighbor_walk<false, kBS>(valid, s, SpanRows(st, segOf[ss]), CandMask{}, SortedFetch{P},
-                              sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x, 0, c.h2, dist2, apply);
+            const float ckj = pk4.w + pj4.w + scorr;
+            acc = add3(acc, add3(mul3(hess_apply(sp, r, da), ckj), mul3(g, dsr * dot3(r, da))));
+        };
+        neighbor_walk<false, kBS>(valid, s, SpanRows(st, segOf[ss]), CandMask{}, SortedFetch{P},
+                                  sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x, 0, c.h2, dist2, apply);
+    });
     if (!valid) return;
     gP[s] = mk4(add3(f3(gP[s]), acc), 0.f);
     const float4 la = lamA[s];
@@ -325,24 +347,26 @@ __global__ __launch_bounds__(kBS) void k_adj_lambda(const float4* __restrict__ P
         const float3 r = sub3(pk, f3(pj));
         return dot3(r, r);
     };
-    auto apply = [&](uint32_t j) {
-        const float3 r = sub3(pk, f3(P[j]));
-        const float r2 = dot3(r, r);
-        const float4 qj = lamQ[j];
-        const uint2 ej = lamE[j];
-        const float epj = __uint_as_float(ej.x);
-        const bool inK = own && j <= cutK;                       // j in Ncap(k)
-        const bool inJ = (qj.w != 0.f || epj != 0.f) && s <= ej.y;  // k in Ncap(j)
-        if (!inK && !inJ) return;
-        const float hr2 = c.h2 - r2;
-        const float3 dW6 = mul3(r, -6.0f * poly6 * hr2 * hr2);
-        const Spiky sp = spiky_of(r2, c);
-        const float3 g = mul3(r, sp.f);
-        if (inK) acc = add3(acc, add3(mul3(dW6, bk), mul3(hess_apply(sp, r, add3(g, Gk)), 2.0f * epk)));
-        if (inJ) acc = add3(acc, add3(mul3(dW6, qj.w), mul3(hess_apply(sp, r, sub3(g, f3(qj))), 2.0f * epj)));
-    };
-    neighbor_walk<false, kBS>(valid, s, SpanRows(st, segOf[ss]), CandMask{}, SortedFetch{P},
-                              sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x, 0, c.h2, dist2, apply);
+    with_pair_form(c, [&](auto form) {
+        auto apply = [&](uint32_t j) {
+            const float3 r = sub3(pk, f3(P[j]));
+            const float r2 = dot3(r, r);
+            const float4 qj = lamQ[j];
+            const uint2 ej = lamE[j];
+            const float epj = __uint_as_float(ej.x);
+            const bool inK = own && j <= cutK;                          // j in Ncap(k)
+            const bool inJ = (qj.w != 0.f || epj != 0.f) && s <= ej.y;  // k in Ncap(j)
+            if (!inK && !inJ) return;
+            const float hr2 = c.h2 - r2;
+            const float3 dW6 = mul3(r, -6.0f * poly6 * hr2 * hr2);
+            const Spiky sp = spiky_of<decltype(form)::value>(r2, c);
+            const float3 g = mul3(r, sp.f);
+            if (inK) acc = add3(acc, add3(mul3(dW6, bk), mul3(hess_apply(sp, r, add3(g, Gk)), 2.0f * epk)));
+            if (inJ) acc = add3(acc, add3(mul3(dW6, qj.w), mul3(hess_apply(sp, r, sub3(g, f3(qj))), 2.0f * epj)));
+        };
+        neighbor_walk<false, kBS>(valid, s, SpanRows(st, segOf[ss]), CandMask{}, SortedFetch{P},
+                                  sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x, 0, c.h2, dist2, apply);
+    });
     if (!valid) return;
     const float3 g = add3(f3(gP[s]), acc);
     gP[s] = mk4(g, 0.f);
