@@ -99,6 +99,10 @@ __device__ __forceinline__ float3 hess_apply(const Spiky& s, float3 r, float3 v)
     return add3(mul3(v, s.f), mul3(r, s.dfr * dot3(r, v)));
 }
 
+// Occupancy: the three neighbour kernels below are held to 72 VGPRs (seven waves per
+// SIMD, amdgpu_waves_per_eu(7)); left alone they take 80-89 (five or six waves), and
+// the extra waves outweigh the few spills (8% faster reverse pass on 4.2M particles).
+//
 // The neighbour kernels below gather over every in-range neighbour j != k of their
 // slot with the forward kernels' wave walk (walk.hpp): the same candidates, the same
 // r^2 <= h^2 test and the same ascending-slot order, with the pair work deferred
@@ -141,7 +145,7 @@ __device__ __forceinline__ float xsph_c_of(uint32_t s, const sim::DeviceParams& 
     return sc.params ? sc.params[sceneOfSlot[s]].xsph_c : dp.xsph_c;
 }
 
-__global__ __launch_bounds__(kBS) void k_adj_xsph(const float4* __restrict__ Pf, const float4* __restrict__ Vn,
+__global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k_adj_xsph(const float4* __restrict__ Pf, const float4* __restrict__ Vn,
                                                   const uint32_t* __restrict__ segOf, SpanTable st,
                                                   sim::DeviceParams dp, Scenes sc,
                                                   const uint32_t* __restrict__ sceneOfSlot, uint32_t N,
@@ -244,7 +248,7 @@ __global__ __launch_bounds__(kBS) void k_adj_velocity(const uint32_t* __restrict
     gX[idxS[s]] = mk4(mul3(gv, -invDt), 0.f);
 }
 
-__global__ __launch_bounds__(kBS) void k_adj_delta(const float4* __restrict__ P, const uint32_t* __restrict__ segOf,
+__global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k_adj_delta(const float4* __restrict__ P, const uint32_t* __restrict__ segOf,
                                                    SpanTable st, sim::DeviceParams dp, uint32_t N,
                                                    const float4* __restrict__ A, const float4* __restrict__ lamA,
                                                    const uint2* __restrict__ lamB, float4* __restrict__ gP,
@@ -323,7 +327,7 @@ __global__ __launch_bounds__(kBS) void k_adj_delta(const float4* __restrict__ P,
 
 // Lambda terms of dL/dP; then a of the iteration before (U = its recorded scaled
 // correction), or nothing after the first iteration (U == nullptr).
-__global__ __launch_bounds__(kBS) void k_adj_lambda(const float4* __restrict__ P, const uint32_t* __restrict__ segOf,
+__global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k_adj_lambda(const float4* __restrict__ P, const uint32_t* __restrict__ segOf,
                                                     SpanTable st, sim::DeviceParams dp, uint32_t N,
                                                     const float4* __restrict__ lamQ, const uint2* __restrict__ lamE,
                                                     const float4* __restrict__ U, float4* __restrict__ gP,
