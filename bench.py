@@ -205,10 +205,27 @@ def bench_batched(count, warmup, steps, tune_steps=0, grid="dense"):
             "speedup_vs_concurrent_streams": round(t_streams / t_ens, 3), "tuning": best}
 
 
-def bench_adjoint_batched(steps, checkpoint, rollouts=64, knobs=None):
+# The walk knobs a rollout tunes for itself: its scenes are not the dam break the main
+# line tunes on, and the candidate mask pays off only at some densities.
+ROLLOUT_SPACE = {"candidate_mask": (True, False), "walk_block": (256, 128), "stream_ratio": (1, 2)}
+
+
+def tune_rollout(sim, tune_steps):
+    """Tune the walk knobs on the simulator's current state (restored afterwards); returns
+    the chosen knobs, or None when tune_steps is 0."""
+    if not tune_steps:
+        return None
+    from pbfx import tune
+    best, _, _ = tune.autotune(sim, steps=tune_steps, space=ROLLOUT_SPACE, key=None)
+    return {k: best[k] for k in ROLLOUT_SPACE}
+
+
+def bench_adjoint_batched(steps, checkpoint, rollouts=64, knobs=None, tune_steps=0):
     """64 differentiable rollouts (each the 64x32x32 configuration, own jitter seed and
     target velocity field) advanced and differentiated together as one ensemble.
-    knobs: launch knobs to apply (the neighbour walk's), or None for the defaults."""
+    knobs: launch knobs to apply first (the neighbour walk's), or None for the defaults;
+    tune_steps: steps per setting when the walk knobs are then tuned on the ensemble's
+    state after the target rollout (0: keep them)."""
     import torch
     from pbfx.diff import EnsembleRollout, differentiable_scene, occupancy_grid, seeded_velocity
     scenes = [differentiable_scene(seed=7 + k) for k in range(rollouts)]
@@ -223,6 +240,7 @@ def bench_adjoint_batched(steps, checkpoint, rollouts=64, knobs=None):
         vt = torch.from_numpy(np.concatenate([seeded_velocity(sc.num_particles, 1000 + k, 0.5)
                                               for k, sc in enumerate(scenes)])).cuda()
         XT, _ = roll.forward(x0, vt)
+        chosen = tune_rollout(roll.sim, tune_steps) or knobs
         spans = [(f, f + sc.num_particles) for f, sc in zip(roll.firsts, scenes)]
         targets = [occupancy_grid(XT[a:b, :3].double(), sc.box_min, sc.box_max, cells=32, sigma=h)
                    for (a, b), sc in zip(spans, scenes)]
@@ -246,17 +264,20 @@ def bench_adjoint_batched(steps, checkpoint, rollouts=64, knobs=None):
         roll.close()
     print(f"[bench] {rollouts} batched rollouts, {n} particles: forward {t_fwd:.4f} s, backward {t_bwd:.4f} s",
           file=sys.stderr, flush=True)
-    return {"rollouts": rollouts, "particles": n, "forward_ms_per_step": round(t_fwd * 1e3 / steps, 4),
+    return {"rollouts": rollouts, "particles": n, "knobs": chosen or "defaults",
+            "forward_ms_per_step": round(t_fwd * 1e3 / steps, 4),
             "backward_over_forward": round(t_bwd / t_fwd, 3),
             "rollout_steps_per_s": round(rollouts * steps / (t_fwd + t_bwd), 1)}
 
 
-def bench_adjoint(steps=60, checkpoint=10, rollouts=64, knobs=None):
+def bench_adjoint(steps=60, checkpoint=10, rollouts=64, knobs=None, tune_steps=0):
     """The differentiable rollout: 65,536 particles (64x32x32 lattice), 60 steps, checkpoint every
     10.  Loss: squared L2 distance between the 32^3 Gaussian occupancy grid (sigma = h) of X_T and
     that of a target rollout from a seeded perturbed initial velocity.  Times the checkpointed
     forward, then the reverse pass split into re-recording the segments and the reverse kernels.
-    knobs: the neighbour walk's launch knobs to apply to forward and reverse, or None."""
+    knobs: the neighbour walk's launch knobs to apply to forward and reverse, or None;
+    tune_steps: steps per setting when the walk knobs are then tuned on the rollout's own
+    state after the target rollout (0: keep them)."""
     import torch
     from pbfx.diff import Rollout, differentiable_scene, initial_state, occupancy_grid, seeded_velocity
     scene = differentiable_scene()
@@ -271,6 +292,7 @@ def bench_adjoint(steps=60, checkpoint=10, rollouts=64, knobs=None):
             roll.sim.set_tuning(**knobs)
         vt = torch.from_numpy(seeded_velocity(n, 77, 0.5)).cuda()
         XT, _ = roll.forward(x0, vt, sp.xsph_c)
+        chosen = tune_rollout(roll.sim, tune_steps) or knobs
         target = occupancy_grid(XT[:, :3].double(), scene.box_min, scene.box_max, cells=32, sigma=h)
         v0 = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
         roll.forward(x0, v0, sp.xsph_c)  # warm-up of the same rollout
@@ -294,9 +316,9 @@ def bench_adjoint(steps=60, checkpoint=10, rollouts=64, knobs=None):
         roll.close()
     print(f"[bench] adjoint rollout {n} particles x {steps} steps: forward {t_fwd:.4f} s, backward {t_bwd:.4f} s",
           file=sys.stderr, flush=True)
-    batched = bench_adjoint_batched(steps, checkpoint, rollouts, knobs) if rollouts > 1 else None
+    batched = bench_adjoint_batched(steps, checkpoint, rollouts, knobs, tune_steps) if rollouts > 1 else None
     return {"particles": n, "steps": steps, "checkpoint": checkpoint, "loss": float(loss.detach()), "batched": batched,
-            "knobs": knobs or "defaults",
+            "knobs": chosen or "defaults",
             "forward_ms_per_step": round(t_fwd * 1e3 / steps, 4),
             "record_ms_per_step": round(tm["record_s"] * 1e3 / steps, 4),
             "reverse_ms_per_step": round(tm["reverse_s"] * 1e3 / steps, 4),
@@ -466,12 +488,13 @@ def main():
                                        0 if a.no_tune else a.tune_steps)
 
     if not a.no_adjoint:
-        # the dam break's tuned walk knobs (the rollout's states are dam breaks too)
+        # the dam break's tuned walk knobs as the start; each rollout then tunes the walk
+        # knobs on its own state
         walk = None
         if tuning and "chosen" in tuning:
             walk = {k: tuning["chosen"][k] for k in ("candidate_mask", "stream_ratio", "chunk_bits", "walk_block")
                     if k in tuning["chosen"]}
-        out["adjoint"] = bench_adjoint(knobs=walk)
+        out["adjoint"] = bench_adjoint(knobs=walk, tune_steps=0 if (a.no_tune or a.tuning) else a.tune_steps)
 
     if not a.no_cpu_baseline:
         from oracle import oracle as O
