@@ -59,14 +59,18 @@ __global__ __launch_bounds__(kBS) void k_predict_hash(const float4* __restrict__
     keys[i] = keyBase + cell_key(px, py, pz, grid);
 }
 
-// Home slot of a cell key.  While the key space fits the table the home is the key
-// itself, so consecutive keys (the cells of an x-window) share cache lines.  Keys past
-// the table size are shifted by a multiplicative hash of their high bits, so key blocks
-// that wrap around the table do not all land on the same slots.
-__device__ __forceinline__ uint32_t table_home(uint32_t key, uint32_t mask) {
-    const int L = __popc(mask);  // mask = 2^L - 1
-    const uint32_t hi = L >= 32 ? 0u : (key >> L);
-    return (key + hi * 0x9E3779B1u) & mask;
+// Home slot of a cell key.  While the key space fits the table (scatter == 0) the home
+// is the key itself, so consecutive keys (the cells of an x-window) share cache lines.
+// Otherwise blocks of four consecutive keys (one 64-byte line of slots) are placed by a
+// multiplicative (Fibonacci) hash of the block index.  Shifting whole key ranges by a
+// constant instead lets the long runs of occupied cells of different ranges overlap, and
+// linear probing then chains them: on the 8M splash's 4.1M cells in 2^24 slots that took
+// 10.8 slots per lookup on average (145 at most), against 1.1 (10) with the blocks.
+__device__ __forceinline__ uint32_t table_home(uint32_t key, uint32_t mask, uint32_t scatter) {
+    if (!scatter) return key & mask;
+    const uint32_t L = (uint32_t)__popc(mask);  // mask = 2^L - 1, 4 <= L <= 30
+    const uint32_t blk = ((key >> 2) * 0x9E3779B1u) >> (34u - L);  // the product's top L - 2 bits
+    return ((blk << 2) | (key & 3u)) & mask;
 }
 
 __global__ __launch_bounds__(kBS) void k_hash_insert(HashTable t, const uint32_t* __restrict__ uniq,
@@ -78,7 +82,7 @@ __global__ __launch_bounds__(kBS) void k_hash_insert(HashTable t, const uint32_t
     const uint32_t key = uniq[m];
     const uint32_t ep = *t.epoch;
     const unsigned long long mine = ((unsigned long long)ep << 32) | key;
-    uint32_t slot = table_home(key, t.mask);
+    uint32_t slot = table_home(key, t.mask, t.scatter);
     for (uint32_t probe = 0; probe <= t.mask; ++probe, slot = (slot + 1) & t.mask) {
         unsigned long long* w = reinterpret_cast<unsigned long long*>(&t.slots[slot]);
         unsigned long long cur = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -99,9 +103,10 @@ __global__ __launch_bounds__(kBS) void k_hash_insert(HashTable t, const uint32_t
 struct HashRows {
     const HashSlot* __restrict__ slots;
     uint32_t mask;
+    uint32_t scatter;
     uint32_t ep;
     __device__ __forceinline__ bool cell(uint32_t key, uint32_t& b, uint32_t& e) const {
-        uint32_t slot = table_home(key, mask);
+        uint32_t slot = table_home(key, mask, scatter);
         for (uint32_t probe = 0; probe <= mask; ++probe, slot = (slot + 1) & mask) {
             const uint4 v = *reinterpret_cast<const uint4*>(&slots[slot]);
             if (v.y != ep) return false;
@@ -191,9 +196,10 @@ struct CompactRowsCount {
 struct HashRowsDev {
     const HashSlot* __restrict__ slots;
     uint32_t mask;
+    uint32_t scatter;
     const uint32_t* __restrict__ epoch;
     __device__ __forceinline__ void span(uint32_t k0, uint32_t k1, uint32_t& lo, uint32_t& hi) const {
-        HashRows r{slots, mask, *epoch};
+        HashRows r{slots, mask, scatter, *epoch};
         r.span(k0, k1, lo, hi);
     }
 };
@@ -427,7 +433,7 @@ void launch_spans_gather(CellLookup lookup, const float4* Xn, const uint32_t* id
             break;
         case CellLookup::Hashed:
             k_spans_gather<<<b, kBS, 0, s>>>(Xn, idxS, Ps, N, uniq, count,
-                                             HashRowsDev{table.slots, table.mask, table.epoch}, st, grid, sc);
+                                             HashRowsDev{table.slots, table.mask, table.scatter, table.epoch}, st, grid, sc);
             break;
     }
     check_launch("spans_gather");

@@ -323,6 +323,9 @@ bool GridBuffers::allocate(uint32_t cap, uint32_t nCells, GridKind k, uint32_t t
             tableLog2 = tLog2;
             const size_t slots = size_t(1) << tLog2;
             table.mask = (uint32_t)(slots - 1);
+            // a key space larger than the table gets hashed homes (placement only: every
+            // lookup compares full keys, so results never depend on it)
+            table.scatter = (uint64_t)nCells > (uint64_t)slots ? 1u : 0u;
             ok = ok && dalloc(table.slots, slots) && dalloc(table.epoch, 1) && dalloc(table.overflow, 1);
             if (ok) {
                 (void)hipMemset(table.slots, 0, sizeof(pbfx::HashSlot) * slots);

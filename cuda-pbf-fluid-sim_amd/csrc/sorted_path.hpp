@@ -26,6 +26,7 @@ struct alignas(16) HashSlot {
 struct HashTable {
     HashSlot* slots = nullptr;
     uint32_t mask = 0;          // table size - 1 (power of two)
+    uint32_t scatter = 0;       // 1 when the key space exceeds the table: homes are hashed
     uint32_t* epoch = nullptr;  // device counter, advanced once per step
     uint32_t* overflow = nullptr;  // device flag, set when an insert finds no slot
 };
