@@ -64,23 +64,30 @@ TRAFFIC_FILE = os.path.join(REPO, "profiles", "hbm_traffic_dam_break_1m.json")
 INSTR_FILE = os.path.join(REPO, "profiles", "instructions_dam_break_1m.json")
 
 
-def measured_traffic(kernel):
+# Launch knobs of the neighbour walk, which change what its kernels execute and read;
+# profiled counts hold only for the setting they were measured at.  (The sort's knobs
+# do not touch these kernels.)
+WALK_KNOBS = ("candidate_mask", "stream_ratio", "chunk_bits", "walk_block")
+
+
+def measured_traffic(kernel, knobs):
     """HBM bytes per launch of `kernel` from the profiler counter summary kept next to
-    the other profiles (FETCH_SIZE + WRITE_SIZE, each scaled by a known-size copy), or None."""
+    the other profiles (FETCH_SIZE + WRITE_SIZE, each scaled by a known-size copy) for
+    the same walk knobs, or (None, reason)."""
     try:
         with open(TRAFFIC_FILE) as f:
             rec = json.load(f)
     except (OSError, ValueError):
         return None, "no counter summary"
-    k = rec.get("kernels", {}).get(kernel)
-    if not k:
-        return None, "kernel not in the counter summary"
-    return k["bytes_per_launch"], f"{os.path.relpath(TRAFFIC_FILE, REPO)} ({rec.get('command', '')})"
-
-
-# Launch knobs that change what the neighbour kernels execute per wave (the sort's
-# knobs do not, nor does the block size, which only groups the same waves).
-WALK_KNOBS = ("candidate_mask", "stream_ratio", "chunk_bits", "walk_block")
+    for run in rec.get("runs", []):
+        prof = run.get("tuning") or {}
+        if all(prof.get(n) == knobs.get(n) for n in WALK_KNOBS):
+            k = run.get("kernels", {}).get(kernel)
+            if not k:
+                return None, "kernel not in the counter summary"
+            cmd = rec.get("command", "").replace("<TUNING>", json.dumps(prof))
+            return k["bytes_per_launch"], f"{os.path.relpath(TRAFFIC_FILE, REPO)} ({cmd})"
+    return None, "no traffic profiled for this run's launch knobs"
 
 
 def measured_valu(kernel, warmup, steps, knobs):
@@ -439,7 +446,7 @@ def main():
     dominant = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
     dk = kernels[dominant]
     achieved = dk["bytes_per_launch"] / (dk["avg_launch_ms"] * 1e-3)
-    traffic, traffic_note = measured_traffic(dominant)
+    traffic, traffic_note = measured_traffic(dominant, knobs)
     valu, valu_note = measured_valu(dominant, a.warmup, a.steps, knobs)
     valu_roofline = {"bound": "valu", "kernel": dominant, "achieved": None, "frac": None, "note": valu_note}
     if valu is not None:
