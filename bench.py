@@ -99,17 +99,25 @@ def measured_valu(kernel, warmup, steps, knobs):
             rec = json.load(f)
     except (OSError, ValueError):
         return None, "no counter summary"
-    if (rec.get("warmup"), rec.get("steps")) != (warmup, steps):
-        return None, (f"instruction counts were profiled for steps {rec.get('warmup')}.."
-                      f"{(rec.get('warmup') or 0) + (rec.get('steps') or 0)}; this run times other steps")
+    same_window = (rec.get("warmup"), rec.get("steps")) == (warmup, steps)
     for run in rec.get("runs", []):
         prof = run.get("tuning") or {}
-        if all(prof.get(n) == knobs.get(n) for n in WALK_KNOBS):
-            k = run.get("kernels", {}).get(kernel)
-            if not k:
-                return None, "kernel not in the counter summary"
-            cmd = rec.get("command", "").replace("<TUNING>", json.dumps(prof))
-            return k["valu_per_launch"], f"{os.path.relpath(INSTR_FILE, REPO)} ({cmd})"
+        if not all(prof.get(n) == knobs.get(n) for n in WALK_KNOBS):
+            continue
+        cmd = rec.get("command", "").replace("<TUNING>", json.dumps(prof))
+        src = f"{os.path.relpath(INSTR_FILE, REPO)} ({cmd})"
+        k = run.get("kernels", {}).get(kernel)
+        if same_window and k:
+            return k["valu_per_launch"], src
+        # the simulation is deterministic, so another window of the same run has the
+        # per-step counts the profiled run recorded for it
+        ps = (run.get("per_step_valu") or {}).get(kernel)
+        if ps and ps["first_step"] <= warmup and warmup + steps <= ps["first_step"] + len(ps["valu"]):
+            win = ps["valu"][warmup - ps["first_step"]:warmup - ps["first_step"] + steps]
+            return sum(win) / (steps * ps["launches_per_step"]), f"{src}, per-step counts of steps " \
+                f"{warmup}..{warmup + steps}"
+        return None, (f"instruction counts were profiled for steps {rec.get('warmup')}.."
+                      f"{(rec.get('warmup') or 0) + (rec.get('steps') or 0)}; this run times other steps")
     return None, "no instruction counts profiled for this run's launch knobs"
 
 
