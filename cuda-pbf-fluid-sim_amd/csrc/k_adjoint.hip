@@ -99,6 +99,10 @@ __device__ __forceinline__ float3 hess_apply(const Spiky& s, float3 r, float3 v)
     return add3(mul3(v, s.f), mul3(r, s.dfr * dot3(r, v)));
 }
 
+// Launch shape: the three neighbour kernels below deal their 256-slot tiles to the XCDs
+// in contiguous runs (xcd_tile), as the forward kernels do, so neighbouring tiles share
+// an L2; dL/dc partials are stored per tile, so their sum keeps slot order.
+//
 // Occupancy: the three neighbour kernels below are held to 72 VGPRs (seven waves per
 // SIMD, amdgpu_waves_per_eu(7)); left alone they take 80-89 (five or six waves), and
 // the extra waves outweigh the few spills (8% faster reverse pass on 4.2M particles).
@@ -155,7 +159,8 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k
     __shared__ double sh[kBS];
     __shared__ uint32_t sQ[kQD * kBS];
     __shared__ float4 sBuf[kBS];
-    const uint32_t s = blockIdx.x * kBS + threadIdx.x;
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t s = tile * kBS + threadIdx.x;
     const bool valid = s < N;
     const uint32_t ss = valid ? s : N - 1;
     double gcl = 0.0;
@@ -207,7 +212,7 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k
         if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+    if (threadIdx.x == 0) partial[tile] = sh[0];
 }
 
 __global__ void k_adj_sum(const double* __restrict__ partial, uint32_t n, double* __restrict__ accum) {
@@ -255,7 +260,8 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k
                                                    float4* __restrict__ lamQ, uint2* __restrict__ lamE) {
     __shared__ uint32_t sQ[kQD * kBS];
     __shared__ float4 sBuf[kBS];
-    const uint32_t s = blockIdx.x * kBS + threadIdx.x;
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t s = tile * kBS + threadIdx.x;
     const bool valid = s < N;
     const uint32_t ss = valid ? s : N - 1;
     const PairConsts c = make_pair_consts(dp);
@@ -334,7 +340,8 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k
                                                     float4* __restrict__ A) {
     __shared__ uint32_t sQ[kQD * kBS];
     __shared__ float4 sBuf[kBS];
-    const uint32_t s = blockIdx.x * kBS + threadIdx.x;
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t s = tile * kBS + threadIdx.x;
     const bool valid = s < N;
     const uint32_t ss = valid ? s : N - 1;
     const PairConsts c = make_pair_consts(dp);
