@@ -12,6 +12,9 @@
 // read through the sorted index array.
 // The headless pipeline uses the cell-reordered kernels in k_solve_sorted.hip;
 // these exist so the sim/ phase API can be served unchanged.
+#include <mutex>
+#include <unordered_map>
+
 #include "pbf_common.hpp"
 #include "pbf_hip.h"
 #include "pbf_math.hpp"
@@ -64,8 +67,14 @@ struct IndexedFetch {
 // with no rows.  The neighbour kernels deal their 256-slot tiles to the XCDs in
 // contiguous runs (xcd_tile), so a tile and its neighbours' tiles share an L2: 12%
 // faster drop-in steps on the 1M dam break.
-template <class Lookup>
+//
+// kStaged: the launcher has gathered the inputs into cell-sorted staging buffers (Ps:
+// position with lambda or 0 in w; Vs: velocity), so the walk reads candidates there
+// in slot order (SortedFetch, queueing slots) instead of through the sorted index
+// array.  The pairs, their order and their arithmetic are the same either way.
+template <class Lookup, bool kStaged>
 __global__ __launch_bounds__(kBS) void k_lambda_ref(float* __restrict__ lambda, const float4* __restrict__ P,
+                                                    const float4* __restrict__ Ps,
                                                     const uint32_t* __restrict__ idxS,
                                                     const uint32_t* __restrict__ keyS, Lookup lookup,
                                                     sim::DeviceParams dp, uint32_t N) {
@@ -75,7 +84,7 @@ __global__ __launch_bounds__(kBS) void k_lambda_ref(float* __restrict__ lambda, 
     const bool valid = s < N;
     const uint32_t ss = valid ? s : N - 1;
     const uint32_t i = idxS[ss];
-    const float4 pi = P[i];
+    const float4 pi = kStaged ? Ps[ss] : P[i];
     const PairConsts c = make_pair_consts(dp);
     LambdaAcc a;
     auto dist2 = [&](const float4& pj) {
@@ -83,34 +92,39 @@ __global__ __launch_bounds__(kBS) void k_lambda_ref(float* __restrict__ lambda, 
         return rx * rx + ry * ry + rz * rz;
     };
     const CellRows<Lookup> rows(lookup, dp.grid, reach_of(c.h, dp.grid.cellSize), keyS[ss]);
-    with_pair_form(c, [&](auto form) {
-        auto apply = [&](uint32_t j) {
-            const float4 pj = P[j];
-            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            lambda_pair<decltype(form)::value>(a, rx, ry, rz, rx * rx + ry * ry + rz * rz, c);
-        };
-        neighbor_walk<false, kBS>(valid, s, rows, CandMask{}, IndexedFetch{P, idxS}, sBuf + (threadIdx.x & ~63u),
-                                  sQ + threadIdx.x, dp.maxNeighbors, c.h2, dist2, apply);
-    });
+    auto walk = [&](const auto& fetch) {
+        with_pair_form(c, [&](auto form) {
+            auto apply = [&](uint32_t j) {
+                const float4 pj = fetch.at(j);
+                const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+                lambda_pair<decltype(form)::value>(a, rx, ry, rz, rx * rx + ry * ry + rz * rz, c);
+            };
+            neighbor_walk<false, kBS>(valid, s, rows, CandMask{}, fetch, sBuf + (threadIdx.x & ~63u),
+                                      sQ + threadIdx.x, dp.maxNeighbors, c.h2, dist2, apply);
+        });
+    };
+    if (kStaged)
+        walk(SortedFetch{Ps});
+    else
+        walk(IndexedFetch{P, idxS});
     if (valid) lambda[i] = lambda_finish(a, dp);
 }
 
 // Held to 72 VGPRs (seven waves per SIMD); at its natural 77 it ran six, and the
 // extra wave made the drop-in Δ pass 7% faster on the 1M dam break.
-template <class Lookup>
-__global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k_delta_ref(float4* __restrict__ delta, const float4* __restrict__ P,
-                                                   const float* __restrict__ lambda,
-                                                   const uint32_t* __restrict__ idxS,
-                                                   const uint32_t* __restrict__ keyS, Lookup lookup,
-                                                   sim::DeviceParams dp, uint32_t N) {
+template <class Lookup, bool kStaged>
+__global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k_delta_ref(
+    float4* __restrict__ delta, const float4* __restrict__ P, const float4* __restrict__ Ps,
+    const float* __restrict__ lambda, const uint32_t* __restrict__ idxS, const uint32_t* __restrict__ keyS,
+    Lookup lookup, sim::DeviceParams dp, uint32_t N) {
     __shared__ uint32_t sQ[kQD * kBS];
     __shared__ float4 sBuf[kBS];
     const uint32_t s = xcd_tile(blockIdx.x, gridDim.x) * kBS + threadIdx.x;
     const bool valid = s < N;
     const uint32_t ss = valid ? s : N - 1;
     const uint32_t i = idxS[ss];
-    const float4 pi = P[i];
-    const float lam_i = lambda[i];
+    const float4 pi = kStaged ? Ps[ss] : P[i];
+    const float lam_i = kStaged ? pi.w : lambda[i];
     const PairConsts c = make_pair_consts(dp);
     float dx = 0.f, dy = 0.f, dz = 0.f;
     auto dist2 = [&](const float4& pj) {
@@ -118,16 +132,23 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k
         return rx * rx + ry * ry + rz * rz;
     };
     const CellRows<Lookup> rows(lookup, dp.grid, reach_of(c.h, dp.grid.cellSize), keyS[ss]);
-    with_pair_form(c, [&](auto form) {
-        auto apply = [&](uint32_t j) {
-            const float4 pj = P[j];
-            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            delta_pair<decltype(form)::value>(dx, dy, dz, rx, ry, rz, rx * rx + ry * ry + rz * rz, lam_i, lambda[j],
-                                              c);
-        };
-        neighbor_walk<false, kBS>(valid, s, rows, CandMask{}, IndexedFetch{P, idxS}, sBuf + (threadIdx.x & ~63u),
-                                  sQ + threadIdx.x, 0, c.h2, dist2, apply);
-    });
+    auto walk = [&](const auto& fetch) {
+        with_pair_form(c, [&](auto form) {
+            auto apply = [&](uint32_t j) {
+                const float4 pj = fetch.at(j);
+                const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+                const float lam_j = kStaged ? pj.w : lambda[j];
+                delta_pair<decltype(form)::value>(dx, dy, dz, rx, ry, rz, rx * rx + ry * ry + rz * rz, lam_i,
+                                                  lam_j, c);
+            };
+            neighbor_walk<false, kBS>(valid, s, rows, CandMask{}, fetch, sBuf + (threadIdx.x & ~63u),
+                                      sQ + threadIdx.x, 0, c.h2, dist2, apply);
+        });
+    };
+    if (kStaged)
+        walk(SortedFetch{Ps});
+    else
+        walk(IndexedFetch{P, idxS});
     if (!valid) return;
     const float3 d = delta_finish(dx, dy, dz, dp);
     delta[i] = make_float4(d.x, d.y, d.z, 0.0f);
@@ -144,9 +165,10 @@ __global__ __launch_bounds__(kBS) void k_apply_delta(float4* __restrict__ P, con
     P[i] = make_float4(b.x + d.x, b.y + d.y, b.z + d.z, b.w);
 }
 
-template <class Lookup>
+template <class Lookup, bool kStaged>
 __global__ __launch_bounds__(kBS) void k_xsph_ref(float4* __restrict__ Vout, const float4* __restrict__ Vin,
-                                                  const float4* __restrict__ P, const uint32_t* __restrict__ idxS,
+                                                  const float4* __restrict__ P, const float4* __restrict__ Ps,
+                                                  const float4* __restrict__ Vs, const uint32_t* __restrict__ idxS,
                                                   const uint32_t* __restrict__ keyS, Lookup lookup,
                                                   sim::DeviceParams dp, uint32_t N) {
     __shared__ uint32_t sQ[kQD * kBS];
@@ -155,8 +177,8 @@ __global__ __launch_bounds__(kBS) void k_xsph_ref(float4* __restrict__ Vout, con
     const bool valid = s < N;
     const uint32_t ss = valid ? s : N - 1;
     const uint32_t i = idxS[ss];
-    const float4 pi = P[i];
-    const float4 vi = Vin[i];
+    const float4 pi = kStaged ? Ps[ss] : P[i];
+    const float4 vi = kStaged ? Vs[ss] : Vin[i];
     const float h2 = dp.kernel.h2, poly6 = dp.kernel.poly6;
     float sx = 0.f, sy = 0.f, sz = 0.f;
     int n = 0;
@@ -164,23 +186,62 @@ __global__ __launch_bounds__(kBS) void k_xsph_ref(float4* __restrict__ Vout, con
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
         return rx * rx + ry * ry + rz * rz;
     };
-    auto apply = [&](uint32_t j) {
-        const float4 pj = P[j];
-        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-        const float hr2 = h2 - (rx * rx + ry * ry + rz * rz);
-        const float w = poly6 * hr2 * hr2 * hr2;
-        const float4 vj = Vin[j];
-        sx += (vj.x - vi.x) * w;
-        sy += (vj.y - vi.y) * w;
-        sz += (vj.z - vi.z) * w;
-        ++n;
-    };
     const CellRows<Lookup> rows(lookup, dp.grid, reach_of(dp.kernel.h, dp.grid.cellSize), keyS[ss]);
-    neighbor_walk<false, kBS>(valid, s, rows, CandMask{}, IndexedFetch{P, idxS}, sBuf + (threadIdx.x & ~63u),
-                         sQ + threadIdx.x, 0, h2, dist2, apply);
+    auto walk = [&](const auto& fetch) {
+        auto apply = [&](uint32_t j) {
+            const float4 pj = fetch.at(j);
+            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+            const float hr2 = h2 - (rx * rx + ry * ry + rz * rz);
+            const float w = poly6 * hr2 * hr2 * hr2;
+            const float4 vj = kStaged ? Vs[j] : Vin[j];
+            sx += (vj.x - vi.x) * w;
+            sy += (vj.y - vi.y) * w;
+            sz += (vj.z - vi.z) * w;
+            ++n;
+        };
+        neighbor_walk<false, kBS>(valid, s, rows, CandMask{}, fetch, sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x,
+                                  0, h2, dist2, apply);
+    };
+    if (kStaged)
+        walk(SortedFetch{Ps});
+    else
+        walk(IndexedFetch{P, idxS});
     if (!valid) return;
     const float3 v = xsph_finish(vi.x, vi.y, vi.z, sx, sy, sz, n, dp, dp.xsph_c);
     Vout[i] = make_float4(v.x, v.y, v.z, 0.0f);
+}
+
+// Staging gather: Ps[s] = (P[i].xyz, lambda[i] or 0), Vs[s] = V[i] (when V), i = idxS[s].
+__global__ __launch_bounds__(kBS) void k_stage(float4* __restrict__ Ps, float4* __restrict__ Vs,
+                                               const float4* __restrict__ P, const float* __restrict__ lambda,
+                                               const float4* __restrict__ V, const uint32_t* __restrict__ idxS,
+                                               uint32_t N) {
+    const uint32_t s = blockIdx.x * kBS + threadIdx.x;
+    if (s >= N) return;
+    const uint32_t i = idxS[s];
+    const float4 p = P[i];
+    Ps[s] = make_float4(p.x, p.y, p.z, lambda ? lambda[i] : 0.0f);
+    if (V) Vs[s] = V[i];
+}
+
+// Staging buffers of one stream, sized by EnsureDropInStaging: N + kSlotPad slots each,
+// the pad zeroed once (the walk's padded loads may read it).
+struct Staging {
+    float4* P = nullptr;
+    float4* V = nullptr;
+    uint32_t capacity = 0;
+};
+std::mutex g_stage_mu;
+std::unordered_map<hipStream_t, Staging> g_stage;
+
+// The stream's staging buffers when they hold N slots, else nullptr (read through the
+// sorted index array).
+bool staging_for(hipStream_t s, uint32_t N, Staging& out) {
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    const auto it = g_stage.find(s);
+    if (it == g_stage.end() || it->second.capacity < N) return false;
+    out = it->second;
+    return true;
 }
 
 // Compact variants read M on the device (the count is produced on the stream).
@@ -207,12 +268,87 @@ bool check_dims(const sim::DeviceParams& dp, const char* where) {
 
 using namespace pbfx;
 
+namespace pbfx {
+namespace {
+
+template <class Lookup>
+void launch_lambda(float* lambda, const float4* P, const uint32_t* idxS, const uint32_t* keyS, const Lookup& lk,
+                   const sim::DeviceParams& dp, uint32_t N, hipStream_t s) {
+    Staging st;
+    if (staging_for(s, N, st)) {
+        k_stage<<<ceil_div(N, kBS), kBS, 0, s>>>(st.P, nullptr, P, nullptr, nullptr, idxS, N);
+        k_lambda_ref<Lookup, true><<<ceil_div(N, kBS), kBS, 0, s>>>(lambda, P, st.P, idxS, keyS, lk, dp, N);
+    } else {
+        k_lambda_ref<Lookup, false><<<ceil_div(N, kBS), kBS, 0, s>>>(lambda, P, nullptr, idxS, keyS, lk, dp, N);
+    }
+}
+
+template <class Lookup>
+void launch_delta(float4* P, float4* delta, const float* lambda, const uint32_t* idxS, const uint32_t* keyS,
+                  const Lookup& lk, const sim::DeviceParams& dp, uint32_t N, hipStream_t s) {
+    Staging st;
+    if (staging_for(s, N, st)) {
+        k_stage<<<ceil_div(N, kBS), kBS, 0, s>>>(st.P, nullptr, P, lambda, nullptr, idxS, N);
+        k_delta_ref<Lookup, true><<<ceil_div(N, kBS), kBS, 0, s>>>(delta, P, st.P, lambda, idxS, keyS, lk, dp, N);
+    } else {
+        k_delta_ref<Lookup, false><<<ceil_div(N, kBS), kBS, 0, s>>>(delta, P, nullptr, lambda, idxS, keyS, lk, dp,
+                                                                     N);
+    }
+    k_apply_delta<<<ceil_div(N, kBS), kBS, 0, s>>>(P, delta, idxS, N);
+}
+
+template <class Lookup>
+void launch_xsph(float4* Vout, const float4* Vin, const float4* P, const uint32_t* idxS, const uint32_t* keyS,
+                 const Lookup& lk, const sim::DeviceParams& dp, uint32_t N, hipStream_t s) {
+    Staging st;
+    if (staging_for(s, N, st)) {
+        k_stage<<<ceil_div(N, kBS), kBS, 0, s>>>(st.P, st.V, P, nullptr, Vin, idxS, N);
+        k_xsph_ref<Lookup, true><<<ceil_div(N, kBS), kBS, 0, s>>>(Vout, Vin, P, st.P, st.V, idxS, keyS, lk, dp, N);
+    } else {
+        k_xsph_ref<Lookup, false><<<ceil_div(N, kBS), kBS, 0, s>>>(Vout, Vin, P, nullptr, nullptr, idxS, keyS, lk,
+                                                                    dp, N);
+    }
+}
+
+}  // namespace
+}  // namespace pbfx
+
+extern "C" bool EnsureDropInStaging(uint32_t N, hipStream_t stream) {
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    Staging& st = g_stage[stream];
+    if (N <= st.capacity) return true;
+    if (st.P) (void)hipFree(st.P);
+    if (st.V) (void)hipFree(st.V);
+    st = Staging{};
+    const size_t slots = (size_t)N + kSlotPad;
+    if (hipMalloc(&st.P, sizeof(float4) * slots) != hipSuccess ||
+        hipMalloc(&st.V, sizeof(float4) * slots) != hipSuccess ||
+        hipMemset(st.P, 0, sizeof(float4) * slots) != hipSuccess ||
+        hipMemset(st.V, 0, sizeof(float4) * slots) != hipSuccess) {
+        if (st.P) (void)hipFree(st.P);
+        if (st.V) (void)hipFree(st.V);
+        g_stage.erase(stream);
+        set_error(PBFX_E_CAPACITY, "EnsureDropInStaging: allocation failed");
+        return false;
+    }
+    st.capacity = N;
+    return true;
+}
+
+extern "C" void ReleaseDropInStaging(hipStream_t stream) {
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    const auto it = g_stage.find(stream);
+    if (it == g_stage.end()) return;
+    if (it->second.P) (void)hipFree(it->second.P);
+    if (it->second.V) (void)hipFree(it->second.V);
+    g_stage.erase(it);
+}
+
 extern "C" void LaunchLambda(float* lambda, const float4* pos_pred, const uint32_t* indicesSorted,
                              const uint32_t* keysSorted, const uint32_t* cellStart, const uint32_t* cellEnd,
                              sim::DeviceParams dp, uint32_t N, hipStream_t s) {
     if (N == 0 || !check_dims(dp, "LaunchLambda: empty grid")) return;
-    k_lambda_ref<DenseRows><<<ceil_div(N, kBS), kBS, 0, s>>>(lambda, pos_pred, indicesSorted, keysSorted,
-                                                            DenseRows{cellStart, cellEnd}, dp, N);
+    launch_lambda(lambda, pos_pred, indicesSorted, keysSorted, DenseRows{cellStart, cellEnd}, dp, N, s);
     check_launch("LaunchLambda");
 }
 
@@ -220,8 +356,8 @@ extern "C" void LaunchLambdaCompact(float* lambda, const float4* pos_pred, const
                                     const uint32_t* keysSorted, const uint32_t* uniqueKeys, const uint32_t* offsets,
                                     const uint32_t* compactCount, sim::DeviceParams dp, uint32_t N, hipStream_t s) {
     if (N == 0 || !check_dims(dp, "LaunchLambdaCompact: empty grid")) return;
-    k_lambda_ref<CompactRowsDev><<<ceil_div(N, kBS), kBS, 0, s>>>(
-        lambda, pos_pred, indicesSorted, keysSorted, CompactRowsDev{uniqueKeys, offsets, compactCount}, dp, N);
+    launch_lambda(lambda, pos_pred, indicesSorted, keysSorted, CompactRowsDev{uniqueKeys, offsets, compactCount}, dp,
+                  N, s);
     check_launch("LaunchLambdaCompact");
 }
 
@@ -229,9 +365,7 @@ extern "C" void LaunchDeltaApply(float4* pos_pred, float4* delta, const float* l
                                  const uint32_t* keysSorted, const uint32_t* cellStart, const uint32_t* cellEnd,
                                  sim::DeviceParams dp, uint32_t N, hipStream_t s) {
     if (N == 0 || !check_dims(dp, "LaunchDeltaApply: empty grid")) return;
-    k_delta_ref<DenseRows><<<ceil_div(N, kBS), kBS, 0, s>>>(delta, pos_pred, lambda, indicesSorted, keysSorted,
-                                                           DenseRows{cellStart, cellEnd}, dp, N);
-    k_apply_delta<<<ceil_div(N, kBS), kBS, 0, s>>>(pos_pred, delta, indicesSorted, N);
+    launch_delta(pos_pred, delta, lambda, indicesSorted, keysSorted, DenseRows{cellStart, cellEnd}, dp, N, s);
     check_launch("LaunchDeltaApply");
 }
 
@@ -241,11 +375,8 @@ extern "C" void LaunchDeltaApplyCompact(float4* pos_pred, float4* delta, const f
                                         const uint32_t* compactCount, sim::DeviceParams dp, uint32_t N,
                                         hipStream_t s) {
     if (N == 0 || !check_dims(dp, "LaunchDeltaApplyCompact: empty grid")) return;
-    k_delta_ref<CompactRowsDev><<<ceil_div(N, kBS), kBS, 0, s>>>(delta, pos_pred, lambda, indicesSorted,
-                                                                keysSorted,
-                                                                CompactRowsDev{uniqueKeys, offsets, compactCount},
-                                                                dp, N);
-    k_apply_delta<<<ceil_div(N, kBS), kBS, 0, s>>>(pos_pred, delta, indicesSorted, N);
+    launch_delta(pos_pred, delta, lambda, indicesSorted, keysSorted, CompactRowsDev{uniqueKeys, offsets, compactCount},
+                 dp, N, s);
     check_launch("LaunchDeltaApplyCompact");
 }
 
@@ -255,8 +386,7 @@ extern "C" void LaunchXSPH(float4* vel_out, const float4* vel_in, const float4* 
     if (N == 0 || dp.xsph_c <= 0.f) return;
     if (vel_out == vel_in) { set_error(PBFX_E_ALIAS, "LaunchXSPH: vel_out aliases vel_in"); return; }
     if (!check_dims(dp, "LaunchXSPH: empty grid")) return;
-    k_xsph_ref<DenseRows><<<ceil_div(N, kBS), kBS, 0, s>>>(vel_out, vel_in, pos_pred, indicesSorted, keysSorted,
-                                                          DenseRows{cellStart, cellEnd}, dp, N);
+    launch_xsph(vel_out, vel_in, pos_pred, indicesSorted, keysSorted, DenseRows{cellStart, cellEnd}, dp, N, s);
     check_launch("LaunchXSPH");
 }
 
@@ -267,9 +397,7 @@ extern "C" void LaunchXSPHCompact(float4* vel_out, const float4* vel_in, const f
     if (N == 0 || dp.xsph_c <= 0.f) return;
     if (vel_out == vel_in) { set_error(PBFX_E_ALIAS, "LaunchXSPHCompact: vel_out aliases vel_in"); return; }
     if (!check_dims(dp, "LaunchXSPHCompact: empty grid")) return;
-    k_xsph_ref<CompactRowsDev><<<ceil_div(N, kBS), kBS, 0, s>>>(vel_out, vel_in, pos_pred, indicesSorted,
-                                                               keysSorted,
-                                                               CompactRowsDev{uniqueKeys, offsets, compactCount},
-                                                               dp, N);
+    launch_xsph(vel_out, vel_in, pos_pred, indicesSorted, keysSorted,
+                CompactRowsDev{uniqueKeys, offsets, compactCount}, dp, N, s);
     check_launch("LaunchXSPHCompact");
 }

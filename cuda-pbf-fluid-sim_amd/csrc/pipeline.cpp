@@ -495,6 +495,8 @@ bool Simulator::initCommon(const SimParams& p, const SimOptions& opt, uint64_t c
     if (opt.path == PathKind::Reference && opt.grid != GridKind::Dense) {
         if (!EnsureCellCompactScratch(cap, 256)) return false;
     }
+    // the drop-in neighbour launchers on this simulator's stream walk cell-sorted copies
+    if (opt.path == PathKind::Reference && !EnsureDropInStaging(cap, m_stream)) return false;
     InitializeKernelDispatcher(m_dispatcher);
     BuildDefaultPipelines(m_pipeline, opt.path);
     m_ctx.bufs = &m_bufs;
@@ -531,7 +533,11 @@ void Simulator::shutdown() {
     releaseScenes();
     m_bufs.release();
     m_grid.release();
-    if (m_stream) (void)hipStreamDestroy(m_stream);
+    if (m_stream) {
+        (void)hipStreamSynchronize(m_stream);
+        ReleaseDropInStaging(m_stream);
+        (void)hipStreamDestroy(m_stream);
+    }
     m_stream = nullptr;
     m_initialized = false;
 }

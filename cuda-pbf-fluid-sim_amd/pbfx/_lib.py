@@ -27,6 +27,8 @@ SIGNATURES = {
     "LaunchSortPairs": (None, [_P, ctypes.c_size_t, _P, _P, _P, _P, _U32, _P]),
     "LaunchCellRanges": (None, [_P, _P, _P, _U32, _U32, _P]),
     "EnsureCellCompactScratch": (ctypes.c_bool, [_U32, _U32]),
+    "EnsureDropInStaging": (ctypes.c_bool, [_U32, _P]),
+    "ReleaseDropInStaging": (None, [_P]),
     "LaunchCellRangesCompact": (None, [_P, _P, _P, _P, _U32, _P]),
     "LaunchLambda": (None, [_P, _P, _P, _P, _P, _P, DeviceParams, _U32, _P]),
     "LaunchLambdaCompact": (None, [_P, _P, _P, _P, _P, _P, _P, DeviceParams, _U32, _P]),

@@ -8,7 +8,8 @@
 //
 // Conventions kept from the reference:
 //   * launchers return void, never allocate (except EnsureCellCompactScratch,
-//     which sizes the compaction scratch before graph capture), never
+//     which sizes the compaction scratch before graph capture, and
+//     EnsureDropInStaging below), never
 //     synchronise, and are safe to record into a hipGraph;
 //   * N == 0, null state pointers (LaunchIntegratePred) and xsph_c <= 0
 //     (LaunchXSPH*) return early without launching.
@@ -89,6 +90,20 @@ void LaunchXSPH(float4* vel_out, const float4* vel_in, const float4* pos_pred, c
 void LaunchXSPHCompact(float4* vel_out, const float4* vel_in, const float4* pos_pred, const uint32_t* indicesSorted,
                        const uint32_t* keysSorted, const uint32_t* uniqueKeys, const uint32_t* offsets,
                        const uint32_t* compactCount, sim::DeviceParams dp, uint32_t N, hipStream_t s);
+
+// ---- staging for the neighbour launchers (no counterpart in the reference) -----
+// EnsureDropInStaging sizes two cell-sorted staging buffers of N particles for
+// launches on `stream`.  With them, LaunchLambda / LaunchDeltaApply / LaunchXSPH and
+// their compact forms on that stream first gather their inputs into slot order (one
+// extra short kernel) and walk the gathered copy, instead of reading every candidate
+// through indicesSorted: the same pairs in the same order, the same bits, faster.
+// Launches on other streams, or with more particles than reserved, read through
+// indicesSorted as before.  Like EnsureCellCompactScratch it allocates, so call it
+// before graph capture; launches on the one stream are ordered, and buffers are never
+// shared between streams.  ReleaseDropInStaging frees a stream's buffers (call it
+// before destroying the stream).
+bool EnsureDropInStaging(uint32_t N, hipStream_t stream);
+void ReleaseDropInStaging(hipStream_t stream);
 
 // ---- statistics (diagnostics: read back to the host and synchronise `s`) -------
 // reference: sim/stats.h:17-32 (StatsKernel, stats.cu:25-98).  Means over the

@@ -19,6 +19,23 @@ def L():
     return launch
 
 
+@pytest.fixture(params=[False, True], ids=["indexed", "staged"])
+def staged(request, dev):
+    """Run the neighbour launchers as the reference's phases would (reading through
+    indicesSorted), or with staging buffers reserved for the stream (EnsureDropInStaging):
+    both must give the oracle's bits."""
+    import ctypes
+    import torch
+    from pbfx import _lib
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if request.param:
+        assert _lib.lib().EnsureDropInStaging(1 << 20, s)
+    yield request.param
+    if request.param:
+        torch.cuda.synchronize()
+        _lib.lib().ReleaseDropInStaging(s)
+
+
 @pytest.mark.parametrize("n", [0, 1, 1000, 65536 + 17])
 def test_integrate_boundary_velocity(oracle, dev, L, n):
     import torch
@@ -152,7 +169,7 @@ def _solver_inputs(oracle, dev, scene, steps=0, squeeze=None):
 @pytest.mark.parametrize("grid", ["dense", "compact"])
 @pytest.mark.parametrize("scene_name,steps,squeeze", [("block16", 0, None), ("block16", 0, (0.8, 0.1)),
                                                       ("block20", 15, None)])
-def test_lambda_delta_xsph(oracle, dev, L, grid, scene_name, steps, squeeze):
+def test_lambda_delta_xsph(oracle, dev, L, staged, grid, scene_name, steps, squeeze):
     import torch
     n = 16 if scene_name == "block16" else 20
     scene = P.small_block((n, n, n))
@@ -198,7 +215,7 @@ def test_lambda_delta_xsph(oracle, dev, L, grid, scene_name, steps, squeeze):
 
 
 @pytest.mark.parametrize("grid", ["dense", "compact"])
-def test_lambda_delta_xsph_reach_two(oracle, dev, L, grid):
+def test_lambda_delta_xsph_reach_two(oracle, dev, L, staged, grid):
     # cells of 0.6 h: every neighbour row spans five cells and there are 25 rows
     import torch
     scene = P.small_block((12, 12, 12))
@@ -240,7 +257,7 @@ def test_lambda_delta_xsph_reach_two(oracle, dev, L, grid):
 
 
 @pytest.mark.parametrize("grid", ["dense", "compact"])
-def test_lambda_delta_xsph_partial_wave(oracle, dev, L, grid):
+def test_lambda_delta_xsph_partial_wave(oracle, dev, L, staged, grid):
     # 1001 particles (13 x 11 x 7), squeezed: the last wave has 41 lanes
     import torch
     scene = P.small_block((13, 11, 7))
@@ -277,7 +294,7 @@ def test_lambda_delta_xsph_partial_wave(oracle, dev, L, grid):
     np.testing.assert_array_equal(to_host(tVout), eV)
 
 
-def test_lambda_neighbor_cap(oracle, dev, L):
+def test_lambda_neighbor_cap(oracle, dev, L, staged):
     import torch
     scene = P.small_block((12, 12, 12))
     sp, X, V, Pp, g, t = _solver_inputs(oracle, dev, scene, 0, (0.6, 0.2))
