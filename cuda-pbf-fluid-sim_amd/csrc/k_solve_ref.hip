@@ -9,7 +9,9 @@
 // formed in the same order as the reference's per-cell loops.  The walk is the
 // cell-reordered kernels' wave walk (walk.hpp: unrolled per-lane candidate loads,
 // in-range pairs deferred through a per-lane FIFO), with each candidate's position
-// read through the sorted index array.
+// read through the sorted index array, or, when staging buffers are reserved for the
+// launch stream (EnsureDropInStaging), from a cell-sorted copy the launcher gathers
+// first.
 // The headless pipeline uses the cell-reordered kernels in k_solve_sorted.hip;
 // these exist so the sim/ phase API can be served unchanged.
 #include <mutex>
