@@ -99,9 +99,10 @@ void LaunchXSPHCompact(float4* vel_out, const float4* vel_in, const float4* pos_
 // through indicesSorted: the same pairs in the same order, the same bits, faster.
 // Launches on other streams, or with more particles than reserved, read through
 // indicesSorted as before.  Like EnsureCellCompactScratch it allocates, so call it
-// before graph capture; launches on the one stream are ordered, and buffers are never
-// shared between streams.  ReleaseDropInStaging frees a stream's buffers (call it
-// before destroying the stream).
+// before graph capture; growing the reservation frees the old buffers, so a graph
+// captured before then must be captured again.  Launches on the one stream are
+// ordered, and buffers are never shared between streams.  ReleaseDropInStaging frees a
+// stream's buffers (call it once the stream is idle, before destroying it).
 bool EnsureDropInStaging(uint32_t N, hipStream_t stream);
 void ReleaseDropInStaging(hipStream_t stream);
 
