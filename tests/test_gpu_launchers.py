@@ -306,6 +306,34 @@ def test_lambda_neighbor_cap(oracle, dev, L, staged):
     np.testing.assert_array_equal(to_host(lam), elam)
 
 
+def test_staging_smaller_than_the_launch_falls_back(oracle, dev, L):
+    """Staging reserved for fewer particles than a launch holds, grown, then released:
+    every launch gives the oracle's bits (the walk reads through indicesSorted whenever
+    the stream's staging is too small or absent)."""
+    import ctypes
+    import torch
+    from pbfx import _lib
+    scene = P.small_block((12, 12, 12))
+    sp, X, V, Pp, g, t = _solver_inputs(oracle, dev, scene, 0, (0.8, 0.1))
+    dp = P.make_device_params(sp)
+    N = len(Pp)
+    elam = oracle.lambda_dense(Pp, g["idxS"], g["keyS"], g["start"], g["end"], dp)
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    try:
+        for reserve in (N // 2, N, 0):
+            if reserve:
+                assert _lib.lib().EnsureDropInStaging(reserve, s)
+            else:
+                torch.cuda.synchronize()
+                _lib.lib().ReleaseDropInStaging(s)
+            lam = torch.zeros(N, dtype=torch.float32, device=dev)
+            L.lambda_dense(lam, to_dev(Pp, dev), t["idxS"], t["keyS"], t["start"], t["end"], dp)
+            np.testing.assert_array_equal(to_host(lam), elam)
+    finally:
+        torch.cuda.synchronize()
+        _lib.lib().ReleaseDropInStaging(s)
+
+
 def test_xsph_rejects_in_place(dev, L):
     import torch
     from pbfx import PbfError
