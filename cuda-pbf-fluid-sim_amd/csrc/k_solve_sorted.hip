@@ -13,6 +13,8 @@
 // Launch shape: 256-lane blocks over sorted slots, block index remapped so each
 // XCD owns one contiguous run of slots (its neighbours' rows then sit in the
 // same L2).
+#include <type_traits>
+
 #include "pbf_common.hpp"
 #include "pbf_hip.h"
 #include "pbf_math.hpp"
@@ -166,6 +168,74 @@ __device__ __forceinline__ void cell_spans(uint32_t key, uint32_t m, const Rows&
     }
 }
 
+struct HashRowsDev;
+
+// Row spans of a one-cell reach from the hash table, the lookups shared across a wave:
+// each lane looks up only its own column's cell of every neighbour row and takes the
+// cells on either side from the neighbouring lanes when those hold them (consecutive
+// occupied cells of one x-line sit in consecutive lanes), looking them up itself
+// otherwise.  The min / max over a window are exact, so the spans equal cell_spans'.
+// Every lane of the wave takes part (has: the lane holds occupied cell m).
+__device__ __forceinline__ void cell_spans_hashed_r1(uint32_t key, uint32_t m, bool has, const HashRows& hr,
+                                                     const SpanTable& st, sim::GridBounds g, const Scenes& sc) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t base0 = 0;
+    if (has && sc.params) {
+        const SceneParams& sp = sc.params[scene_of_key(sc, key)];
+        g = sp.grid;
+        base0 = sp.cellBase;
+        key -= base0;
+    }
+    int cx = 0, cy = 0, cz = 0;
+    if (has) decode_key(key, g.dim, cx, cy, cz);
+    int r = 0;
+    for (int dz = -1; dz <= 1; ++dz) {
+        for (int dy = -1; dy <= 1; ++dy, ++r) {
+            const int z = cz + dz, y = cy + dy;
+            const bool row = has && z >= 0 && z < g.dim.z && y >= 0 && y < g.dim.y;
+            const uint32_t c = row ? base0 + (uint32_t)((z * g.dim.y + y) * g.dim.x + cx) : 0xFFFFFFFFu;
+            uint32_t b = 0, e = 0;
+            const bool occ = row && hr.cell(c, b, e);
+            const uint32_t cL = (uint32_t)__shfl_up((int)c, 1), bL = (uint32_t)__shfl_up((int)b, 1),
+                           eL = (uint32_t)__shfl_up((int)e, 1);
+            const int occL = __shfl_up((int)occ, 1);
+            const uint32_t cR = (uint32_t)__shfl_down((int)c, 1), bR = (uint32_t)__shfl_down((int)b, 1),
+                           eR = (uint32_t)__shfl_down((int)e, 1);
+            const int occR = __shfl_down((int)occ, 1);
+            uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+            auto take = [&](bool o, uint32_t bb, uint32_t ee) {
+                if (o) {
+                    lo = min(lo, bb);
+                    hi = max(hi, ee);
+                }
+            };
+            if (row) {
+                if (cx > 0) {
+                    if (lane > 0u && cL == c - 1u) {
+                        take(occL != 0, bL, eL);
+                    } else {
+                        uint32_t bb = 0, ee = 0;
+                        const bool o = hr.cell(c - 1u, bb, ee);
+                        take(o, bb, ee);
+                    }
+                }
+                take(occ, b, e);
+                if (cx + 1 < g.dim.x) {
+                    if (lane < 63u && cR == c + 1u) {
+                        take(occR != 0, bR, eR);
+                    } else {
+                        uint32_t bb = 0, ee = 0;
+                        const bool o = hr.cell(c + 1u, bb, ee);
+                        take(o, bb, ee);
+                    }
+                }
+            }
+            if (lo == 0xFFFFFFFFu) lo = hi = 0u;
+            if (has) st.spans[(uint64_t)r * st.stride + m] = make_uint2(lo, hi);
+        }
+    }
+}
+
 template <class Rows>
 __global__ __launch_bounds__(kBS) void k_spans_gather(const float4* __restrict__ Xn, const uint32_t* __restrict__ idxS,
                                                       float4* __restrict__ Ps, uint32_t N,
@@ -180,6 +250,14 @@ __global__ __launch_bounds__(kBS) void k_spans_gather(const float4* __restrict__
         if (sc.sceneOfSlot) sc.sceneOfSlot[t] = sc.sceneOf[i];
     }
     const uint32_t M = *count;
+    if constexpr (std::is_same<Rows, HashRowsDev>::value) {
+        if (st.reach == 1) {
+            const bool has = t < M;
+            cell_spans_hashed_r1(has ? uniq[t] : 0u, t, has, HashRows{rows.slots, rows.mask, rows.scatter, *rows.epoch},
+                                 st, g, sc);
+            return;
+        }
+    }
     if (t < M) cell_spans(uniq[t], t, rows, st, g, sc);
 }
 
