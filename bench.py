@@ -480,7 +480,8 @@ def main():
         "config": {"workload": "dam_break_1m", "particles": N, "lattice": [100, 100, 100],
                    "box_m": [2.0, 1.0, 1.0], "r_m": 0.005, "h_over_r": 2.5, "solver_iters": sp.solverIters,
                    "dt": sp.dt, "xsph_c": sp.xsph_c, "grid": a.grid, "table_log2": scene.table_log2,
-                   "cells": list(sp.grid.dim), "graph": True},
+                   "cells": list(sp.grid.dim),
+                   "timed_steps": "eager launches with a HIP event after each (graph replay reported beside)"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5), "traffic": traffic,
                      "traffic_note": traffic_note, "bytes_per_launch": dk["bytes_per_launch"],
