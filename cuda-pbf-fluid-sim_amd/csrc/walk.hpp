@@ -297,36 +297,41 @@ __device__ __forceinline__ void mask_walk(bool valid, uint32_t s, uint32_t m, co
     bool active = valid && total > 0;
     while (__any(active)) {
         pop_above(kQD - (int)kUnroll);
+        // A round takes up to kUnroll candidates from one mask word (its lowest set bits),
+        // or from the unmasked tail once the words are used up; empty words are skipped
+        // once per round rather than once per candidate.
+        if (active) {
+            while (word == 0u && w + 1 < nw) {
+                ++w;
+                word = next;
+                next = (w + 1 < nw) ? mk.words[(uint64_t)(w + 1) * mk.stride + s] : 0u;
+            }
+        }
+        const bool fromTail = word == 0u;
         // ks[u] for u >= cnt repeats the last taken slot (slot 0 when none was taken), so
-        // every load below is in bounds and branch-free
+        // every load below is in bounds and branch-free; taken candidates come first
         uint32_t ks[kUnroll];
         uint32_t cnt = 0, prev = 0u;
-        bool more = active;
 #pragma unroll
         for (uint32_t u = 0; u < kUnroll; ++u) {
-            if (more) {
-                while (word == 0u && w + 1 < nw) {
-                    ++w;
-                    word = next;
-                    next = (w + 1 < nw) ? mk.words[(uint64_t)(w + 1) * mk.stride + s] : 0u;
-                }
-                uint32_t ci = 0u;
-                if (word != 0u) {
-                    ci = w * 32u + (uint32_t)__ffs((int)word) - 1u;
-                    word &= word - 1u;
-                } else if (tail < total) {
-                    ci = tail++;
-                } else {
-                    more = false;
-                }
-                if (more) {
-                    prev = slot_of(ci);
-                    ++cnt;
-                }
+            bool take;
+            uint32_t ci;
+            if (!fromTail) {
+                take = word != 0u;
+                ci = w * 32u + (uint32_t)__ffs((int)word) - 1u;
+                word &= word - 1u;
+            } else {
+                take = tail < total;
+                ci = tail;
+                tail += take ? 1u : 0u;
+            }
+            if (active && take) {
+                prev = slot_of(ci);
+                ++cnt;
             }
             ks[u] = prev;
         }
-        if (cnt < kUnroll) active = false;  // every candidate taken
+        if (word == 0u && w + 1 >= nw && tail >= total) active = false;  // every candidate taken
         float4 pj[kUnroll];
 #pragma unroll
         for (uint32_t u = 0; u < kUnroll; ++u) pj[u] = P[ks[u]];
