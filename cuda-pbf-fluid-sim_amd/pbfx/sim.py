@@ -171,6 +171,16 @@ class Simulator:
             _lib.check("pbfSimStep")
             raise _lib.PbfError("step failed")
 
+    def set_params(self, params):
+        """New parameters for the following steps (reference: ParamChangeTracker,
+        param_change_tracker.h:37-56).  Coefficients, gravity, dt and the like take
+        effect at the next step (a captured graph is re-captured); a change of particle
+        count, grid dims, cell size, h or solver iterations raises PbfError."""
+        if _lib.lib().pbfSimSetParams(self._h, ctypes.byref(params)) != 0:
+            _lib.check("pbfSimSetParams")
+            raise _lib.PbfError("set_params: structural change (needs a new simulator)")
+        self.params = params
+
     def synchronize(self):
         if _lib.lib().pbfSimSynchronize(self._h) != 0:
             raise _lib.PbfError("synchronize failed")

@@ -220,3 +220,71 @@ def test_8m_splash_hashed_equals_compact(dev):
     lo = np.array(list(sp.grid.mins), np.float32)
     hi = np.array(list(sp.grid.maxs), np.float32)
     assert np.all(np.isfinite(X)) and np.all(X[:, :3] >= lo) and np.all(X[:, :3] <= hi)
+
+
+def _changed_params(scene):
+    """Dynamic changes of the kind ParamChangeTracker (param_change_tracker.h:37-56)
+    passes through without a rebuild: step size, gravity, XSPH, s_corr, the displacement
+    clamp (which also widens the candidate mask's skin) and the neighbour cap."""
+    sp = P.build_sim_params(scene)
+    sp.dt = sp.dt * 0.75
+    sp.gravity.x = 2.0
+    sp.xsph_c = 0.03
+    sp.pbf.scorr_k = 0.005
+    sp.pbf.disp_clamp_max_h = 0.08
+    sp.maxNeighbors = 48
+    return sp
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+@pytest.mark.parametrize("path,grid", [("reordered", "hashed"), ("reordered", "dense"), ("reference", "compact")])
+def test_set_params_between_steps_matches_oracle(oracle, dev, use_graph, path, grid):
+    scene = P.small_block((14, 14, 14), table_log2=16)
+    spA = P.build_sim_params(scene)
+    spB = _changed_params(scene)
+    sim = _sim(scene, grid, path=path, use_graph=use_graph)
+    try:
+        sim.step(3)
+        sim.set_params(spB)
+        sim.step(3)
+        X, V = sim.state()
+    finally:
+        sim.close()
+    X0 = P.seed_scene_reference(scene, oracle.seed_lattice)
+    compact = grid != "dense"
+    eX, eV = oracle.run(X0, np.zeros_like(X0), spA, 3, compact=compact)
+    eX, eV = oracle.run(eX, eV, spB, 3, compact=compact)
+    np.testing.assert_array_equal(X, eX)
+    np.testing.assert_array_equal(V, eV)
+
+
+def test_set_params_rejects_structural_changes(dev):
+    from pbfx import _lib
+    scene = P.small_block((8, 8, 8), table_log2=16)
+    sim = _sim(scene, "hashed")
+    try:
+        sim.step(1)
+        X1, _ = sim.state()
+        sim.set_params(P.build_sim_params(scene))  # unchanged: accepted, nothing to do
+        for field, value in (("solverIters", 3), ("numParticles", 100)):
+            sp = P.build_sim_params(scene)
+            setattr(sp, field, value)
+            with pytest.raises(_lib.PbfError):
+                sim.set_params(sp)
+        sp = P.build_sim_params(scene)
+        sp.kernel.h = sp.kernel.h * 1.5
+        with pytest.raises(_lib.PbfError):
+            sim.set_params(sp)
+        # the rejected calls left the simulator stepping with its own parameters
+        sim.step(1)
+        X2, _ = sim.state()
+    finally:
+        sim.close()
+    ref = _sim(scene, "hashed")
+    try:
+        ref.step(2)
+        eX, _ = ref.state()
+    finally:
+        ref.close()
+    assert not np.array_equal(X1, X2)
+    np.testing.assert_array_equal(X2, eX)
