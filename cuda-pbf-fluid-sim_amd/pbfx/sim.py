@@ -18,17 +18,17 @@ class _Tuning(ctypes.Structure):
 class _CudaArray:
     """__cuda_array_interface__ holder for a device pointer owned by a simulator."""
 
-    def __init__(self, ptr, n, owner):
+    def __init__(self, ptr, shape, typestr, owner):
         self.owner = owner
-        self.__cuda_array_interface__ = {"shape": (n, 4), "typestr": "<f4", "data": (int(ptr or 0), False),
+        self.__cuda_array_interface__ = {"shape": shape, "typestr": typestr, "data": (int(ptr or 0), False),
                                          "version": 2, "strides": None}
 
 
-def _device_view(ptr, n, owner):
+def _device_view(ptr, n, owner, shape=None, typestr="<f4"):
     import torch
     if not ptr:
         raise _lib.PbfError("no device buffer")
-    return torch.as_tensor(_CudaArray(ptr, n, owner), device="cuda")
+    return torch.as_tensor(_CudaArray(ptr, shape or (n, 4), typestr, owner), device="cuda")
 
 
 class Simulator:
@@ -149,6 +149,16 @@ class Simulator:
     def velocities_view(self):
         """(N, 4) float32 CUDA tensor viewing the velocity buffer (no copy)."""
         return _device_view(_lib.lib().pbfSimDeviceVelocities(self._h), self.n, self)
+
+    def sorted_grid(self):
+        """Host copies (uint32) of the last grid build's sorted cell keys and particle
+        indices (reference: d_cellKeys_sorted / d_indices_sorted, phases.cpp:44-63)."""
+        self.synchronize()
+        out = []
+        for fn in (_lib.lib().pbfSimDeviceSortedKeys, _lib.lib().pbfSimDeviceSortedIndices):
+            t = _device_view(fn(self._h), self.n, self, shape=(self.n,), typestr="<i4")
+            out.append(t.cpu().numpy().view(np.uint32).copy())
+        return tuple(out)
 
     def set_state(self, pos, vel):
         pos = np.ascontiguousarray(pos, np.float32)

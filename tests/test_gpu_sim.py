@@ -288,3 +288,47 @@ def test_set_params_rejects_structural_changes(dev):
         ref.close()
     assert not np.array_equal(X1, X2)
     np.testing.assert_array_equal(X2, eX)
+
+
+@pytest.mark.parametrize("dims", [(16, 16, 16), (13, 11, 7)])
+@pytest.mark.parametrize("path,grid", [("reordered", "hashed"), ("reordered", "dense"), ("reference", "compact")])
+def test_simulator_grid_build_matches_oracle(oracle, dev, dims, path, grid):
+    """The simulator's own grid build (pbfSimDeviceSortedKeys / Indices after a step):
+    the cell keys of the predicted, clamped positions, stably sorted with their particle
+    indices (phases.cpp:33-63), bit for bit."""
+    scene = P.small_block(dims, table_log2=16)
+    sp = P.build_sim_params(scene)
+    sim = _sim(scene, grid, path=path)
+    try:
+        sim.step(1)
+        keyS, idxS = sim.sorted_grid()
+    finally:
+        sim.close()
+    X0 = P.seed_scene_reference(scene, oracle.seed_lattice)
+    V0 = np.zeros_like(X0)
+    Pp = oracle.integrate_pred(X0, V0, (sp.gravity.x, sp.gravity.y, sp.gravity.z), sp.dt)
+    Pp, _ = oracle.boundary(Pp, V0, sp.grid, 0.0)
+    keys, idx = oracle.hash_keys(Pp, sp.grid)
+    eK, eI = oracle.sort_pairs(keys, idx)
+    assert np.all(np.diff(eK.astype(np.int64)) >= 0)
+    np.testing.assert_array_equal(keyS, eK)
+    np.testing.assert_array_equal(idxS, eI)
+
+
+def test_1m_grid_build_matches_oracle(oracle, dev):
+    """The benchmark scene's grid build at full size (1,000,000 particles, hashed grid)."""
+    scene = P.dam_break_1m()
+    sp = P.build_sim_params(scene)
+    sim = _sim(scene, "hashed")
+    try:
+        sim.step(1)
+        keyS, idxS = sim.sorted_grid()
+    finally:
+        sim.close()
+    X0 = P.seed_scene_reference(scene, oracle.seed_lattice)
+    V0 = np.zeros_like(X0)
+    Pp = oracle.integrate_pred(X0, V0, (sp.gravity.x, sp.gravity.y, sp.gravity.z), sp.dt)
+    Pp, _ = oracle.boundary(Pp, V0, sp.grid, 0.0)
+    eK, eI = oracle.sort_pairs(*oracle.hash_keys(Pp, sp.grid))
+    np.testing.assert_array_equal(keyS, eK)
+    np.testing.assert_array_equal(idxS, eI)
