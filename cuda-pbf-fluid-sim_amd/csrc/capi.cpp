@@ -101,6 +101,12 @@ extern "C" int pbfSimSetParams(PbfSim* sim, const sim::SimParams* params) {
     return sim->s.setParams(*params) ? 0 : -1;
 }
 
+extern "C" int pbfSimGetParams(PbfSim* sim, sim::SimParams* out) {
+    if (!sim || !out) return -1;
+    *out = sim->s.params();
+    return 0;
+}
+
 extern "C" int pbfSimSynchronize(PbfSim* sim) {
     if (!sim) return -1;
     return hipStreamSynchronize(sim->s.stream()) == hipSuccess ? 0 : -1;

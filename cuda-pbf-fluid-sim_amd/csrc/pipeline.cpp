@@ -575,6 +575,16 @@ bool Simulator::setParams(const SimParams& p) {
         set_error(PBFX_E_INVALID_ARGUMENT, "Simulator::setParams: structural change needs initialize()");
         return false;
     }
+    // an ensemble holds box, gravity, XSPH and restitution per scene; only the shared
+    // parameters (dt, solver coefficients, clamps, neighbour cap) can change here
+    if (!m_scenes.empty() &&
+        (std::memcmp(&p.grid, &m_params.grid, sizeof(p.grid)) != 0 || p.gravity.x != m_params.gravity.x ||
+         p.gravity.y != m_params.gravity.y || p.gravity.z != m_params.gravity.z || p.xsph_c != m_params.xsph_c ||
+         p.boundaryRestitution != m_params.boundaryRestitution)) {
+        set_error(PBFX_E_INVALID_ARGUMENT,
+                  "Simulator::setParams: box, gravity, xsph_c and restitution are per scene in an ensemble");
+        return false;
+    }
     m_params = p;
     m_graphDirty = true;
     return true;

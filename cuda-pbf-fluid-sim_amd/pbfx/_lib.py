@@ -51,6 +51,7 @@ SIGNATURES = {
     "pbfSimGetState": (_I, [_P, _P, _P, _I]),
     "pbfSimStep": (_I, [_P, _I]),
     "pbfSimSetParams": (_I, [_P, ctypes.POINTER(SimParams)]),
+    "pbfSimGetParams": (_I, [_P, ctypes.POINTER(SimParams)]),
     "pbfSimSynchronize": (_I, [_P]),
     "pbfSimStream": (_P, [_P]),
     "pbfSimDevicePositions": (_P, [_P]),

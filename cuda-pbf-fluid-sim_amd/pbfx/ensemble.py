@@ -101,6 +101,22 @@ class Ensemble:
             _lib.check("pbfSimStep")
             raise _lib.PbfError("step failed")
 
+    def current_params(self):
+        """A copy of the parameters the ensemble steps with: its shared parameters, with
+        the ensemble's particle count and key space (the base for set_params)."""
+        out = type(self.params)()
+        if _lib.lib().pbfSimGetParams(self._h, ctypes.byref(out)) != 0:
+            raise _lib.PbfError("get_params failed")
+        return out
+
+    def set_params(self, params):
+        """New shared parameters (dt, solver coefficients, clamps, neighbour cap) for the
+        following steps, given as a changed copy of current_params().  Box, gravity, XSPH
+        and restitution are per scene; changing them raises PbfError."""
+        if _lib.lib().pbfSimSetParams(self._h, ctypes.byref(params)) != 0:
+            _lib.check("pbfSimSetParams")
+            raise _lib.PbfError("set_params refused")
+
     def synchronize(self):
         if _lib.lib().pbfSimSynchronize(self._h) != 0:
             raise _lib.PbfError("synchronize failed")

@@ -44,6 +44,9 @@ int pbfSimGetState(PbfSim* sim, float4* pos, float4* vel, int toDevice);
 int pbfSimStep(PbfSim* sim, int steps);
 // New parameters for the following steps (same grid and particle count).
 int pbfSimSetParams(PbfSim* sim, const sim::SimParams* params);
+// The parameters the simulator steps with (for an ensemble: the shared ones, with the
+// ensemble's particle count and key space), as the base for pbfSimSetParams.
+int pbfSimGetParams(PbfSim* sim, sim::SimParams* out);
 int pbfSimSynchronize(PbfSim* sim);
 hipStream_t pbfSimStream(PbfSim* sim);
 const float4* pbfSimDevicePositions(PbfSim* sim);
