@@ -35,6 +35,10 @@ struct StepTape {
     float4* dispU[kMaxTapeIters] = {};  // scaled correction before the displacement clamp
     float* xsphGm = nullptr;            // XSPH gate * m / rho0
     uint32_t* sceneOfSlot = nullptr;    // ensembles: scene of each sorted slot
+    // the parameters the step was recorded with: its reverse uses them, whatever the
+    // simulator's parameters are by the time the reverse pass runs
+    sim::SimParams params{};
+    bool recorded = false;
 };
 
 // Scratch of the reverse pass (sorted order unless noted).
@@ -48,6 +52,7 @@ struct AdjointScratch {
     uint2* lamE = nullptr;    // lambda adjoint: (bits of eps = gLam e0, cap cut-off slot)
     double* partial = nullptr;  // per-block partial sums of dL/dc (one scene)
     double* gcSlot = nullptr;   // ensembles: dL/dc contribution of each sorted slot
+    uint32_t* gmax = nullptr;   // renormalised reverse: bits of max |dL/dX_t|, |dL/dV_t| (one word)
     uint32_t blocks = 0;
 };
 
@@ -56,9 +61,18 @@ struct AdjointScratch {
 // inputs).  Adds dL/dc of the step into gcAccum (device doubles: one, or one per
 // scene of an ensemble, whose scene s owns sorted slots [particleBase_s,
 // particleBase_s + n_s) because its keys follow the previous scene's).
+//
+// Renormalised form (log2Scale != nullptr, a device int): the true gradients are the
+// buffers times 2^(*log2Scale).  The step's dL/dc is added unscaled (times 2^E_in, in
+// double); then gXout and gVout are rescaled by a power of two so their largest
+// magnitude lies in [0.5, 1) (exact: a power-of-two product of a normal float) and
+// *log2Scale grows by the exponent taken out.  Every kernel of the step is linear in
+// the incoming gradients, so the rescaled buffers hold the same digits; the
+// dynamics' growth of the gradient (about 8x per step on the dam break) then never
+// overflows float32 over a long rollout.
 void adjoint_step(const StepTape& t, const SpanTable& st, const sim::SimParams& p, const Scenes& sc,
                   const float4* gXin, const float4* gVin, float4* gXout, float4* gVout, double* gcAccum,
-                  AdjointScratch& w, hipStream_t s);
+                  AdjointScratch& w, hipStream_t s, int32_t* log2Scale = nullptr);
 
 // Clamp bits (bit a: axis a was moved by the box clamp) are written by the
 // forward kernels when recording: launch_predict_hash(..., clampBits) in particle

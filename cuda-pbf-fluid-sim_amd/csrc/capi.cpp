@@ -233,3 +233,9 @@ extern "C" int pbfSimBackwardStep(PbfSim* sim, int slot, const float4* gXin, con
     if (!sim || !gXin || !gVin || !gXout || !gVout || !gcAccum) return -1;
     return sim->s.backwardStep(slot, gXin, gVin, gXout, gVout, gcAccum) ? 0 : -1;
 }
+
+extern "C" int pbfSimBackwardStepScaled(PbfSim* sim, int slot, const float4* gXin, const float4* gVin,
+                                        float4* gXout, float4* gVout, double* gcAccum, int32_t* log2Scale) {
+    if (!sim || !gXin || !gVin || !gXout || !gVout || !gcAccum || !log2Scale) return -1;
+    return sim->s.backwardStep(slot, gXin, gVin, gXout, gVout, gcAccum, log2Scale) ? 0 : -1;
+}

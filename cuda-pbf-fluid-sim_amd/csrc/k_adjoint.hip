@@ -115,10 +115,14 @@ __device__ __forceinline__ float3 hess_apply(const Spiky& s, float3 r, float3 v)
 
 // gVo carries the slot's recorded XSPH factor gm = gate m / rho0 in w, so the XSPH
 // adjoint gathers a neighbour's gradient and factor with one load.
+// gmax (renormalised form): the step's running maximum, cleared here before
+// k_adj_predict of the same step raises it.
 __global__ __launch_bounds__(kBS) void k_adj_gather(const uint32_t* __restrict__ idxS, const float4* __restrict__ gX,
                                                     const float4* __restrict__ gV, const float* __restrict__ gm,
-                                                    float4* __restrict__ gPf, float4* __restrict__ gVo, uint32_t N) {
+                                                    float4* __restrict__ gPf, float4* __restrict__ gVo, uint32_t N,
+                                                    uint32_t* __restrict__ gmax) {
     const uint32_t s = blockIdx.x * kBS + threadIdx.x;
+    if (gmax && s == 0) *gmax = 0u;
     if (s >= N) return;
     const uint32_t i = idxS[s];
     gPf[s] = gX[i];
@@ -215,16 +219,20 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k
     if (threadIdx.x == 0) partial[tile] = sh[0];
 }
 
-__global__ void k_adj_sum(const double* __restrict__ partial, uint32_t n, double* __restrict__ accum) {
+// log2Scale (renormalised form): the incoming gradients are 2^-E of the true ones, so
+// the step's dL/dc is scaled back by 2^E (exact in double).
+__global__ void k_adj_sum(const double* __restrict__ partial, uint32_t n, double* __restrict__ accum,
+                          const int32_t* __restrict__ log2Scale) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     double t = 0.0;
     for (uint32_t b = 0; b < n; ++b) t += partial[b];
-    *accum += t;
+    *accum += log2Scale ? ldexp(t, *log2Scale) : t;
 }
 
 // One block per scene: dL/dc of the scene's slots, summed in a fixed order.
 __global__ __launch_bounds__(kBS) void k_adj_scene_sum(const double* __restrict__ gcSlot, Scenes sc,
-                                                       double* __restrict__ accum) {
+                                                       double* __restrict__ accum,
+                                                       const int32_t* __restrict__ log2Scale) {
     __shared__ double sh[kBS];
     const SceneParams& sp = sc.params[blockIdx.x];
     double t = 0.0;
@@ -235,7 +243,7 @@ __global__ __launch_bounds__(kBS) void k_adj_scene_sum(const double* __restrict_
         if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) accum[blockIdx.x] += sh[0];
+    if (threadIdx.x == 0) accum[blockIdx.x] += log2Scale ? ldexp(sh[0], *log2Scale) : sh[0];
 }
 
 __global__ __launch_bounds__(kBS) void k_adj_velocity(const uint32_t* __restrict__ idxS,
@@ -384,23 +392,64 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k
     if (U) A[s] = mk4(disp_adjoint(g, U[s], dp), 0.f);
 }
 
+// |x| as ordered bits: for non-negative floats the unsigned order of the bits is the
+// order of the values, with NaN above +inf.
+__device__ __forceinline__ uint32_t abs_bits(float x) { return __float_as_uint(x) & 0x7FFFFFFFu; }
+
+// gmax (renormalised form): raised to the largest |component| this kernel writes.
 __global__ __launch_bounds__(kBS) void k_adj_predict(const uint32_t* __restrict__ idxS,
                                                      const uint32_t* __restrict__ predClamp,
                                                      const float4* __restrict__ gP, float dt, uint32_t N,
-                                                     float4* __restrict__ gX, float4* __restrict__ gV) {
+                                                     float4* __restrict__ gX, float4* __restrict__ gV,
+                                                     uint32_t* __restrict__ gmax) {
     const uint32_t s = blockIdx.x * kBS + threadIdx.x;
-    if (s >= N) return;
-    const uint32_t i = idxS[s];
-    const float3 g = mask3(f3(gP[s]), predClamp[i]);
-    gX[i] = mk4(add3(f3(gX[i]), g), 0.f);
-    gV[i] = mk4(mul3(g, dt), 0.f);
+    uint32_t m = 0u;
+    if (s < N) {
+        const uint32_t i = idxS[s];
+        const float3 g = mask3(f3(gP[s]), predClamp[i]);
+        const float3 x = add3(f3(gX[i]), g), v = mul3(g, dt);
+        gX[i] = mk4(x, 0.f);
+        gV[i] = mk4(v, 0.f);
+        m = max(max(max(abs_bits(x.x), abs_bits(x.y)), max(abs_bits(x.z), abs_bits(v.x))),
+                max(abs_bits(v.y), abs_bits(v.z)));
+    }
+    if (!gmax) return;
+    __shared__ uint32_t sMax[kBS / 64];
+    m = wave_max_u32(m);
+    if ((threadIdx.x & 63u) == 0) sMax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t b = sMax[0];
+        for (int w = 1; w < kBS / 64; ++w) b = max(b, sMax[w]);
+        if (b) atomicMax(gmax, b);
+    }
+}
+
+// Renormalised form: scale dL/dX_t and dL/dV_t by 2^-e so their largest magnitude
+// lies in [0.5, 1), and add e to the exponent the caller keeps.  e is the frexp
+// exponent of the step's maximum (k_adj_predict), limited to [-126, 126] so 2^-e is a
+// normal float and every product is exact; a zero or non-finite maximum leaves the
+// buffers as they are.
+__global__ __launch_bounds__(kBS) void k_adj_renorm(float4* __restrict__ gX, float4* __restrict__ gV, uint32_t N,
+                                                    const uint32_t* __restrict__ gmax,
+                                                    int32_t* __restrict__ log2Scale) {
+    const uint32_t mb = *gmax;
+    if (mb == 0u || mb >= 0x7F800000u) return;
+    const int e = min(126, max(-126, (int)(mb >> 23) - 126));
+    if (e == 0) return;
+    const uint32_t i = blockIdx.x * kBS + threadIdx.x;
+    if (i == 0) *log2Scale += e;
+    if (i >= N) return;
+    const float sc = __uint_as_float((uint32_t)(127 - e) << 23);
+    gX[i] = mk4(mul3(f3(gX[i]), sc), 0.f);
+    gV[i] = mk4(mul3(f3(gV[i]), sc), 0.f);
 }
 
 }  // namespace
 
 void adjoint_step(const StepTape& t, const SpanTable& stIn, const sim::SimParams& p, const Scenes& sc,
                   const float4* gXin, const float4* gVin, float4* gXout, float4* gVout, double* gcAccum,
-                  AdjointScratch& w, hipStream_t s) {
+                  AdjointScratch& w, hipStream_t s, int32_t* log2Scale) {
     const uint32_t N = p.numParticles;
     if (N == 0) return;
     const int I = p.solverIters;
@@ -417,7 +466,12 @@ void adjoint_step(const StepTape& t, const SpanTable& stIn, const sim::SimParams
     SpanTable st = stIn;
     st.spans = t.spans;
     const float4* Pf = t.P[I];
-    k_adj_gather<<<B, kBS, 0, s>>>(t.idxS, gXin, gVin, t.xsphGm, w.gPf, w.gVo, N);
+    if (log2Scale && !w.gmax) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "adjoint_step: renormalised form without its scratch");
+        return;
+    }
+    uint32_t* gmax = log2Scale ? w.gmax : nullptr;
+    k_adj_gather<<<B, kBS, 0, s>>>(t.idxS, gXin, gVin, t.xsphGm, w.gPf, w.gVo, N, gmax);
     const bool ens = sc.params != nullptr;
     if (ens && (!t.sceneOfSlot || !w.gcSlot)) {
         set_error(PBFX_E_INVALID_ARGUMENT, "adjoint_step: ensemble tape without scene slots");
@@ -426,9 +480,9 @@ void adjoint_step(const StepTape& t, const SpanTable& stIn, const sim::SimParams
     k_adj_xsph<<<B, kBS, 0, s>>>(Pf, t.Vs, t.segOf, st, dp, sc, t.sceneOfSlot, N, w.gVo, w.gVn, w.gPf,
                                  w.partial, ens ? w.gcSlot : nullptr);
     if (ens)
-        k_adj_scene_sum<<<sc.count, kBS, 0, s>>>(w.gcSlot, sc, gcAccum);
+        k_adj_scene_sum<<<sc.count, kBS, 0, s>>>(w.gcSlot, sc, gcAccum, log2Scale);
     else
-        k_adj_sum<<<1, 1, 0, s>>>(w.partial, B, gcAccum);
+        k_adj_sum<<<1, 1, 0, s>>>(w.partial, B, gcAccum, log2Scale);
     k_adj_velocity<<<B, kBS, 0, s>>>(t.idxS, t.finalClamp, w.gPf, w.gVn, t.dispU[I - 1], dp, 1.0f / p.dt, N, w.gP,
                                      w.a, gXout);
     for (int it = I - 1; it >= 0; --it) {
@@ -437,7 +491,8 @@ void adjoint_step(const StepTape& t, const SpanTable& stIn, const sim::SimParams
         k_adj_lambda<<<B, kBS, 0, s>>>(Pit, t.segOf, st, dp, N, w.lamQ, w.lamE, it > 0 ? t.dispU[it - 1] : nullptr,
                                        w.gP, w.a);
     }
-    k_adj_predict<<<B, kBS, 0, s>>>(t.idxS, t.predClamp, w.gP, p.dt, N, gXout, gVout);
+    k_adj_predict<<<B, kBS, 0, s>>>(t.idxS, t.predClamp, w.gP, p.dt, N, gXout, gVout, gmax);
+    if (log2Scale) k_adj_renorm<<<B, kBS, 0, s>>>(gXout, gVout, N, gmax, log2Scale);
     check_launch("adjoint_step");
 }
 

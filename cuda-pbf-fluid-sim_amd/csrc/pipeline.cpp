@@ -853,7 +853,7 @@ void Simulator::releaseAdjoint() {
     }
     m_tapes.clear();
     dfree(m_adj.gPf); dfree(m_adj.gVo); dfree(m_adj.gVn); dfree(m_adj.gP); dfree(m_adj.a);
-    dfree(m_adj.lamQ); dfree(m_adj.lamE); dfree(m_adj.partial); dfree(m_adj.gcSlot);
+    dfree(m_adj.lamQ); dfree(m_adj.lamE); dfree(m_adj.partial); dfree(m_adj.gcSlot); dfree(m_adj.gmax);
     m_adj = pbfx::AdjointScratch{};
 }
 
@@ -883,12 +883,13 @@ bool Simulator::adjointInit(int slots) {
     m_adj.blocks = pbfx::ceil_div(cap, 256);
     ok = ok && dalloc(m_adj.gPf, cap) && dalloc(m_adj.gVo, cap) && dalloc(m_adj.gVn, cap) && dalloc(m_adj.gP, cap) &&
          dalloc(m_adj.a, cap) && dalloc(m_adj.lamQ, cap) && dalloc(m_adj.lamE, cap) &&
-         dalloc(m_adj.partial, m_adj.blocks);
+         dalloc(m_adj.partial, m_adj.blocks) && dalloc(m_adj.gmax, 1);
     if (!m_scenes.empty()) ok = ok && dalloc(m_adj.gcSlot, cap);
     if (!ok) {
         releaseAdjoint();
         return false;
     }
+    (void)hipMemset(m_adj.gmax, 0, sizeof(uint32_t));
     return true;
 }
 
@@ -898,6 +899,8 @@ bool Simulator::stepRecord(int slot) {
         return false;
     }
     pbfx::StepTape& t = m_tapes[slot];
+    t.params = m_params;
+    t.recorded = true;
     const uint32_t N = m_params.numParticles;
     const size_t b4 = sizeof(float4) * N;
     bool ok = hipMemcpyAsync(t.X, m_bufs.d_pos, b4, hipMemcpyDeviceToDevice, m_stream) == hipSuccess &&
@@ -924,17 +927,20 @@ bool Simulator::stepRecord(int slot) {
 }
 
 bool Simulator::backwardStep(int slot, const float4* gXin, const float4* gVin, float4* gXout, float4* gVout,
-                             double* gcAccum) {
-    if (!m_initialized || slot < 0 || slot >= (int)m_tapes.size()) {
-        set_error(PBFX_E_INVALID_ARGUMENT, "backwardStep: no such tape slot");
+                             double* gcAccum, int32_t* log2Scale) {
+    if (!m_initialized || slot < 0 || slot >= (int)m_tapes.size() || !m_tapes[slot].recorded) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "backwardStep: no step recorded in that tape slot");
         return false;
     }
     if (gXin == gXout || gVin == gVout) {
         set_error(PBFX_E_ALIAS, "backwardStep: outputs alias inputs");
         return false;
     }
-    pbfx::adjoint_step(m_tapes[slot], m_grid.spans, m_params, m_grid.scenes, gXin, gVin, gXout, gVout, gcAccum,
-                       m_adj, m_stream);
+    // the tape's own parameters: a parameter change after recording does not alter
+    // which step is differentiated (the structural fields cannot change at all)
+    const pbfx::StepTape& t = m_tapes[slot];
+    pbfx::adjoint_step(t, m_grid.spans, t.params, m_grid.scenes, gXin, gVin, gXout, gVout, gcAccum, m_adj, m_stream,
+                       log2Scale);
     return true;
 }
 

@@ -246,10 +246,11 @@ public:
     bool adjointInit(int slots);
     // One eager forward step recorded into tape `slot`.
     bool stepRecord(int slot);
-    // Reverse of the step in tape `slot` (device pointers, particle order); adds
-    // dL/dc into gcAccum (device double).
+    // Reverse of the step in tape `slot` (device pointers, particle order), with the
+    // parameters the step was recorded with; adds dL/dc into gcAccum (device double).
+    // log2Scale (device int, or nullptr): the renormalised form of adjoint_step.
     bool backwardStep(int slot, const float4* gXin, const float4* gVin, float4* gXout, float4* gVout,
-                      double* gcAccum);
+                      double* gcAccum, int32_t* log2Scale = nullptr);
 
 private:
     bool captureGraphs();

@@ -64,6 +64,7 @@ SIGNATURES = {
     "pbfSimAdjointInit": (_I, [_P, _I]),
     "pbfSimStepRecord": (_I, [_P, _I]),
     "pbfSimBackwardStep": (_I, [_P, _I, _P, _P, _P, _P, _P]),
+    "pbfSimBackwardStepScaled": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
     "pbfEnsembleCreate": (_P, [ctypes.POINTER(SimParams), _P, _U32, _I, _I, _I]),
     "pbfEnsembleSeedLattice": (_I, [_P, _U32, _U32, _U32, _U32, _F, _F, _F, _F, _F, ctypes.c_uint64]),
     "pbfEnsembleGetScene": (_I, [_P, _U32, _P, _P]),

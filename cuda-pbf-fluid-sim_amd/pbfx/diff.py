@@ -90,12 +90,25 @@ class Rollout:
             raise _lib.PbfError("get state failed")
         return X, V
 
+    def _params_now(self):
+        out = type(self.params)()
+        if _lib.lib().pbfSimGetParams(self.sim._h, ctypes.byref(out)) != 0:
+            raise _lib.PbfError("get params failed")
+        return out
+
+    def _apply_params(self, sp):
+        if _lib.lib().pbfSimSetParams(self.sim._h, ctypes.byref(sp)) != 0:
+            _lib.check("pbfSimSetParams")
+            raise _lib.PbfError("set params failed")
+
     def forward(self, x0, v0, c):
         """Run the rollout; returns (X_T, V_T) as (N, 4) float32 CUDA tensors."""
         self._set_c(c)
         X, V = _as4(x0), _as4(v0)
         self._set_state(X, V)
         self._ckpts = []
+        # backward() re-records the segments with these parameters, whatever changes later
+        self._fwd_params = self._params_now()
         L = _lib.lib()
         done = 0
         while done < self.steps:
@@ -108,8 +121,16 @@ class Rollout:
         return self._get_state()
 
     def backward(self, gX, gV=None):
-        """dL/dX_0, dL/dV_0 (N, 4 float32) and dL/dc from dL/dX_T and dL/dV_T (a float, or
-        one float per scene for an ensemble rollout)."""
+        """dL/dX_0, dL/dV_0 ((N, 4) float64 CUDA tensors) and dL/dc from dL/dX_T and dL/dV_T
+        (a float, or one float per scene for an ensemble rollout).
+
+        The reverse pass is renormalised every step (pbfSimBackwardStepScaled): its float32
+        buffers hold the gradients times a power of two whose exponent stays on the device,
+        so a long rollout whose gradients outgrow float32 (the dam break amplifies them
+        about 8x a step) still ends finite.  The result is that exponent applied in float64;
+        ``log2_scale`` and ``scaled_grads`` keep the float32 mantissas and the exponent."""
+        if not self._ckpts:
+            raise _lib.PbfError("backward() before forward()")
         L = _lib.lib()
         gX = _as4(gX).clone()
         gV = torch.zeros_like(gX) if gV is None else _as4(gV).clone()
@@ -117,33 +138,58 @@ class Rollout:
         gV[:, 3] = 0
         oX, oV = torch.empty_like(gX), torch.empty_like(gV)
         gc = torch.zeros(self.scenes, dtype=torch.float64, device="cuda")
+        E = torch.zeros(1, dtype=torch.int32, device="cuda")
+        now = self._params_now()
+        changed = bytes(now) != bytes(self._fwd_params)
+        if changed:
+            self._apply_params(self._fwd_params)
         starts = list(range(0, self.steps, self.ck))
         t_rec = t_rev = 0.0
-        for seg in reversed(range(len(starts))):
-            k = min(self.ck, self.steps - starts[seg])
-            self._set_state(*self._ckpts[seg])
-            t0 = self._clock()
-            for t in range(k):
-                if L.pbfSimStepRecord(self.sim._h, t) != 0:
-                    _lib.check("pbfSimStepRecord")
-                    raise _lib.PbfError("record failed")
-            t1 = self._clock()
-            for t in reversed(range(k)):
-                if L.pbfSimBackwardStep(self.sim._h, t, _p(gX), _p(gV), _p(oX), _p(oV), _p(gc)) != 0:
-                    _lib.check("pbfSimBackwardStep")
-                    raise _lib.PbfError("backward failed")
-                gX, oX = oX, gX
-                gV, oV = oV, gV
-            t2 = self._clock()
-            t_rec += t1 - t0
-            t_rev += t2 - t1
+        try:
+            for seg in reversed(range(len(starts))):
+                k = min(self.ck, self.steps - starts[seg])
+                self._set_state(*self._ckpts[seg])
+                t0 = self._clock()
+                for t in range(k):
+                    if L.pbfSimStepRecord(self.sim._h, t) != 0:
+                        _lib.check("pbfSimStepRecord")
+                        raise _lib.PbfError("record failed")
+                t1 = self._clock()
+                for t in reversed(range(k)):
+                    if L.pbfSimBackwardStepScaled(self.sim._h, t, _p(gX), _p(gV), _p(oX), _p(oV), _p(gc),
+                                                  _p(E)) != 0:
+                        _lib.check("pbfSimBackwardStepScaled")
+                        raise _lib.PbfError("backward failed")
+                    gX, oX = oX, gX
+                    gV, oV = oV, gV
+                t2 = self._clock()
+                t_rec += t1 - t0
+                t_rev += t2 - t1
+        finally:
+            if changed:
+                self._apply_params(now)
         if self.timed:
             self.timing = {"record_s": t_rec, "reverse_s": t_rev}
         self.sim.synchronize()
         _lib.check("Rollout.backward")
+        self.log2_scale = int(E.item())
+        self.scaled_grads = (gX, gV)
+        gX64, gV64 = _ldexp64(gX, self.log2_scale), _ldexp64(gV, self.log2_scale)
         if self.scenes == 1:
-            return gX, gV, float(gc.item())
-        return gX, gV, gc.cpu().numpy()
+            return gX64, gV64, float(gc.item())
+        return gX64, gV64, gc.cpu().numpy()
+
+
+def _ldexp64(t, e):
+    """t * 2^e in float64 (exact; overflows to +-inf only past the float64 range)."""
+    out = t.double()
+    while e > 1000:
+        out = out * 2.0 ** 1000
+        e -= 1000
+    while e < -1000:
+        out = out * 2.0 ** -1000
+        e += 1000
+    return out * 2.0 ** e
 
 
 class EnsembleRollout(Rollout):
@@ -174,7 +220,8 @@ class PBFRollout(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gXT):
         gX0, gV0, gc = ctx.roll.backward(gXT.contiguous())
-        return gV0[:, :3].clone(), torch.tensor(gc, dtype=torch.float32, device=gXT.device), gX0[:, :3].clone(), None
+        return (gV0[:, :3].to(gXT.dtype), torch.tensor(gc, dtype=gXT.dtype, device=gXT.device),
+                gX0[:, :3].to(gXT.dtype), None)
 
 
 def occupancy_grid(pos, box_min, box_max, cells=32, sigma=0.0125, reach=3.0):

@@ -146,6 +146,7 @@ class Ensemble:
     set_state = Simulator.set_state
     tuning = Simulator.tuning
     set_tuning = Simulator.set_tuning
+    hash_overflow = Simulator.hash_overflow
 
 
 def solo(scene, grid="hashed", use_graph=True):

@@ -176,6 +176,29 @@ class Simulator:
             raise _lib.PbfError("get_state failed")
         return pos, vel
 
+    def state_device(self):
+        """Device copies of the current state: two (N, 4) float32 CUDA tensors."""
+        import torch
+        pos = torch.empty((self.n, 4), dtype=torch.float32, device="cuda")
+        vel = torch.empty_like(pos)
+        torch.cuda.synchronize()
+        if _lib.lib().pbfSimGetState(self._h, ctypes.c_void_p(pos.data_ptr()), ctypes.c_void_p(vel.data_ptr()),
+                                     1) != 0:
+            _lib.check("pbfSimGetState")
+            raise _lib.PbfError("get_state failed")
+        return pos, vel
+
+    def set_state_device(self, pos, vel):
+        """Replace the state from device tensors ((N, 4) float32, contiguous, CUDA)."""
+        import torch
+        for t in (pos, vel):
+            assert t.is_cuda and t.dtype == torch.float32 and t.is_contiguous() and tuple(t.shape) == (self.n, 4)
+        torch.cuda.synchronize()
+        if _lib.lib().pbfSimSetState(self._h, ctypes.c_void_p(pos.data_ptr()), ctypes.c_void_p(vel.data_ptr()),
+                                     self.n, 1) != 0:
+            _lib.check("pbfSimSetState")
+            raise _lib.PbfError("set_state failed")
+
     def step(self, steps=1):
         if _lib.lib().pbfSimStep(self._h, int(steps)) != 0:
             _lib.check("pbfSimStep")

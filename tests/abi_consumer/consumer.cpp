@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <vector>
 
 #include "pbf_hip.h"
@@ -19,6 +20,79 @@
             return 1;                                                    \
         }                                                                \
     } while (0)
+
+// One whole step through the drop-in launchers on caller-owned buffers, in the
+// reference's phase order: PhaseIntegrate (phases.cpp:33-37), PhaseGridBuildFull
+// (:40-63), PhaseSolveIterations (:70-95), PhaseVelocity (:100-105), then XsphOp
+// (post_ops.cpp:54-71, over a velocity snapshot) and BoundaryOp (:25-29).  X and V
+// advance in place; returns false on a launcher error.
+static bool dropin_step(const sim::SimParams& p, bool compact, std::vector<float4>& X, std::vector<float4>& V,
+                        hipStream_t st) {
+    const uint32_t N = p.numParticles;
+    const uint32_t cells = (uint32_t)(p.grid.dim.x * p.grid.dim.y * p.grid.dim.z);
+    const sim::DeviceParams dp = sim::MakeDeviceParams(p);
+    float4 *pos = nullptr, *pred = nullptr, *vel = nullptr, *velSnap = nullptr, *delta = nullptr;
+    float* lambda = nullptr;
+    uint32_t *keys = nullptr, *idx = nullptr, *keysS = nullptr, *idxS = nullptr, *a = nullptr, *b = nullptr,
+             *count = nullptr;
+    bool ok = hipMalloc(&pos, 16 * N) == hipSuccess && hipMalloc(&pred, 16 * N) == hipSuccess &&
+              hipMalloc(&vel, 16 * N) == hipSuccess && hipMalloc(&velSnap, 16 * N) == hipSuccess &&
+              hipMalloc(&delta, 16 * N) == hipSuccess && hipMalloc(&lambda, 4 * N) == hipSuccess &&
+              hipMalloc(&keys, 4 * N) == hipSuccess && hipMalloc(&idx, 4 * N) == hipSuccess &&
+              hipMalloc(&keysS, 4 * N) == hipSuccess && hipMalloc(&idxS, 4 * N) == hipSuccess &&
+              hipMalloc(&count, 4) == hipSuccess;
+    // dense: cellStart/cellEnd over every cell; compact: unique keys and N + 1 offsets
+    ok = ok && hipMalloc(&a, 4 * (size_t)(compact ? N : cells)) == hipSuccess &&
+         hipMalloc(&b, 4 * (size_t)(compact ? N + 1 : cells)) == hipSuccess;
+    size_t tempBytes = 0;
+    void* temp = nullptr;
+    if (ok) LaunchSortPairsQuery(&tempBytes, keys, keysS, idx, idxS, N, st);
+    ok = ok && hipMalloc(&temp, tempBytes ? tempBytes : 1) == hipSuccess;
+    if (compact) ok = ok && EnsureCellCompactScratch(N, 256);
+    ok = ok && hipMemcpy(pos, X.data(), 16 * N, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(vel, V.data(), 16 * N, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) {
+        LaunchIntegratePred(pos, vel, pred, p.gravity, p.dt, N, st);
+        LaunchBoundary(pred, vel, p.grid, 0.0f, N, st);
+        LaunchHashKeys(keys, idx, pred, p.grid, N, st);
+        LaunchSortPairs(temp, tempBytes, keys, keysS, idx, idxS, N, st);
+        if (compact)
+            LaunchCellRangesCompact(a, b, count, keysS, N, st);
+        else
+            LaunchCellRanges(a, b, keysS, N, cells, st);
+        for (int it = 0; it < p.solverIters; ++it) {
+            if (compact) {
+                LaunchLambdaCompact(lambda, pred, idxS, keysS, a, b, count, dp, N, st);
+                LaunchDeltaApplyCompact(pred, delta, lambda, idxS, keysS, a, b, count, dp, N, st);
+            } else {
+                LaunchLambda(lambda, pred, idxS, keysS, a, b, dp, N, st);
+                LaunchDeltaApply(pred, delta, lambda, idxS, keysS, a, b, dp, N, st);
+            }
+        }
+        LaunchBoundary(pred, vel, p.grid, 0.0f, N, st);
+        LaunchVelocity(vel, pos, pred, 1.0f / p.dt, N, st);
+        if (p.xsph_c > 0.f) {
+            ok = hipMemcpyAsync(velSnap, vel, 16 * N, hipMemcpyDeviceToDevice, st) == hipSuccess;
+            if (compact)
+                LaunchXSPHCompact(vel, velSnap, pred, idxS, keysS, a, b, count, dp, N, st);
+            else
+                LaunchXSPH(vel, velSnap, pred, idxS, keysS, a, b, dp, N, st);
+        }
+        LaunchBoundary(pred, vel, p.grid, p.boundaryRestitution, N, st);
+        // the step's result is the predicted buffer (the ping-pong swap)
+        ok = ok && hipStreamSynchronize(st) == hipSuccess && pbfGetLastError() == 0 &&
+             hipMemcpy(X.data(), pred, 16 * N, hipMemcpyDeviceToHost) == hipSuccess &&
+             hipMemcpy(V.data(), vel, 16 * N, hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    hipFree(pos); hipFree(pred); hipFree(vel); hipFree(velSnap); hipFree(delta); hipFree(lambda);
+    hipFree(keys); hipFree(idx); hipFree(keysS); hipFree(idxS); hipFree(a); hipFree(b); hipFree(count);
+    hipFree(temp);
+    return ok;
+}
+
+static bool same_bits(const std::vector<float4>& a, const std::vector<float4>& b) {
+    return a.size() == b.size() && std::memcmp(a.data(), b.data(), a.size() * sizeof(float4)) == 0;
+}
 
 int main() {
     // the 16^3 dam-break block in a 1 m box, parameters derived as the reference app does
@@ -85,7 +159,27 @@ int main() {
 
     hipFree(temp); hipFree(dPos); hipFree(keys); hipFree(idx); hipFree(keysS); hipFree(idxS);
     hipFree(start); hipFree(end);
+
+    // three more steps from the simulator's state through the neighbour launchers
+    // (dense and compact forms), against the simulator's reference path stepping from
+    // the same state: bit for bit
+    for (int compact = 0; compact < 2; ++compact) {
+        std::vector<float4> Xl = X, Vl = V;
+        hipStream_t ls = nullptr;
+        CHECK(hipStreamCreateWithFlags(&ls, hipStreamNonBlocking) == hipSuccess, "stream");
+        for (int k = 0; k < 3; ++k) CHECK(dropin_step(p, compact != 0, Xl, Vl, ls), "drop-in step");
+        (void)hipStreamDestroy(ls);
+        PbfSim* r = pbfSimCreate(&p, compact ? PBF_GRID_COMPACT : PBF_GRID_DENSE, 16, PBF_PATH_REFERENCE, 0);
+        CHECK(r != nullptr, "pbfSimCreate (reference path)");
+        CHECK(pbfSimSetState(r, X.data(), V.data(), N, 0) == 0, "pbfSimSetState");
+        CHECK(pbfSimStep(r, 3) == 0, "pbfSimStep (reference path)");
+        std::vector<float4> Xr(N), Vr(N);
+        CHECK(pbfSimGetState(r, Xr.data(), Vr.data(), 0) == 0, "pbfSimGetState (reference path)");
+        pbfSimDestroy(r);
+        CHECK(same_bits(Xl, Xr) && same_bits(Vl, Vr),
+              compact ? "compact neighbour launchers equal the simulator" : "dense neighbour launchers equal the simulator");
+    }
     pbfSimDestroy(s);
-    std::printf("OK %u particles, 10 steps, launchers checked\n", N);
+    std::printf("OK %u particles, 10 steps, launchers checked, 3 drop-in steps (dense, compact) bit-identical\n", N);
     return 0;
 }

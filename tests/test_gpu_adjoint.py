@@ -100,13 +100,15 @@ def test_rollout_gradients_match_fp64_reference(oracle, dev, variant, steps, ck)
     assert ev <= REL_TOL and ex <= REL_TOL and ec <= REL_TOL
 
 
-@pytest.mark.parametrize("variant", sorted(VARIANTS))
-def test_chained_step_gradients_along_gpu_trajectory(oracle, dev, variant):
-    """8 steps: the GPU reverse pass against float64 vector-Jacobian products of the
-    reference step evaluated at the GPU's own states X_t, V_t."""
+@pytest.mark.parametrize("variant,steps", [(v, 8) for v in sorted(VARIANTS)] + [("default", 60), ("xsph_gate", 60)])
+def test_chained_step_gradients_along_gpu_trajectory(oracle, dev, variant, steps):
+    """8 and 60 steps: the GPU reverse pass (renormalised every step) against float64
+    vector-Jacobian products of the reference step evaluated at the GPU's own states
+    X_t, V_t.  Over 60 steps the gradient grows by many orders of magnitude; the
+    renormalised float32 pass must still agree with the float64 chain."""
     from torch_pbf import pbf_step
     scene, sp, X, V = block_state(oracle, seed=21, **VARIANTS[variant])
-    n, steps = len(X), 8
+    n = len(X)
     rng = np.random.default_rng(8)
     wX = rng.normal(0, 1, (n, 3))
     wV = 0.01 * rng.normal(0, 1, (n, 3))
@@ -133,7 +135,8 @@ def test_chained_step_gradients_along_gpu_trajectory(oracle, dev, variant):
     ev = rel_err(gV0[:, :3].cpu().numpy(), gv.numpy())
     ex = rel_err(gX0[:, :3].cpu().numpy(), gx.numpy())
     ec = gc_err(gc, (float(gcv.sum()), float(gcv.abs().sum())))
-    print(f"chained {variant} {steps} steps: |gV0| {float(gv.norm()):.4g}; rel err gV0 {ev:.3g} gX0 {ex:.3g} gc {ec:.3g}")
+    print(f"chained {variant} {steps} steps: |gV0| {float(gv.norm()):.4g} (2^{roll.log2_scale} scale); "
+          f"rel err gV0 {ev:.3g} gX0 {ex:.3g} gc {ec:.3g}")
     assert ev <= REL_TOL and ex <= REL_TOL and ec <= REL_TOL
 
 
@@ -279,3 +282,76 @@ def test_ensemble_rollout_matches_single_scenes(oracle, dev):
         print(f"scene {k}: gc ensemble {egc[k]:.9g} single {sgc:.9g}", flush=True)
         assert np.array_equal(eX[a:b], sX.cpu().numpy()) and np.array_equal(eV[a:b], sV.cpu().numpy())
         assert abs(egc[k] - sgc) <= 1e-9 * max(1.0, abs(sgc))
+
+
+def _configured_rollout(ck, dev):
+    """The differentiable-rollout configuration (65,536 particles, 64x32x32 lattice, 60
+    steps, dt = 1/120, 2 iterations): the target occupancy grid comes from a forward run
+    with a seeded perturbed initial velocity; returns (loss, gX0, gV0, gc, log2 scale)."""
+    from pbfx.diff import initial_state
+    scene = differentiable_scene()
+    sp = P.build_sim_params(scene)
+    n, h = sp.numParticles, float(sp.kernel.h)
+    X0, _ = initial_state(scene)
+    x0 = torch.from_numpy(X0).to(dev)
+    roll = Rollout(sp, 60, checkpoint=ck)
+    try:
+        XT, _ = roll.forward(x0, torch.from_numpy(seeded_velocity(n, 77, 0.5)).to(dev), sp.xsph_c)
+        target = occupancy_grid(XT[:, :3].double(), scene.box_min, scene.box_max, cells=32, sigma=h)
+        XT, _ = roll.forward(x0, torch.zeros((n, 4), dtype=torch.float32, device=dev), sp.xsph_c)
+        xT = XT[:, :3].double().requires_grad_(True)
+        loss = ((occupancy_grid(xT, scene.box_min, scene.box_max, cells=32, sigma=h) - target) ** 2).sum()
+        loss.backward()
+        gX0, gV0, gc = roll.backward(xT.grad.float())
+        return float(loss), gX0.cpu().numpy(), gV0.cpu().numpy(), gc, roll.log2_scale
+    finally:
+        roll.close()
+
+
+@pytest.mark.timeout(300)
+def test_configured_rollout_gradient_finite_and_checkpoint_invariant(dev):
+    """The configured 65,536 x 60 rollout with the occupancy loss: the gradient w.r.t. v0
+    and c is finite and non-zero (the reverse pass is renormalised, so it does not
+    overflow float32 although it outgrows it), and identical for checkpoint intervals
+    10 and 20 (each reverse segment is re-recorded from its checkpoint)."""
+    outs = {}
+    for ck in (10, 20):
+        print(f"65,536 x 60 rollout, checkpoint {ck}", flush=True)
+        outs[ck] = _configured_rollout(ck, dev)
+    loss, gX0, gV0, gc, e = outs[10]
+    gnorm = float(np.linalg.norm(gV0[:, :3]))
+    print(f"loss {loss:.6g}; |dL/dv0| {gnorm:.4g} (renormalised by 2^{e}); dL/dc {gc:.4g}", flush=True)
+    assert np.isfinite(loss) and loss > 0
+    assert np.all(np.isfinite(gX0)) and np.all(np.isfinite(gV0)) and np.isfinite(gc)
+    assert gnorm > 0 and gc != 0
+    for a, b in zip(outs[10][1:], outs[20][1:]):
+        assert np.array_equal(a, b)
+
+
+def test_parameter_change_after_forward_keeps_the_recorded_step(oracle, dev):
+    """A parameter change between forward() and backward() does not change which
+    trajectory is differentiated: each tape keeps the parameters it was recorded with
+    and the rollout re-records with the forward's, so the gradients equal those of an
+    unchanged rollout bit for bit, and the simulator keeps the new parameters."""
+    scene, sp, X, V = block_state(oracle, n=8, seed=4)
+    n = len(X)
+    g = torch.zeros((n, 4), dtype=torch.float32, device=dev)
+    g[:, :3] = torch.from_numpy(np.random.default_rng(4).normal(0, 1, (n, 3))).float().to(dev)
+    outs = []
+    for change in (False, True):
+        roll = Rollout(sp, 4, checkpoint=2)
+        try:
+            roll.forward(torch.from_numpy(X).to(dev), torch.from_numpy(V).to(dev), sp.xsph_c)
+            if change:
+                sp2 = P.build_sim_params(scene)
+                sp2.dt = sp2.dt * 0.5
+                sp2.pbf.scorr_k = 0.003
+                roll.sim.set_params(sp2)
+            gX0, gV0, gc = roll.backward(g)
+            outs.append((gX0.cpu().numpy(), gV0.cpu().numpy(), gc))
+            if change:
+                assert roll._params_now().dt == sp2.dt
+        finally:
+            roll.close()
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    assert outs[0][2] == outs[1][2]

@@ -2,7 +2,8 @@
 
 Seeding is checked bit for bit against the oracle's numpy restatement. External
 position buffers must not change a trajectory: a simulator stepping in caller-owned
-buffers gives the same bits as one using its own."""
+buffers (reference: bindExternalPosPingPong, simulator.cpp:676-757) gives the oracle's
+states bit for bit, and the same bits as one using its own."""
 import numpy as np
 import pytest
 import torch
@@ -56,10 +57,12 @@ def test_cube_mix_runs_and_rejects_overflow(dev):
 
 
 @pytest.mark.parametrize("use_graph", [False, True])
-def test_external_positions_bit_identical(oracle, dev, use_graph):
-    scene = P.small_block((10, 10, 10))
-    a = Simulator.from_scene(scene, use_graph=use_graph)
-    b = Simulator.from_scene(scene, use_graph=use_graph)
+@pytest.mark.parametrize("path,grid", [("reordered", "dense"), ("reordered", "hashed"), ("reference", "compact")])
+def test_external_positions_bit_identical(oracle, dev, use_graph, path, grid):
+    scene = P.small_block((10, 10, 10), table_log2=16)
+    sp = P.build_sim_params(scene)
+    a = Simulator.from_scene(scene, use_graph=use_graph, path=path, grid=grid)
+    b = Simulator.from_scene(scene, use_graph=use_graph, path=path, grid=grid)
     try:
         a.step(3)
         b.step(3)
@@ -71,6 +74,13 @@ def test_external_positions_bit_identical(oracle, dev, use_graph):
         b.step(4)
         xa, va = a.state()
         xb, vb = b.state()
+        # the oracle's 7 steps of the same scene, with the binding between steps 3 and 4
+        X0 = P.seed_scene_reference(scene, oracle.seed_lattice)
+        compact = grid != "dense"
+        eX, eV = oracle.run(X0, np.zeros_like(X0), sp, 3, compact=compact)
+        eX, eV = oracle.run(eX, eV, sp, 4, compact=compact)
+        np.testing.assert_array_equal(xb, eX)
+        np.testing.assert_array_equal(vb, eV)
         assert np.array_equal(xa, xb) and np.array_equal(va, vb)
         # the current state lives in one of the caller's buffers, viewed without a copy
         view = b.positions_view()

@@ -3,9 +3,11 @@
 One step: positions within 1e-5*h and velocities within 1e-5*h/dt of the oracle.
 Many steps: the dam break is chaotic (SI-unit PBF with a saturating displacement
 clamp), so after 200 steps the density-constraint statistics must agree within 1%.
-At 1M particles the default run checks the three grid kinds against each other (they
-must give identical bits) and against size-independent invariants; with PBF_LONG_PARITY=1
-a 200-step run is also compared with the oracle bit for bit (minutes of oracle time).
+At full size the default run compares the 1M dam break (3 steps, hashed grid with the
+candidate mask, the benchmark's setting) and the 8M splash (1 step, 2^24-slot table) with
+the oracle bit for bit, checks the three grid kinds against each other and against
+size-independent invariants; with PBF_LONG_PARITY=1 a 200-step 1M run is also compared
+with the oracle bit for bit (minutes of oracle time).
 """
 import os
 
@@ -176,6 +178,51 @@ def test_1m_grid_kinds_identical_and_invariants(dev):
     hi = np.array(list(sp.grid.maxs), np.float32)
     assert np.all(np.isfinite(X)) and np.all(X[:, :3] >= lo) and np.all(X[:, :3] <= hi)
     assert np.all(X[:, 3] == 1.0) and np.all(V[:, 3] == 0.0)
+
+
+@pytest.mark.timeout(300)
+def test_1m_three_steps_bit_identical_to_oracle(oracle, dev):
+    """dam_break_1m at full size, 3 steps on the hashed grid with the candidate mask (the
+    benchmark's setting) against the oracle's step (reference: phases.cpp:108-117), bit
+    for bit after every step."""
+    scene = P.dam_break_1m()
+    sp = P.build_sim_params(scene)
+    sim = _sim(scene, "hashed")
+    try:
+        sim.set_tuning(candidate_mask=True, stream_ratio=1, chunk_bits=False)
+        X = P.seed_scene_reference(scene, oracle.seed_lattice)
+        V = np.zeros_like(X)
+        for t in range(1, 4):
+            sim.step(1)
+            X, V = oracle.run(X, V, sp, 1, compact=True)
+            gX, gV = sim.state()
+            print(f"1M step {t}: bit-identical {np.array_equal(gX, X) and np.array_equal(gV, V)}", flush=True)
+            np.testing.assert_array_equal(gX, X)
+            np.testing.assert_array_equal(gV, V)
+        assert not sim.hash_overflow()
+    finally:
+        sim.close()
+
+
+@pytest.mark.timeout(600)
+def test_8m_one_step_bit_identical_to_oracle(oracle, dev):
+    """splash_8m at full size (8,000,000 particles, 4 solver iterations, the 2^24-slot table
+    smaller than its 32M-cell key space, so block-hashed homes): one step equals the
+    oracle's, bit for bit."""
+    scene = P.splash_8m()
+    sp = P.build_sim_params(scene)
+    sim = _sim(scene, "hashed")
+    try:
+        sim.step(1)
+        gX, gV = sim.state()
+        assert not sim.hash_overflow()
+    finally:
+        sim.close()
+    X = P.seed_scene_reference(scene, oracle.seed_lattice)
+    eX, eV = oracle.run(X, np.zeros_like(X), sp, 1, compact=True)
+    print(f"8M step 1: bit-identical {np.array_equal(gX, eX) and np.array_equal(gV, eV)}", flush=True)
+    np.testing.assert_array_equal(gX, eX)
+    np.testing.assert_array_equal(gV, eV)
 
 
 @pytest.mark.timeout(1200)

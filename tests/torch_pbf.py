@@ -52,7 +52,11 @@ def pbf_step(X, V, sp, c, box_min, box_max):
         d = P[:, None, :] - P[None, :, :]
         r2 = (d * d).sum(-1)
         nb = cand & (r2 <= h2)
-        r = torch.sqrt(torch.where(nb, r2, torch.ones_like(r2)))
+        # coincident neighbours (two particles clamped into the same box corner) have
+        # r = 0 and a zero gradient vector; the square root is taken of a safe value so
+        # its derivative at 0 (infinite) never meets a zero in the backward pass
+        pos = nb & (r2 > 0)
+        r = torch.sqrt(torch.where(pos, r2, torch.ones_like(r2))) * pos.to(r2.dtype)
         return d, r2, r, nb
 
     dq = pbf.scorr_dq_h * h
@@ -92,8 +96,12 @@ def pbf_step(X, V, sp, c, box_min, box_max):
         if pbf.enable_relax:
             D = D * pbf.relax_omega
         if pbf.enable_disp_clamp:
-            ln = torch.sqrt((D * D).sum(1, keepdim=True))
-            D = torch.where(ln > maxDisp, D * (maxDisp / ln), D)
+            l2 = (D * D).sum(1, keepdim=True)
+            over = l2 > maxDisp * maxDisp
+            # the unclamped branch's backward must not see maxDisp / 0 (an isolated
+            # particle has D = 0): the length is taken of a safe value
+            ln = torch.sqrt(torch.where(over, l2, torch.ones_like(l2)))
+            D = torch.where(over, D * (maxDisp / ln), D)
         P = P + D
     P = torch.minimum(torch.maximum(P, lo), hi)
     Vn = (P - X) / dt
