@@ -194,6 +194,39 @@ CompactScratch* scratch_for_current_device() {
     return &g_scratch[dev];
 }
 
+// Allocation-free compaction: the tile counts and tile offsets live in this file-scope
+// device array instead of memory the library allocates.  Every GPU holds its own copy
+// (at its own address) from module load until the process ends.  Its host-side name
+// is only a stand-in: host code uses the device address hipGetSymbolAddress returns,
+// looked up once per device on the first call there.  Nothing in it needs clearing
+// between calls: k_compact_count writes every tile count, k_scan_tiles every offset.
+constexpr uint32_t kNoAllocMaxKeys = 1u << 26;
+constexpr uint32_t kNoAllocMaxTiles = kNoAllocMaxKeys / kTile;
+static __device__ __align__(256) uint32_t scratch[2 * kNoAllocMaxTiles];
+
+std::mutex g_noalloc_mu;
+std::vector<uint32_t*> g_noalloc_ptr;  // per device: the address of `scratch` there
+
+uint32_t* noalloc_scratch_for_current_device() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "LaunchCellRangesCompactNoAlloc: no current device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if ((int)g_noalloc_ptr.size() <= dev) g_noalloc_ptr.resize(dev + 1, nullptr);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error((int)e, "LaunchCellRangesCompactNoAlloc: hipGetSymbolAddress(scratch)");
+            return nullptr;
+        }
+        g_noalloc_ptr[dev] = static_cast<uint32_t*>(p);
+    }
+    return g_noalloc_ptr[dev];
+}
+
 }  // namespace
 
 uint32_t compact_scratch_elems(uint32_t N) { return ceil_div(N > 0 ? N : 1, kTile); }
@@ -273,4 +306,20 @@ extern "C" void LaunchCellRangesCompact(uint32_t* d_uniqueKeys, uint32_t* d_offs
     if (!EnsureCellCompactScratch(N, kBS)) return;
     CompactScratch* cs = scratch_for_current_device();
     compact_segments(d_uniqueKeys, d_offsets, d_compactCount, nullptr, d_keysSorted, N, cs->counts, cs->offsets, s);
+}
+
+extern "C" void LaunchCellRangesCompactNoAlloc(uint32_t* d_uniqueKeys, uint32_t* d_offsets,
+                                               uint32_t* d_compactCount, const uint32_t* d_keysSorted, uint32_t N,
+                                               hipStream_t s) {
+    if (N == 0) {
+        compact_segments(d_uniqueKeys, d_offsets, d_compactCount, nullptr, d_keysSorted, 0, nullptr, nullptr, s);
+        return;
+    }
+    if (N > kNoAllocMaxKeys) {
+        set_error(PBFX_E_CAPACITY, "LaunchCellRangesCompactNoAlloc: more than 2^26 sorted keys (its static scratch)");
+        return;
+    }
+    uint32_t* sc = noalloc_scratch_for_current_device();
+    if (!sc) return;
+    compact_segments(d_uniqueKeys, d_offsets, d_compactCount, nullptr, d_keysSorted, N, sc, sc + kNoAllocMaxTiles, s);
 }

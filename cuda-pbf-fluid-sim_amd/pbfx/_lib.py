@@ -30,6 +30,7 @@ SIGNATURES = {
     "EnsureDropInStaging": (ctypes.c_bool, [_U32, _P]),
     "ReleaseDropInStaging": (None, [_P]),
     "LaunchCellRangesCompact": (None, [_P, _P, _P, _P, _U32, _P]),
+    "LaunchCellRangesCompactNoAlloc": (None, [_P, _P, _P, _P, _U32, _P]),
     "LaunchLambda": (None, [_P, _P, _P, _P, _P, _P, DeviceParams, _U32, _P]),
     "LaunchLambdaCompact": (None, [_P, _P, _P, _P, _P, _P, _P, DeviceParams, _U32, _P]),
     "LaunchDeltaApply": (None, [_P, _P, _P, _P, _P, _P, _P, DeviceParams, _U32, _P]),

@@ -64,6 +64,15 @@ bool EnsureCellCompactScratch(uint32_t N, uint32_t threadsPerBlock);
 // offsets[M] = N, *compactCount = M.
 void LaunchCellRangesCompact(uint32_t* d_uniqueKeys, uint32_t* d_offsets, uint32_t* d_compactCount,
                              const uint32_t* d_keysSorted, uint32_t N, hipStream_t s);
+// No counterpart in the reference: LaunchCellRangesCompact for callers that cannot
+// allocate (code under graph capture without a prior EnsureCellCompactScratch, tight
+// loops, harnesses that forbid hipMalloc).  Same outputs; the compaction scratch is a
+// static device array of the library (512 KiB per GPU, held until the process ends),
+// so it allocates nothing.  N <= 2^26 sorted keys (larger N records
+// PBFX_E_CAPACITY and launches nothing).  Calls that overlap on one GPU share that
+// array and race: order them on one stream.
+void LaunchCellRangesCompactNoAlloc(uint32_t* d_uniqueKeys, uint32_t* d_offsets, uint32_t* d_compactCount,
+                                    const uint32_t* d_keysSorted, uint32_t N, hipStream_t s);
 
 // ---- neighbour solves --------------------------------------------------------
 // reference: sim/pbf_lambda.cu:246-258 (KLambdaGlobalsDense :10-108)
