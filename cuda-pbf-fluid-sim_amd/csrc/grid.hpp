@@ -17,6 +17,12 @@ void compact_segments(uint32_t* uniq, uint32_t* offsets, uint32_t* count, uint32
                       const uint32_t* keysSorted, uint32_t N, uint32_t* blockCounts, uint32_t* blockOffsets,
                       hipStream_t s);
 
+// Compaction scratch of one stream, for LaunchCellRangesCompact calls on it (reserved
+// and released with the stream's drop-in staging).  The reference keeps one scratch per
+// process (cell_compact.cu:50-56), which two streams compacting at once would share.
+bool reserve_stream_compact_scratch(uint32_t N, hipStream_t s);
+void release_stream_compact_scratch(hipStream_t s);
+
 // Fill n words with a value (a kernel, so it is safe inside captured graphs).
 void fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
 

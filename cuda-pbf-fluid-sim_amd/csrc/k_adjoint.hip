@@ -185,11 +185,11 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k
         float3 accV = make_float3(0.f, 0.f, 0.f), accP = accV, dv = accV;
         auto dist2 = [&](const float4& pj) {
             const float3 r = sub3(pk, f3(pj));
-            return dot3(r, r);
+            return pair_r2(r.x, r.y, r.z);
         };
         auto apply = [&](uint32_t j) {
             const float3 r = sub3(pk, f3(Pf[j]));
-            const float r2 = dot3(r, r);
+            const float r2 = pair_r2(r.x, r.y, r.z);
             const float hr2 = h2 - r2;
             const float W = poly6 * hr2 * hr2 * hr2;
             const float3 vj = f3(Vn[j]);
@@ -285,14 +285,14 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k
     float3 acc = make_float3(0.f, 0.f, 0.f);
     auto dist2 = [&](const float4& pj) {
         const float3 r = sub3(pk, f3(pj));
-        return dot3(r, r);
+        return pair_r2(r.x, r.y, r.z);
     };
     with_pair_form(c, [&](auto form) {
         constexpr bool kFixed = decltype(form)::value;
         auto apply = [&](uint32_t j) {
             const float4 pj4 = P[j];
             const float3 r = sub3(pk, f3(pj4));
-            const float r2 = dot3(r, r);
+            const float r2 = pair_r2(r.x, r.y, r.z);
             const Spiky sp = spiky_of<kFixed>(r2, c);
             const float3 g = mul3(r, sp.f);
             const float3 da = sub3(ak, f3(A[j]));
@@ -304,14 +304,13 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k
                 const float hr2 = c.h2 - r2;
                 const float W = c.poly6 * hr2 * hr2 * hr2;
                 const float ratio = div_unscaled(W, c.w_q, c.rcp_w_q);
-                const double x = (double)ratio;
-                const double x2 = x * x;
-                const float raw = c.neg_scorr_k * (float)(x2 * x2);
+                const float x2 = ratio * ratio;
+                const float raw = c.neg_scorr_k * (x2 * x2);
                 scorr = raw;
                 if (raw < c.scorr_min) {
                     scorr = c.scorr_min;
                 } else if (ratio > 0.f) {
-                    const float pn1 = (float)(x * (x * x));
+                    const float pn1 = ratio * (ratio * ratio);
                     dsr = c.neg_scorr_k * c.scorr_n * pn1 * inv_w_q * (-6.0f * c.poly6 * hr2 * hr2);
                 }
             } else if (c.scorr_enable) {
@@ -369,12 +368,12 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k
     float3 acc = make_float3(0.f, 0.f, 0.f);
     auto dist2 = [&](const float4& pj) {
         const float3 r = sub3(pk, f3(pj));
-        return dot3(r, r);
+        return pair_r2(r.x, r.y, r.z);
     };
     with_pair_form(c, [&](auto form) {
         auto apply = [&](uint32_t j) {
             const float3 r = sub3(pk, f3(P[j]));
-            const float r2 = dot3(r, r);
+            const float r2 = pair_r2(r.x, r.y, r.z);
             const float4 qj = lamQ[j];
             const uint2 ej = lamE[j];
             const float epj = __uint_as_float(ej.x);

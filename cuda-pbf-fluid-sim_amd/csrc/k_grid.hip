@@ -8,6 +8,7 @@
 // consecutive slots, one 16-byte load), scan the tile counts, scatter unique
 // keys and offsets (reference: sim/cell_compact.cu:96-231).
 #include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "grid.hpp"
@@ -185,6 +186,7 @@ struct CompactScratch {
 };
 std::mutex g_scratch_mu;
 std::vector<CompactScratch> g_scratch;
+std::unordered_map<hipStream_t, CompactScratch> g_stream_scratch;  // under g_scratch_mu
 
 CompactScratch* scratch_for_current_device() {
     int dev = 0;
@@ -230,6 +232,34 @@ uint32_t* noalloc_scratch_for_current_device() {
 }  // namespace
 
 uint32_t compact_scratch_elems(uint32_t N) { return ceil_div(N > 0 ? N : 1, kTile); }
+
+bool reserve_stream_compact_scratch(uint32_t N, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    CompactScratch& cs = g_stream_scratch[s];
+    const uint32_t need = compact_scratch_elems(N);
+    if (need <= cs.capacity) return true;
+    if (cs.counts) (void)hipFree(cs.counts);
+    if (cs.offsets) (void)hipFree(cs.offsets);
+    cs = CompactScratch{};
+    if (hipMalloc(&cs.counts, sizeof(uint32_t) * need) != hipSuccess ||
+        hipMalloc(&cs.offsets, sizeof(uint32_t) * need) != hipSuccess) {
+        if (cs.counts) (void)hipFree(cs.counts);
+        g_stream_scratch.erase(s);
+        set_error(PBFX_E_CAPACITY, "stream compaction scratch: allocation failed");
+        return false;
+    }
+    cs.capacity = need;
+    return true;
+}
+
+void release_stream_compact_scratch(hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    const auto it = g_stream_scratch.find(s);
+    if (it == g_stream_scratch.end()) return;
+    if (it->second.counts) (void)hipFree(it->second.counts);
+    if (it->second.offsets) (void)hipFree(it->second.offsets);
+    g_stream_scratch.erase(it);
+}
 
 void fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s) {
     if (n == 0) return;
@@ -301,8 +331,19 @@ extern "C" void LaunchCellRangesCompact(uint32_t* d_uniqueKeys, uint32_t* d_offs
         compact_segments(d_uniqueKeys, d_offsets, d_compactCount, nullptr, d_keysSorted, 0, nullptr, nullptr, s);
         return;
     }
-    // Like the reference (cell_compact.cu:204-210) the scratch is grown here when
-    // needed; graph capture requires EnsureCellCompactScratch beforehand.
+    // A stream with its own scratch (reserved with its drop-in staging) uses it, so
+    // compactions on different streams never share tile counts.
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        const auto it = g_stream_scratch.find(s);
+        if (it != g_stream_scratch.end() && it->second.capacity >= compact_scratch_elems(N)) {
+            compact_segments(d_uniqueKeys, d_offsets, d_compactCount, nullptr, d_keysSorted, N, it->second.counts,
+                             it->second.offsets, s);
+            return;
+        }
+    }
+    // Like the reference (cell_compact.cu:204-210) the device-wide scratch is grown
+    // here when needed; graph capture requires EnsureCellCompactScratch beforehand.
     if (!EnsureCellCompactScratch(N, kBS)) return;
     CompactScratch* cs = scratch_for_current_device();
     compact_segments(d_uniqueKeys, d_offsets, d_compactCount, nullptr, d_keysSorted, N, cs->counts, cs->offsets, s);

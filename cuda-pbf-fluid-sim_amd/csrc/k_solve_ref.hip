@@ -17,6 +17,7 @@
 #include <mutex>
 #include <unordered_map>
 
+#include "grid.hpp"
 #include "pbf_common.hpp"
 #include "pbf_hip.h"
 #include "pbf_math.hpp"
@@ -91,7 +92,7 @@ __global__ __launch_bounds__(kBS) void k_lambda_ref(float* __restrict__ lambda, 
     LambdaAcc a;
     auto dist2 = [&](const float4& pj) {
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-        return rx * rx + ry * ry + rz * rz;
+        return pair_r2(rx, ry, rz);
     };
     const CellRows<Lookup> rows(lookup, dp.grid, reach_of(c.h, dp.grid.cellSize), keyS[ss]);
     auto walk = [&](const auto& fetch) {
@@ -99,7 +100,7 @@ __global__ __launch_bounds__(kBS) void k_lambda_ref(float* __restrict__ lambda, 
             auto apply = [&](uint32_t j) {
                 const float4 pj = fetch.at(j);
                 const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-                lambda_pair<decltype(form)::value>(a, rx, ry, rz, rx * rx + ry * ry + rz * rz, c);
+                lambda_pair<decltype(form)::value>(a, rx, ry, rz, pair_r2(rx, ry, rz), c);
             };
             neighbor_walk<false, kBS>(valid, s, rows, CandMask{}, fetch, sBuf + (threadIdx.x & ~63u),
                                       sQ + threadIdx.x, dp.maxNeighbors, c.h2, dist2, apply);
@@ -131,7 +132,7 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k
     float dx = 0.f, dy = 0.f, dz = 0.f;
     auto dist2 = [&](const float4& pj) {
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-        return rx * rx + ry * ry + rz * rz;
+        return pair_r2(rx, ry, rz);
     };
     const CellRows<Lookup> rows(lookup, dp.grid, reach_of(c.h, dp.grid.cellSize), keyS[ss]);
     auto walk = [&](const auto& fetch) {
@@ -140,7 +141,7 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k
                 const float4 pj = fetch.at(j);
                 const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
                 const float lam_j = kStaged ? pj.w : lambda[j];
-                delta_pair<decltype(form)::value>(dx, dy, dz, rx, ry, rz, rx * rx + ry * ry + rz * rz, lam_i,
+                delta_pair<decltype(form)::value>(dx, dy, dz, rx, ry, rz, pair_r2(rx, ry, rz), lam_i,
                                                   lam_j, c);
             };
             neighbor_walk<false, kBS>(valid, s, rows, CandMask{}, fetch, sBuf + (threadIdx.x & ~63u),
@@ -186,19 +187,17 @@ __global__ __launch_bounds__(kBS) void k_xsph_ref(float4* __restrict__ Vout, con
     int n = 0;
     auto dist2 = [&](const float4& pj) {
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-        return rx * rx + ry * ry + rz * rz;
+        return pair_r2(rx, ry, rz);
     };
     const CellRows<Lookup> rows(lookup, dp.grid, reach_of(dp.kernel.h, dp.grid.cellSize), keyS[ss]);
     auto walk = [&](const auto& fetch) {
         auto apply = [&](uint32_t j) {
             const float4 pj = fetch.at(j);
             const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            const float hr2 = h2 - (rx * rx + ry * ry + rz * rz);
+            const float hr2 = h2 - pair_r2(rx, ry, rz);
             const float w = poly6 * hr2 * hr2 * hr2;
             const float4 vj = kStaged ? Vs[j] : Vin[j];
-            sx += (vj.x - vi.x) * w;
-            sy += (vj.y - vi.y) * w;
-            sz += (vj.z - vi.z) * w;
+            xsph_pair(sx, sy, sz, vj, vi, w);
             ++n;
         };
         neighbor_walk<false, kBS>(valid, s, rows, CandMask{}, fetch, sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x,
@@ -318,7 +317,7 @@ void launch_xsph(float4* Vout, const float4* Vin, const float4* P, const uint32_
 extern "C" bool EnsureDropInStaging(uint32_t N, hipStream_t stream) {
     std::lock_guard<std::mutex> lk(g_stage_mu);
     Staging& st = g_stage[stream];
-    if (N <= st.capacity) return true;
+    if (N <= st.capacity) return reserve_stream_compact_scratch(N, stream);
     if (st.P) (void)hipFree(st.P);
     if (st.V) (void)hipFree(st.V);
     st = Staging{};
@@ -334,10 +333,11 @@ extern "C" bool EnsureDropInStaging(uint32_t N, hipStream_t stream) {
         return false;
     }
     st.capacity = N;
-    return true;
+    return reserve_stream_compact_scratch(N, stream);
 }
 
 extern "C" void ReleaseDropInStaging(hipStream_t stream) {
+    release_stream_compact_scratch(stream);
     std::lock_guard<std::mutex> lk(g_stage_mu);
     const auto it = g_stage.find(stream);
     if (it == g_stage.end()) return;

@@ -300,13 +300,13 @@ __global__ __launch_bounds__(BS) void k_lambda_sorted(float4* Ps, const uint32_t
     const int cap = dp.maxNeighbors;
     auto dist2 = [&](const float4& pj) {
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-        return rx * rx + ry * ry + rz * rz;
+        return pair_r2(rx, ry, rz);
     };
     with_pair_form(c, [&](auto form) {
         auto apply = [&](uint32_t k) {
             const float4 pj = Ps[k];
             const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            lambda_pair<decltype(form)::value>(a, rx, ry, rz, rx * rx + ry * ry + rz * rz, c);
+            lambda_pair<decltype(form)::value>(a, rx, ry, rz, pair_r2(rx, ry, rz), c);
             if (recB && a.n == cap) cut = k;
         };
         if (useMask)
@@ -346,14 +346,14 @@ __global__ __launch_bounds__(BS) void k_delta_sorted(const float4* __restrict__ 
     float dx = 0.f, dy = 0.f, dz = 0.f;
     auto dist2 = [&](const float4& pj) {
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-        return rx * rx + ry * ry + rz * rz;
+        return pair_r2(rx, ry, rz);
     };
     float4* buf = sBuf + (threadIdx.x & ~63u);
     with_pair_form(c, [&](auto form) {
         auto apply = [&](uint32_t k) {
             const float4 pj = Pa[k];
             const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            delta_pair<decltype(form)::value>(dx, dy, dz, rx, ry, rz, rx * rx + ry * ry + rz * rz, pi.w, pj.w, c);
+            delta_pair<decltype(form)::value>(dx, dy, dz, rx, ry, rz, pair_r2(rx, ry, rz), pi.w, pj.w, c);
         };
         if (maskMode == 2)
             mask_walk<BS>(valid, s, m, st, mk, Pa, sQ + threadIdx.x, 0, c.h2, dist2, apply);
@@ -432,17 +432,15 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8))) void k_
         int n = 0;
         auto dist2 = [&](const float4& pj) {
             const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            return rx * rx + ry * ry + rz * rz;
+            return pair_r2(rx, ry, rz);
         };
         auto apply = [&](uint32_t k) {
             const float4 pj = Ps[k];
             const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            const float hr2 = h2 - (rx * rx + ry * ry + rz * rz);
+            const float hr2 = h2 - pair_r2(rx, ry, rz);
             const float w = poly6 * hr2 * hr2 * hr2;
             const float4 vj = Vs[k];
-            sx += (vj.x - vi.x) * w;
-            sy += (vj.y - vi.y) * w;
-            sz += (vj.z - vi.z) * w;
+            xsph_pair(sx, sy, sz, vj, vi, w);
             ++n;
         };
         if (useMask)

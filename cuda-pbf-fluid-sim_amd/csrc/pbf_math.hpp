@@ -1,14 +1,22 @@
 // pbf_math.hpp — per-pair PBF arithmetic shared by every neighbour kernel.
 //
-// Expression association follows the reference exactly (the code is built
-// without FMA contraction, with IEEE division and square root), so the
-// reordered-layout kernels and the drop-in kernels produce the same bits:
-//   density += ((m * poly6) * hr2) * hr2 * hr2          pbf_lambda.cu:73-74
-//   coeff    = ((-3 * spiky) * (t * t)) / max(r, eps)  pbf_lambda.cu:76-79
-//   lambda   = -(rho/rho0 - 1) / max(denom, eps)       pbf_lambda.cu:86-107
-//   s_corr   = -k * (W / W(dq))^n, floored             pbf_delta_apply.cu:77-84
-//   delta   += (lambda_i + lambda_j + s_corr) * grad   pbf_delta_apply.cu:86-89
-//   dv      += (v_j - v_i) * W                         xsph.cu:66-72
+// Expression association follows the reference (pbf_lambda.cu, pbf_delta_apply.cu,
+// xsph.cu).  The code is built without implicit FMA contraction, with IEEE division
+// and square root; in the neighbour-pair expressions a product feeding an add is one
+// explicit fused multiply-add, in the fixed association below (the reference is
+// built with nvcc's default contraction, which fuses such pairs too), and the CPU
+// oracle forms the same fmaf calls, so every kernel and the oracle produce the same
+// bits:
+//   r2       = fma(rz, rz, fma(ry, ry, rx * rx))                  pbf_lambda.cu:68
+//   density  = fma(((m * poly6) * hr2) * hr2, hr2, density)       pbf_lambda.cu:73-74
+//   coeff    = ((-3 * spiky) * (t * t)) / max(r, eps)             pbf_lambda.cu:76-79
+//   sumGrad2 += fma(gz, gz, fma(gy, gy, gx * gx))                 pbf_lambda.cu:80-82
+//   lambda   = -(rho/rho0 - 1) / max(denom, eps)                  pbf_lambda.cu:86-107
+//   s_corr   = -k * (W / W(dq))^n, floored                        pbf_delta_apply.cu:77-84
+//   delta    = fma(lambda_i + lambda_j + s_corr, grad, delta)     pbf_delta_apply.cu:86-89
+//   dv       = fma(v_j - v_i, W, dv)                              xsph.cu:66-72
+// The per-particle expressions (lambda from its sums, the scaled and clamped
+// correction, the XSPH update) keep separate roundings.
 #pragma once
 
 #include <type_traits>
@@ -17,23 +25,20 @@
 
 namespace pbfx {
 
-// s_corr's pow(W / W(dq), n).  An integral exponent in [1, 16] is evaluated as a
-// product in double by binary exponentiation (x*x is exact in double, so n = 4
-// rounds once in double and once to float: the correctly rounded power except in
-// double-rounding ties); other exponents use powf.  The CPU oracle evaluates the
-// same product in the same order, so both sides produce the same bits.  The
-// reference calls powf (CUDA: <= 4 ulp), pbf_delta_apply.cu:81.
+// s_corr's pow(W / W(dq), n) (the reference calls powf, pbf_delta_apply.cu:81; CUDA
+// documents 2 ulp for it).  Exponents 1 to 4 are float products by squaring:
+// x, x*x, x*(x*x), (x*x)*(x*x), within 1.5 ulp of the power (two roundings, the first
+// doubled by the squaring).  Other integral exponents up to 16 are the same binary
+// exponentiation in double, rounded once to float; the rest use powf.  The CPU oracle
+// evaluates the same products in the same order, so both sides produce the same bits.
 __device__ __forceinline__ float pow_scorr(float x, float n, int ni) {
-    // the common exponents, straight-line: the same products as the loop below
-    // (1 * y is exact), without its loop overhead
-    const double xd = (double)x;
     switch (ni) {
         case 1: return x;
-        case 2: return (float)(xd * xd);
-        case 3: return (float)(xd * (xd * xd));
+        case 2: return x * x;
+        case 3: return x * (x * x);
         case 4: {
-            const double x2 = xd * xd;
-            return (float)(x2 * x2);
+            const float x2 = x * x;
+            return x2 * x2;
         }
         default: break;
     }
@@ -49,6 +54,12 @@ __device__ __forceinline__ float pow_scorr(float x, float n, int ni) {
         return (float)r;
     }
     return powf(x, n);
+}
+
+// Squared distance of a pair, r2 = fma(rz, rz, fma(ry, ry, rx * rx)): the one form
+// every neighbour test and pair expression uses (kernels and oracle alike).
+__device__ __forceinline__ float pair_r2(float rx, float ry, float rz) {
+    return __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
 }
 
 // The integral exponent of pow_scorr (0 when n is not an integer in [1, 16]).
@@ -192,7 +203,7 @@ __device__ __forceinline__ void lambda_pair(LambdaAcc& a, float rx, float ry, fl
                                             const PairConsts& c) {
     a.n++;
     const float hr2 = c.h2 - r2;
-    a.density += c.m_poly6 * hr2 * hr2 * hr2;
+    a.density = __builtin_fmaf(c.m_poly6 * hr2 * hr2, hr2, a.density);
     float coeff;
     if (kFixed || c.unscaled) {
         const float r = sqrt_unscaled(r2);  // +0 or positive: r2 is a sum of squares
@@ -209,7 +220,7 @@ __device__ __forceinline__ void lambda_pair(LambdaAcc& a, float rx, float ry, fl
     a.gx += ax;
     a.gy += ay;
     a.gz += az;
-    a.sumGrad2 += ax * ax + ay * ay + az * az;
+    a.sumGrad2 += __builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax));
 }
 
 // Lambda from the accumulated sums (reference: pbf_lambda.cu:85-107).
@@ -277,9 +288,8 @@ __device__ __forceinline__ void delta_pair(float& dx, float& dy, float& dz, floa
         const float hr2 = c.h2 - r2;
         const float w = c.poly6 * hr2 * hr2 * hr2;
         const float ratio = div_unscaled(w, c.w_q, c.rcp_w_q);
-        const double x = (double)ratio;  // pow_scorr's n = 4 product
-        const double x2 = x * x;
-        scorr = fmaxf(c.neg_scorr_k * (float)(x2 * x2), c.scorr_min);
+        const float x2 = ratio * ratio;  // pow_scorr's n = 4 product
+        scorr = fmaxf(c.neg_scorr_k * (x2 * x2), c.scorr_min);
     } else if (c.scorr_enable) {
         const float hr2 = c.h2 - r2;
         const float w = c.poly6 * hr2 * hr2 * hr2;
@@ -288,9 +298,16 @@ __device__ __forceinline__ void delta_pair(float& dx, float& dy, float& dz, floa
         if (c.scorr_min < 0.f) scorr = fmaxf(scorr, c.scorr_min);
     }
     const float lij = (lam_i + lam_j + scorr);
-    dx += lij * ax;
-    dy += lij * ay;
-    dz += lij * az;
+    dx = __builtin_fmaf(lij, ax, dx);
+    dy = __builtin_fmaf(lij, ay, dy);
+    dz = __builtin_fmaf(lij, az, dz);
+}
+
+// One in-range neighbour of the XSPH sum: dv += (v_j - v_i) W (fused).
+__device__ __forceinline__ void xsph_pair(float& sx, float& sy, float& sz, float4 vj, float4 vi, float w) {
+    sx = __builtin_fmaf(vj.x - vi.x, w, sx);
+    sy = __builtin_fmaf(vj.y - vi.y, w, sy);
+    sz = __builtin_fmaf(vj.z - vi.z, w, sz);
 }
 
 // Scale and relax of the summed correction, before the clamp

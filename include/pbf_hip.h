@@ -110,7 +110,10 @@ void LaunchXSPHCompact(float4* vel_out, const float4* vel_in, const float4* pos_
 // indicesSorted as before.  Like EnsureCellCompactScratch it allocates, so call it
 // before graph capture; growing the reservation frees the old buffers, so a graph
 // captured before then must be captured again.  Launches on the one stream are
-// ordered, and buffers are never shared between streams.  ReleaseDropInStaging frees a
+// ordered, and buffers are never shared between streams.  It also gives the stream its
+// own compaction scratch for LaunchCellRangesCompact (the reference's one scratch per
+// process, cell_compact.cu:50-56, is shared by every stream, so two streams compacting
+// at once would overwrite each other's tile counts).  ReleaseDropInStaging frees a
 // stream's buffers (call it once the stream is idle, before destroying it).
 bool EnsureDropInStaging(uint32_t N, hipStream_t stream);
 void ReleaseDropInStaging(hipStream_t stream);

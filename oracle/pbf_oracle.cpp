@@ -71,12 +71,26 @@ inline void decode_key(uint32_t key, const oi3& dim, int& x, int& y, int& z) {
     z = (int)(kdx / (uint32_t)dim.y);
 }
 
-// s_corr power (reference: pbf_delta_apply.cu:81 calls powf).  An integral exponent
-// in [1, 16] is evaluated as a product in double by binary exponentiation: for
-// n = 4, x*x is exact in double and x^4 is rounded once in double and once to
-// float, i.e. the correctly rounded power except in double-rounding ties.  Other
-// exponents call powf.  Pinned against float64 powers and powf in the tests.
+// Squared distance of a neighbour pair (reference: pbf_lambda.cu:68,
+// pbf_delta_apply.cu:68, xsph.cu:62), with the products fused into the adds in this
+// fixed association, as the reference's nvcc build contracts them; the GPU kernels
+// form the same expression.  (The density statistics keep the unfused form of the
+// statistics kernel they restate.)
+inline float pair_r2(float rx, float ry, float rz) { return std::fma(rz, rz, std::fma(ry, ry, rx * rx)); }
+
+// s_corr power (reference: pbf_delta_apply.cu:81 calls powf, documented by CUDA to
+// 2 ulp).  Exponents 1 to 4 are float products by squaring, x, x*x, x*(x*x),
+// (x*x)*(x*x): within 1.5 ulp of the power.  Other integral exponents up to 16 are the
+// same binary exponentiation in double, rounded once to float; the rest call powf.
+// Pinned against float64 powers and powf in the tests.
 inline float pow_scorr(float x, float n) {
+    if (n == 1.0f) return x;
+    if (n == 2.0f) return x * x;
+    if (n == 3.0f) return x * (x * x);
+    if (n == 4.0f) {
+        const float x2 = x * x;
+        return x2 * x2;
+    }
     if (n >= 1.0f && n <= 16.0f && (float)(int)n == n) {
         double b = (double)x, r = 1.0;
         int e = (int)n;
@@ -153,18 +167,18 @@ float lambda_one(uint32_t s, const of4* P, const uint32_t* idxS, const uint32_t*
         for_each_neighbor(s, idxS, keyS, dp.grid, kc.h, rg, [&](uint32_t j) -> bool {
             const of4 pj = P[j];
             const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            const float r2 = rx * rx + ry * ry + rz * rz;
+            const float r2 = pair_r2(rx, ry, rz);
             if (r2 > kc.h2) return true;
             n++;
             const float hr2 = kc.h2 - r2;
-            density += m * kc.poly6 * hr2 * hr2 * hr2;
+            density = std::fma(m * kc.poly6 * hr2 * hr2, hr2, density);
             const float r = std::sqrt(r2);
             const float r_safe = std::fmax(r, pbf.grad_r_eps);
             const float t = kc.h - r;
             const float coeff = (-3.0f * kc.spiky * (t * t) / r_safe);
             const float ax = coeff * rx, ay = coeff * ry, az = coeff * rz;
             gx += ax; gy += ay; gz += az;
-            sumGrad2 += ax * ax + ay * ay + az * az;
+            sumGrad2 += std::fma(az, az, std::fma(ay, ay, ax * ax));
             // cap: the reference stops at the next candidate or cell once n >= cap
             return !(hasCap && n >= cap);
         });
@@ -199,7 +213,7 @@ of4 delta_one(uint32_t s, const of4* P, const float* lambda, const uint32_t* idx
     for_each_neighbor(s, idxS, keyS, dp.grid, kc.h, rg, [&](uint32_t j) -> bool {
         const of4 pj = P[j];
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
+        const float r2 = pair_r2(rx, ry, rz);
         if (r2 > kc.h2) return true;
         const float r = std::sqrt(r2);
         const float t = kc.h - r;
@@ -215,9 +229,9 @@ of4 delta_one(uint32_t s, const of4* P, const float* lambda, const uint32_t* idx
             if (pbf.scorr_min < 0.f) scorr = std::fmax(scorr, pbf.scorr_min);
         }
         const float lij = (lambda[i] + lambda[j] + scorr);
-        dx += lij * ax;
-        dy += lij * ay;
-        dz += lij * az;
+        dx = std::fma(lij, ax, dx);
+        dy = std::fma(lij, ay, dy);
+        dz = std::fma(lij, az, dz);
         return true;
     });
     const float invRest = (dp.restDensity > 0.f) ? (1.0f / dp.restDensity) : 0.f;
@@ -253,14 +267,14 @@ of4 xsph_one(uint32_t s, const of4* V, const of4* P, const uint32_t* idxS, const
     for_each_neighbor(s, idxS, keyS, dp.grid, kc.h, rg, [&](uint32_t j) -> bool {
         const of4 pj = P[j];
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
+        const float r2 = pair_r2(rx, ry, rz);
         if (r2 > kc.h2) return true;
         const float hr2 = kc.h2 - r2;
         const float w = kc.poly6 * hr2 * hr2 * hr2;
         const of4 vj = V[j];
-        sx += (vj.x - vi.x) * w;
-        sy += (vj.y - vi.y) * w;
-        sz += (vj.z - vi.z) * w;
+        sx = std::fma(vj.x - vi.x, w, sx);
+        sy = std::fma(vj.y - vi.y, w, sy);
+        sz = std::fma(vj.z - vi.z, w, sz);
         ++n;
         return true;
     });

