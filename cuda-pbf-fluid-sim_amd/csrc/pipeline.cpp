@@ -565,16 +565,100 @@ bool Simulator::captureGraphs() {
     return true;
 }
 
+// Structural changes, as the reference's step takes them (simulator.cpp:583-607 with
+// ParamChangeTracker::structuralChanged, param_change_tracker.h:37-44): a particle
+// count beyond the buffers re-allocates them, a new grid (box, cell size or dims) or
+// smoothing length re-allocates the cell grid, and the step is re-captured.  Particle
+// state is kept: the first min(old, new) particles keep their positions and
+// velocities, new ones start at rest at the origin (or wherever the caller seeds them,
+// e.g. pbfSimSeedLattice at an offset).
+bool Simulator::restructure(const SimParams& p) {
+    if (!m_scenes.empty()) {
+        set_error(PBFX_E_INVALID_ARGUMENT,
+                  "Simulator::setParams: an ensemble's particle count, grid, h and iterations are fixed");
+        return false;
+    }
+    if (!m_tapes.empty()) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "Simulator::setParams: structural change while adjoint tapes are held");
+        return false;
+    }
+    if (p.grid.dim.x <= 0 || p.grid.dim.y <= 0 || p.grid.dim.z <= 0 || !(p.grid.cellSize > 0.f) ||
+        p.solverIters < 0) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "Simulator::setParams: empty grid or negative iterations");
+        return false;
+    }
+    const uint64_t cells = (uint64_t)p.grid.dim.x * (uint64_t)p.grid.dim.y * (uint64_t)p.grid.dim.z;
+    if (cells > 0xFFFFFFFFull) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "Simulator::setParams: more than 2^32 cells");
+        return false;
+    }
+    const uint32_t need = std::max<uint32_t>(p.maxParticles, p.numParticles);
+    const bool grow = need > m_bufs.capacity;
+    if (grow && m_bufs.externalPos) {
+        set_error(PBFX_E_CAPACITY, "Simulator::setParams: more particles than the bound external buffers hold");
+        return false;
+    }
+    if (m_opt.grid == GridKind::Hashed && (uint64_t(1) << m_opt.tableLog2) < std::max<uint64_t>(p.numParticles, 1)) {
+        set_error(PBFX_E_INVALID_ARGUMENT, "Simulator::setParams: hash table smaller than the particle count");
+        return false;
+    }
+    if (hipStreamSynchronize(m_stream) != hipSuccess) return false;
+    destroyGraphs();
+    const uint32_t cap = grow ? need : m_bufs.capacity;
+    if (grow) {
+        // keep the current state of the existing particles across the re-allocation
+        const uint32_t keep = std::min(m_params.numParticles, m_bufs.capacity);
+        float4 *pos = nullptr, *vel = nullptr;
+        if (!dalloc(pos, keep) || !dalloc(vel, keep)) {
+            dfree(pos);
+            return false;
+        }
+        bool ok = hipMemcpy(pos, m_bufs.d_pos, sizeof(float4) * keep, hipMemcpyDeviceToDevice) == hipSuccess &&
+                  hipMemcpy(vel, m_bufs.d_vel, sizeof(float4) * keep, hipMemcpyDeviceToDevice) == hipSuccess;
+        ok = ok && m_bufs.allocate(cap);
+        m_parity = 0;
+        ok = ok && hipMemcpy(m_bufs.d_pos, pos, sizeof(float4) * keep, hipMemcpyDeviceToDevice) == hipSuccess &&
+             hipMemcpy(m_bufs.d_pos_pred, pos, sizeof(float4) * keep, hipMemcpyDeviceToDevice) == hipSuccess &&
+             hipMemcpy(m_bufs.d_vel, vel, sizeof(float4) * keep, hipMemcpyDeviceToDevice) == hipSuccess;
+        dfree(pos);
+        dfree(vel);
+        if (!ok) {
+            set_error(PBFX_E_CAPACITY, "Simulator::setParams: re-allocation failed");
+            m_initialized = false;
+            return false;
+        }
+    }
+    const pbfx::SpanTable keepSpans = m_grid.spans;
+    const bool keepCandidate = m_grid.candidateMask;
+    const int items = m_grid.sortItems, digits = m_grid.sortDigitBits;
+    if (!m_grid.allocate(cap, (uint32_t)cells, m_opt.grid, m_opt.tableLog2, reach_for(p),
+                         m_opt.path == PathKind::Reordered)) {
+        m_initialized = false;
+        return false;
+    }
+    // launch knobs survive the re-allocation
+    m_grid.candidateMask = keepCandidate;
+    m_grid.sortItems = items;
+    m_grid.sortDigitBits = digits;
+    m_grid.spans.streamRatio = keepSpans.streamRatio;
+    m_grid.spans.chunkBits = keepSpans.chunkBits;
+    m_grid.spans.block = keepSpans.block;
+    if (m_opt.path == PathKind::Reference) {
+        if (m_opt.grid != GridKind::Dense && !EnsureCellCompactScratch(cap, 256)) return false;
+        if (!EnsureDropInStaging(cap, m_stream)) return false;
+    }
+    m_params = p;
+    m_graphDirty = true;
+    return true;
+}
+
 bool Simulator::setParams(const SimParams& p) {
     if (!m_initialized) return false;
     if (std::memcmp(&p, &m_params, sizeof(SimParams)) == 0) return true;
-    if (p.numParticles != m_params.numParticles || p.grid.dim.x != m_params.grid.dim.x ||
-        p.grid.dim.y != m_params.grid.dim.y || p.grid.dim.z != m_params.grid.dim.z ||
-        p.grid.cellSize != m_params.grid.cellSize || p.kernel.h != m_params.kernel.h ||
-        p.solverIters != m_params.solverIters) {
-        set_error(PBFX_E_INVALID_ARGUMENT, "Simulator::setParams: structural change needs initialize()");
-        return false;
-    }
+    if (p.numParticles != m_params.numParticles || p.maxParticles != m_params.maxParticles ||
+        std::memcmp(&p.grid, &m_params.grid, sizeof(p.grid)) != 0 || p.kernel.h != m_params.kernel.h ||
+        p.solverIters != m_params.solverIters)
+        return restructure(p);
     // an ensemble holds box, gravity, XSPH and restitution per scene; only the shared
     // parameters (dt, solver coefficients, clamps, neighbour cap) can change here
     if (!m_scenes.empty() &&

@@ -253,6 +253,7 @@ public:
                       double* gcAccum, int32_t* log2Scale = nullptr);
 
 private:
+    bool restructure(const SimParams& p);
     bool captureGraphs();
     void destroyGraphs();
     void runPipeline(hipStream_t s);

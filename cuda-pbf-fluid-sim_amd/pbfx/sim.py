@@ -205,14 +205,17 @@ class Simulator:
             raise _lib.PbfError("step failed")
 
     def set_params(self, params):
-        """New parameters for the following steps (reference: ParamChangeTracker,
-        param_change_tracker.h:37-56).  Coefficients, gravity, dt and the like take
-        effect at the next step (a captured graph is re-captured); a change of particle
-        count, grid dims, cell size, h or solver iterations raises PbfError."""
+        """New parameters for the following steps (reference: Simulator::step's parameter
+        handling, simulator.cpp:575-607, ParamChangeTracker param_change_tracker.h:23-61).
+        Coefficients, gravity, dt and the like take effect at the next step (the graph is
+        re-captured); a new particle count, grid, h or iteration count re-allocates what
+        depends on it and keeps the particles' state.  Raises PbfError when refused (see
+        pbfSimSetParams)."""
         if _lib.lib().pbfSimSetParams(self._h, ctypes.byref(params)) != 0:
             _lib.check("pbfSimSetParams")
-            raise _lib.PbfError("set_params: structural change (needs a new simulator)")
+            raise _lib.PbfError("set_params refused")
         self.params = params
+        self.n = int(params.numParticles)
 
     def synchronize(self):
         if _lib.lib().pbfSimSynchronize(self._h) != 0:

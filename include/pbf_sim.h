@@ -42,7 +42,15 @@ int pbfSimGetState(PbfSim* sim, float4* pos, float4* vel, int toDevice);
 // reference: Simulator::step (simulator.cpp:560-643).  Enqueues `steps` steps on the
 // simulator's stream (graph replay when enabled) without synchronising.
 int pbfSimStep(PbfSim* sim, int steps);
-// New parameters for the following steps (same grid and particle count).
+// New parameters for the following steps (reference: the parameter handling of
+// Simulator::step, simulator.cpp:575-607, ParamChangeTracker param_change_tracker.h:
+// 23-61).  Coefficients, gravity, dt, clamps and the neighbour cap re-capture the
+// step.  A new particle count, grid (box, cell size, dims), smoothing length or
+// iteration count re-allocates what depends on it (the particle buffers only when
+// they must grow) and keeps the state of the particles that remain; particles added
+// start at rest at the origin until seeded.  Refused (-1, PBFX_E_*): structural
+// changes of an ensemble or while adjoint tapes are held, growth beyond bound external
+// position buffers, a hash table smaller than the particle count.
 int pbfSimSetParams(PbfSim* sim, const sim::SimParams* params);
 // The parameters the simulator steps with (for an ensemble: the shared ones, with the
 // ensemble's particle count and key space), as the base for pbfSimSetParams.

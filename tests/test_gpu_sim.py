@@ -270,9 +270,11 @@ def test_8m_splash_hashed_equals_compact(dev):
 
 
 def _changed_params(scene):
-    """Dynamic changes of the kind ParamChangeTracker (param_change_tracker.h:37-56)
-    passes through without a rebuild: step size, gravity, XSPH, s_corr, the displacement
-    clamp (which also widens the candidate mask's skin) and the neighbour cap."""
+    """Dynamic changes, which ParamChangeTracker (param_change_tracker.h:46-56) passes
+    through to the captured step: step size, gravity, XSPH, s_corr, the displacement
+    clamp (which also widens the candidate mask's skin), and the neighbour cap (which
+    the reference counts as structural, param_change_tracker.h:39, a re-capture either
+    way)."""
     sp = P.build_sim_params(scene)
     sp.dt = sp.dt * 0.75
     sp.gravity.x = 2.0
@@ -305,36 +307,115 @@ def test_set_params_between_steps_matches_oracle(oracle, dev, use_graph, path, g
     np.testing.assert_array_equal(V, eV)
 
 
-def test_set_params_rejects_structural_changes(dev):
+def _structural(scene, kind):
+    """A structural change of the reference's kind (param_change_tracker.h:37-44 and the
+    re-allocation of simulator.cpp:583-597) and the scene it describes."""
+    if kind == "iters":
+        sp = P.build_sim_params(scene)
+        sp.solverIters = 3
+        return scene, sp
+    if kind == "box":
+        sc = P.small_block((8, 8, 8), box_max=(1.25, 1.0, 0.75), table_log2=16)
+        return sc, P.build_sim_params(sc)
+    if kind == "h":
+        sc = P.small_block((8, 8, 8), h_over_r=2.2, table_log2=16)
+        return sc, P.build_sim_params(sc)
+    raise ValueError(kind)
+
+
+@pytest.mark.parametrize("kind", ["iters", "box", "h"])
+@pytest.mark.parametrize("path,grid", [("reordered", "hashed"), ("reordered", "dense"), ("reference", "compact")])
+def test_set_params_structural_changes_match_oracle(oracle, dev, kind, path, grid):
+    """Solver iterations, the box (hence the grid dims) and the smoothing length (hence
+    kernel constants, mass and cell size) changed between steps: the simulator
+    re-allocates what depends on them, keeps the particles' state and re-captures, and
+    equals the oracle run with the old then the new parameters, bit for bit."""
+    scene = P.small_block((8, 8, 8), table_log2=16)
+    spA = P.build_sim_params(scene)
+    _, spB = _structural(scene, kind)
+    sim = _sim(scene, grid, path=path)
+    try:
+        sim.step(2)
+        sim.set_params(spB)
+        sim.step(2)
+        X, V = sim.state()
+    finally:
+        sim.close()
+    X0 = P.seed_scene_reference(scene, oracle.seed_lattice)
+    compact = grid != "dense"
+    eX, eV = oracle.run(X0, np.zeros_like(X0), spA, 2, compact=compact)
+    eX, eV = oracle.run(eX, eV, spB, 2, compact=compact)
+    np.testing.assert_array_equal(X, eX)
+    np.testing.assert_array_equal(V, eV)
+
+
+@pytest.mark.parametrize("path,grid", [("reordered", "hashed"), ("reference", "compact")])
+def test_set_params_more_particles(oracle, dev, path, grid):
+    """A particle count beyond the buffers (reference: simulator.cpp:583-597 re-allocates):
+    the first 512 particles keep their state, a second block is seeded at offset 512, and
+    the run equals the oracle's on the concatenated state."""
+    from pbfx import _lib
+    scene = P.small_block((8, 8, 8), table_log2=16)
+    spA = P.build_sim_params(scene)
+    spB = P.build_sim_params(scene)
+    spB.numParticles = spB.maxParticles = 1024
+    sim = _sim(scene, grid, path=path)
+    o2 = (0.4, 0.005, 0.005)
+    amp = float(np.float32(np.float32(scene.jitter_frac_r) * np.float32(scene.radius)))
+    try:
+        sim.step(2)
+        sim.set_params(spB)
+        assert sim.n == 1024
+        assert _lib.lib().pbfSimSeedLattice(sim._h, 512, 8, 8, 8, o2[0], o2[1], o2[2], scene.spacing, amp, 99) == 0
+        sim.step(2)
+        X, V = sim.state()
+    finally:
+        sim.close()
+    X0 = P.seed_scene_reference(scene, oracle.seed_lattice)
+    compact = grid != "dense"
+    eX, eV = oracle.run(X0, np.zeros_like(X0), spA, 2, compact=compact)
+    B = oracle.seed_lattice(8, 8, 8, o2, scene.spacing, spB.grid, amp, 99)
+    eX = np.concatenate([eX, B])
+    eV = np.concatenate([eV, np.zeros_like(B)])
+    eX, eV = oracle.run(eX, eV, spB, 2, compact=compact)
+    np.testing.assert_array_equal(X, eX)
+    np.testing.assert_array_equal(V, eV)
+
+
+def test_set_params_structural_refusals(oracle, dev):
+    """Refused with an error, the simulator stepping on with its own parameters: a
+    structural change while adjoint tapes are held, and more particles than bound
+    external position buffers hold."""
+    import torch
     from pbfx import _lib
     scene = P.small_block((8, 8, 8), table_log2=16)
     sim = _sim(scene, "hashed")
     try:
-        sim.step(1)
-        X1, _ = sim.state()
-        sim.set_params(P.build_sim_params(scene))  # unchanged: accepted, nothing to do
-        for field, value in (("solverIters", 3), ("numParticles", 100)):
-            sp = P.build_sim_params(scene)
-            setattr(sp, field, value)
-            with pytest.raises(_lib.PbfError):
-                sim.set_params(sp)
+        assert _lib.lib().pbfSimAdjointInit(sim._h, 1) == 0
         sp = P.build_sim_params(scene)
-        sp.kernel.h = sp.kernel.h * 1.5
+        sp.solverIters = 3
         with pytest.raises(_lib.PbfError):
             sim.set_params(sp)
-        # the rejected calls left the simulator stepping with its own parameters
-        sim.step(1)
-        X2, _ = sim.state()
     finally:
         sim.close()
-    ref = _sim(scene, "hashed")
+    sim = _sim(scene, "hashed")
     try:
-        ref.step(2)
-        eX, _ = ref.state()
+        n = sim.n
+        ta = torch.zeros((n, 4), dtype=torch.float32, device=dev)
+        tb = torch.zeros((n, 4), dtype=torch.float32, device=dev)
+        sim.bind_external_positions(ta, tb)
+        sp = P.build_sim_params(scene)
+        sp.numParticles = sp.maxParticles = n + 1
+        with pytest.raises(_lib.PbfError):
+            sim.set_params(sp)
+        sim.step(1)
+        X1, _ = sim.state()
     finally:
-        ref.close()
-    assert not np.array_equal(X1, X2)
-    np.testing.assert_array_equal(X2, eX)
+        sim.close()
+    X0 = P.seed_scene_reference(scene, oracle.seed_lattice)
+    eX, _ = oracle.run(X0, np.zeros_like(X0), P.build_sim_params(scene), 1, compact=True)
+    np.testing.assert_array_equal(X1, eX)
+    del ta, tb
 
 
 @pytest.mark.parametrize("dims", [(16, 16, 16), (13, 11, 7)])
