@@ -107,7 +107,7 @@ __global__ __launch_bounds__(kBS) void k_lambda_ref(float* __restrict__ lambda, 
         });
     };
     if (kStaged)
-        walk(SortedFetch{Ps});
+        walk(SortedFetch{SlotBuf(Ps, N + kSlotPad)});
     else
         walk(IndexedFetch{P, idxS});
     if (valid) lambda[i] = lambda_finish(a, dp);
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k
         });
     };
     if (kStaged)
-        walk(SortedFetch{Ps});
+        walk(SortedFetch{SlotBuf(Ps, N + kSlotPad)});
     else
         walk(IndexedFetch{P, idxS});
     if (!valid) return;
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(kBS) void k_xsph_ref(float4* __restrict__ Vout, con
                                   0, h2, dist2, apply);
     };
     if (kStaged)
-        walk(SortedFetch{Ps});
+        walk(SortedFetch{SlotBuf(Ps, N + kSlotPad)});
     else
         walk(IndexedFetch{P, idxS});
     if (!valid) return;

@@ -295,6 +295,7 @@ __global__ __launch_bounds__(BS) void k_lambda_sorted(float4* Ps, const uint32_t
     const float4 pi = Ps[ss];
     const uint32_t m = segOf[ss];
     const PairConsts c = make_pair_consts(dp);
+    const SlotBuf PS(Ps, N + kSlotPad);
     LambdaAcc a;
     uint32_t cut = 0xFFFFFFFFu;  // recording: slot of the neighbour that reached the cap
     const int cap = dp.maxNeighbors;
@@ -304,15 +305,15 @@ __global__ __launch_bounds__(BS) void k_lambda_sorted(float4* Ps, const uint32_t
     };
     with_pair_form(c, [&](auto form) {
         auto apply = [&](uint32_t k) {
-            const float4 pj = Ps[k];
+            const float4 pj = PS[k];
             const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
             lambda_pair<decltype(form)::value>(a, rx, ry, rz, pair_r2(rx, ry, rz), c);
             if (recB && a.n == cap) cut = k;
         };
         if (useMask)
-            mask_walk<BS>(valid, s, m, st, mk, Ps, sQ + threadIdx.x, dp.maxNeighbors, c.h2, dist2, apply);
+            mask_walk<BS>(valid, s, m, st, mk, PS, sQ + threadIdx.x, dp.maxNeighbors, c.h2, dist2, apply);
         else
-            neighbor_walk<false, BS>(valid, s, SpanRows(st, m), mk, SortedFetch{Ps}, sBuf + (threadIdx.x & ~63u),
+            neighbor_walk<false, BS>(valid, s, SpanRows(st, m), mk, SortedFetch{PS}, sBuf + (threadIdx.x & ~63u),
                                      sQ + threadIdx.x, dp.maxNeighbors, c.h2, dist2, apply);
     });
     if (!valid) return;
@@ -343,6 +344,7 @@ __global__ __launch_bounds__(BS) void k_delta_sorted(const float4* __restrict__ 
     const float4 pi = Pa[ss];
     const uint32_t m = segOf[ss];
     const PairConsts c = make_pair_consts(dp);
+    const SlotBuf PA(Pa, N + kSlotPad);
     float dx = 0.f, dy = 0.f, dz = 0.f;
     auto dist2 = [&](const float4& pj) {
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
@@ -351,17 +353,17 @@ __global__ __launch_bounds__(BS) void k_delta_sorted(const float4* __restrict__ 
     float4* buf = sBuf + (threadIdx.x & ~63u);
     with_pair_form(c, [&](auto form) {
         auto apply = [&](uint32_t k) {
-            const float4 pj = Pa[k];
+            const float4 pj = PA[k];
             const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
             delta_pair<decltype(form)::value>(dx, dy, dz, rx, ry, rz, pair_r2(rx, ry, rz), pi.w, pj.w, c);
         };
         if (maskMode == 2)
-            mask_walk<BS>(valid, s, m, st, mk, Pa, sQ + threadIdx.x, 0, c.h2, dist2, apply);
+            mask_walk<BS>(valid, s, m, st, mk, PA, sQ + threadIdx.x, 0, c.h2, dist2, apply);
         else if (maskMode == 1)
-            neighbor_walk<true, BS>(valid, s, SpanRows(st, m), mk, SortedFetch{Pa}, buf, sQ + threadIdx.x, 0, c.h2,
+            neighbor_walk<true, BS>(valid, s, SpanRows(st, m), mk, SortedFetch{PA}, buf, sQ + threadIdx.x, 0, c.h2,
                                     dist2, apply);
         else
-            neighbor_walk<false, BS>(valid, s, SpanRows(st, m), mk, SortedFetch{Pa}, buf, sQ + threadIdx.x, 0, c.h2,
+            neighbor_walk<false, BS>(valid, s, SpanRows(st, m), mk, SortedFetch{PA}, buf, sQ + threadIdx.x, 0, c.h2,
                                      dist2, apply);
     });
     if (!valid) return;
@@ -428,6 +430,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8))) void k_
     if (__any(walk)) {
         const uint32_t m = segOf[ss];
         const float h2 = dp.kernel.h2, poly6 = dp.kernel.poly6;
+        const SlotBuf PS(Ps, N + kSlotPad), VS(Vs, N);
         float sx = 0.f, sy = 0.f, sz = 0.f;
         int n = 0;
         auto dist2 = [&](const float4& pj) {
@@ -435,18 +438,18 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8))) void k_
             return pair_r2(rx, ry, rz);
         };
         auto apply = [&](uint32_t k) {
-            const float4 pj = Ps[k];
+            const float4 pj = PS[k];
             const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
             const float hr2 = h2 - pair_r2(rx, ry, rz);
             const float w = poly6 * hr2 * hr2 * hr2;
-            const float4 vj = Vs[k];
+            const float4 vj = VS[k];
             xsph_pair(sx, sy, sz, vj, vi, w);
             ++n;
         };
         if (useMask)
-            mask_walk<BS>(valid && walk, s, m, st, mk, Ps, sQ + threadIdx.x, 0, h2, dist2, apply);
+            mask_walk<BS>(valid && walk, s, m, st, mk, PS, sQ + threadIdx.x, 0, h2, dist2, apply);
         else
-            neighbor_walk<false, BS>(valid && walk, s, SpanRows(st, m), mk, SortedFetch{Ps}, sBuf + (threadIdx.x & ~63u),
+            neighbor_walk<false, BS>(valid && walk, s, SpanRows(st, m), mk, SortedFetch{PS}, sBuf + (threadIdx.x & ~63u),
                                  sQ + threadIdx.x, 0, h2, dist2, apply);
         if (recGm && valid)
             recGm[s] = xsph_gate(n, dp) * ((dp.restDensity > 0.f) ? (dp.particleMass / dp.restDensity) : 0.f);

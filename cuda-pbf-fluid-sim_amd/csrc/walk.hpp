@@ -12,6 +12,27 @@ namespace walk {
 
 constexpr int kBS = 256;  // lanes per block of the kernels that walk (default)
 
+// A slot-indexed float4 array read through a buffer resource: each read is a 32-bit
+// byte offset (slot * 16) from the descriptor's base, so the walk's candidate and pair
+// loads need no 64-bit address arithmetic per candidate, and a round's consecutive
+// slots become immediate offsets of one address.  Reads at or past `slots` return
+// zeros.  Arrays of up to kMaxSlotBufSlots slots (the offsets are 32-bit).
+constexpr uint32_t kMaxSlotBufSlots = 1u << 27;
+typedef int SlotBufV4 __attribute__((ext_vector_type(4)));
+struct SlotBuf {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ SlotBuf(const float4* p, uint32_t slots)
+        // word 3 of a gfx9 buffer descriptor: 32-bit data format, raw addressing
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(p), (short)0, (int)(slots * 16u), 0x00020000)) {}
+    __device__ __forceinline__ float4 operator[](uint32_t k) const { return at(k, 0); }
+    // slot k + u with u a compile-time constant in [0, 4): the u * 16 bytes ride in the
+    // load's scalar offset as an inline constant, so a round of four shares one address
+    __device__ __forceinline__ float4 at(uint32_t k, uint32_t u) const {
+        const SlotBufV4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(k << 4), (int)(u << 4), 0);
+        return make_float4(__int_as_float(v.x), __int_as_float(v.y), __int_as_float(v.z), __int_as_float(v.w));
+    }
+};
+
 // Neighbour walk of one wave (64 consecutive sorted slots, one particle per lane).
 //
 // Rows.  A row source gives the lane's particle its neighbour rows: count(), and
@@ -69,6 +90,18 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 // The mask covers the first 32 * kMaskWords candidates; candidates past that are
 // tested directly.
 constexpr uint32_t kMaskWords = kCandMaskWords;
+
+// Candidate u of a per-lane round starting at slot k: with a padded slot-order fetch,
+// slot k + u read at an immediate offset from slot k's address; otherwise the fetch of
+// the candidate's id.
+template <class Fetch>
+__device__ __forceinline__ auto fetch_round(const Fetch& f, uint32_t k, uint32_t u, uint32_t id)
+    -> decltype(f.at(id)) {
+    if constexpr (Fetch::kPadded)
+        return f.at_next(k, u);
+    else
+        return f.at(id);
+}
 
 // Bits [0, n) of a 64-bit word (n <= 64).
 __device__ __forceinline__ uint64_t low_bits64(uint32_t n) { return n >= 64u ? ~0ull : ((1ull << n) - 1ull); }
@@ -143,7 +176,7 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, const Rows
 #pragma unroll
                 for (uint32_t u = 0; u < kUnroll; ++u) id[u] = fetch.id(Fetch::kPadded ? k + u : min(k + u, last));
 #pragma unroll
-                for (uint32_t u = 0; u < kUnroll; ++u) pj[u] = fetch.at(id[u]);
+                for (uint32_t u = 0; u < kUnroll; ++u) pj[u] = fetch_round(fetch, k, u, id[u]);
                 const uint32_t inSpan = (1u << cnt) - 1u;
                 const uint32_t self = s - k < kUnroll ? 1u << (s - k) : 0u;
                 uint32_t bits = 0u, skin = 0u;
@@ -248,7 +281,7 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, const Rows
 // together; the next mask word is loaded while the current one is consumed.
 template <int kLanes, class Dist2, class Apply>
 __device__ __forceinline__ void mask_walk(bool valid, uint32_t s, uint32_t m, const SpanTable& st,
-                                          const CandMask& mk, const float4* P, uint32_t* q, int cap, float h2,
+                                          const CandMask& mk, const SlotBuf& P, uint32_t* q, int cap, float h2,
                                           Dist2&& dist2, Apply&& apply) {
     int qh = 0, qn = 0, found = 0;
     auto pop_one = [&]() {
@@ -375,9 +408,11 @@ struct SpanRows {
 // is its slot.
 struct SortedFetch {
     static constexpr bool kPadded = true;
-    const float4* P;
+    SlotBuf P;
     __device__ __forceinline__ uint32_t id(uint32_t k) const { return k; }
     __device__ __forceinline__ float4 at(uint32_t k) const { return P[k]; }
+    // the particle in slot k + u (u < kUnroll, a compile-time constant)
+    __device__ __forceinline__ float4 at_next(uint32_t k, uint32_t u) const { return P.at(k, u); }
 };
 
 }  // namespace walk
