@@ -11,10 +11,12 @@ solve, velocity, XSPH, boundary); state stays in HBM.
 The dam break changes character along its 500 steps (about 30 lattice steps with ~6
 neighbours a particle, then a collapsed fluid with ~300), so the timed steps sample
 the whole trajectory:
-- W warm-up steps (graph replay), then the autotuner at step 60 (state restored);
-- an untimed pre-pass runs the configured trajectory, steps W .. W+500, eager with a
-  HIP event after every launch (its mean is reported as `trajectory`), keeping device
-  snapshots of the state at K stratified steps W + floor((k + 1/2) 500 / K);
+- W warm-up steps (graph replay);
+- an untimed graph-replay pass over the configured trajectory, steps W .. W+500,
+  keeping device snapshots of the state at K stratified steps W + floor((k + 1/2) 500 / K);
+- the autotuner, timing each setting from 4 of those states spread over the trajectory;
+- an untimed pre-pass runs the trajectory again from the end of the warm-up, eager with
+  a HIP event after every launch (its mean is reported as `trajectory`);
 - the timed region: K steps, one from each snapshot (restored outside the timed
   intervals), each eager with per-launch events and bracketed by synchronisations;
   `value` is particle-steps/s over them and the per-kernel durations (the roofline)
@@ -83,10 +85,12 @@ def sample_offsets(total, k):
 
 
 def _profiled(path, kernel, steps, knobs, field):
-    """Per-launch mean of a per-step profiled figure (`field`: "valu" or "bytes") of
-    `kernel` over the dam break's `steps` (absolute step numbers), from a counter
-    summary kept next to the other profiles, for the same walk knobs; or (None, reason).
-    The simulation is deterministic, so a step's counts are the same in every run."""
+    """Per-launch mean of a profiled figure (`field`: "valu" or "bytes") of `kernel` over the
+    dam break's `steps` (absolute step numbers), from a counter summary kept next to the
+    other profiles, for the same walk knobs; or (None, reason).  The summary keeps the
+    figure of each step the bench samples by default and the mean over the trajectory;
+    other steps get that mean (said in the note).  The simulation is deterministic, so a
+    step's counts are the same in every run."""
     try:
         with open(path) as f:
             rec = json.load(f)
@@ -100,11 +104,13 @@ def _profiled(path, kernel, steps, knobs, field):
         src = f"{os.path.relpath(path, REPO)} ({rec.get('command', '').replace('<TUNING>', json.dumps(prof))})"
         if not ps:
             return None, f"{kernel} not in {os.path.relpath(path, REPO)}"
-        first, vals = ps["first_step"], ps[field]
-        if not all(first <= t < first + len(vals) for t in steps):
-            return None, f"profiled steps {first}..{first + len(vals)} do not cover the timed steps"
-        win = [vals[t - first] for t in steps]
-        return sum(win) / (len(steps) * ps["launches_per_step"]), src
+        per = ps.get("steps") or {}
+        if per and all(str(t) in per for t in steps):
+            return sum(per[str(t)] for t in steps) / (len(steps) * ps["launches_per_step"]), src
+        if "trajectory_mean" in ps:
+            return ps["trajectory_mean"], f"{src}; mean over the profiled trajectory {ps['trajectory']} " \
+                                          "(other steps timed)"
+        return None, f"{os.path.relpath(path, REPO)} holds no figures for the timed steps"
     return None, "nothing profiled at this run's launch knobs"
 
 
@@ -483,9 +489,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--trajectory", type=int, default=None,
                     help="steps of the configured run the timed steps sample (default: the configuration's 500)")
-    ap.add_argument("--tune-at", type=int, default=60,
-                    help="step at which the autotuner times the launch knobs (the dam break has collapsed "
-                         "by then); the state is then restored to the end of the warm-up")
+    ap.add_argument("--tune-states", type=int, default=4,
+                    help="the autotuner times each setting from this many states evenly spread over the "
+                         "trajectory (taken by a graph-replay pass over it), then the run restarts from the end "
+                         "of the warm-up")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch knobs")
     ap.add_argument("--tune-steps", type=int, default=5)
     ap.add_argument("--tuning", default=None,
@@ -529,30 +536,36 @@ def main():
           file=sys.stderr, flush=True)
     sim.step(W)
     sim.synchronize()
+    # Untimed graph-replay pass over the configured trajectory, keeping a device snapshot
+    # of the state before each sampled step (the knobs never change results, so these
+    # are the states of any later run).
+    warm = sim.state_device()
+    snaps = []
+    done = 0
+    for o in offs:
+        sim.step(o - done)
+        done = o
+        snaps.append(sim.state_device())
     tuning = None
     if a.tuning:
         sim.set_tuning(**json.loads(a.tuning))
         tuning = {"chosen": sim.tuning(), "fixed": True}
     elif not a.no_tune:
         from pbfx import tune
-        X0, V0 = sim.state_device()
-        ahead = max(0, a.tune_at - W)
-        sim.step(ahead)
+        pick = sorted({(2 * j + 1) * len(snaps) // (2 * a.tune_states) for j in range(a.tune_states)})
         best, best_ms, table = tune.autotune(sim, steps=a.tune_steps, key=tune.workload_key(sim, "dam_break_1m"),
-                                             log=lambda m: print(m, file=sys.stderr, flush=True))
-        sim.set_state_device(X0, V0)  # back to the end of the warm-up: the knobs never change results
-        del X0, V0
+                                             log=lambda m: print(m, file=sys.stderr, flush=True),
+                                             states=[snaps[i] for i in pick])
         tuning = {"chosen": best, "ms_per_step_at_tuning": round(best_ms, 4), "settings_tried": len(table),
-                  "steps_per_setting": a.tune_steps, "state": f"step {W + ahead}, then restored to step {W}"}
+                  "steps_per_setting": a.tune_steps,
+                  "states": f"{a.tune_steps} steps from each of the states before steps {[W + offs[i] for i in pick]}"}
+    sim.set_state_device(*warm)
+    del warm
 
-    # Untimed pre-pass: the configured trajectory, eager with per-launch events, keeping
-    # a device snapshot of the state before each sampled step.
+    # Untimed pre-pass: the configured trajectory again, eager with per-launch events (the
+    # trajectory's mean step and per-kernel times).
     traj = Window()
-    snaps = []
-    want = set(offs)
     for t in range(T):
-        if t in want:
-            snaps.append(sim.state_device())
         t0 = time.perf_counter()
         launches = sim.profile_launches()
         traj.add(launches, time.perf_counter() - t0)

@@ -85,15 +85,29 @@ class _Snapshot:
             raise _lib.PbfError("restore failed")
 
 
-def autotune(sim, steps=3, sweeps=2, space=None, key=None, path=DEFAULT_CACHE, log=None, snapshot=None,
-             clock=time.perf_counter):
-    """Time `steps` steps per setting from the current state and keep the fastest.
+class _StateSnapshot:
+    """A caller's device state (X, V) as a snapshot of `sim` (restore() loads it)."""
 
-    Returns (best setting, ms per step of it, [(setting, ms per step), ...]).  The
-    simulator's state is the same before and after the call.  `snapshot(sim)` makes an
-    object whose restore() puts the state back (default: a device copy)."""
+    def __init__(self, sim, state):
+        self.sim, self.X, self.V = sim, state[0], state[1]
+
+    def restore(self):
+        self.sim.set_state_device(self.X, self.V)
+
+
+def autotune(sim, steps=3, sweeps=2, space=None, key=None, path=DEFAULT_CACHE, log=None, snapshot=None,
+             clock=time.perf_counter, states=None):
+    """Time `steps` steps per setting and keep the fastest.
+
+    The steps start from the current state, or, with `states` (a list of device (X, V)
+    tensor pairs, e.g. snapshots along a trajectory whose character changes), from each
+    of them in turn, and a setting's time is the sum over them.  Returns (best setting,
+    ms per step of it, [(setting, ms per step), ...]).  The simulator's state is the same
+    before and after the call.  `snapshot(sim)` makes an object whose restore() puts the
+    state back (default: a device copy)."""
     space = space or SPACE
     snap = (snapshot or _Snapshot)(sim)
+    starts = [_StateSnapshot(sim, st) for st in states] if states else [snap]
     timed = {}
 
     def measure(cfg):
@@ -101,14 +115,18 @@ def autotune(sim, steps=3, sweeps=2, space=None, key=None, path=DEFAULT_CACHE, l
         if k in timed:
             return timed[k]
         sim.set_tuning(**cfg)
-        snap.restore()
+        starts[0].restore()
         sim.step(1)              # first step after a change also captures the graph
         sim.synchronize()
-        snap.restore()
-        t0 = clock()
-        sim.step(steps)
-        sim.synchronize()
-        ms = (clock() - t0) * 1e3 / steps
+        total = 0.0
+        for st in starts:
+            st.restore()
+            sim.synchronize()
+            t0 = clock()
+            sim.step(steps)
+            sim.synchronize()
+            total += clock() - t0
+        ms = total * 1e3 / (steps * len(starts))
         timed[k] = ms
         if log:
             log(f"[tune] {cfg} {ms:.3f} ms/step")
