@@ -24,8 +24,9 @@ the whole trajectory:
 - the same K states stepped by graph replay, and by the drop-in launchers.
 
 Prints ONE JSON line with the roofline of the dominant kernel, the batched ensemble,
-the adjoint rollout and the CPU oracle timed on this host on a bounded sample of the
-same window (plus the 32k-particle 200-step CPU configuration in full).
+the 8M splash (sampled over its 300 steps the same way), the adjoint rollout and the CPU
+oracle timed on this host on a bounded sample of the same window (plus the
+32k-particle 200-step CPU configuration in full).
 """
 import argparse
 import json
@@ -278,10 +279,68 @@ def bench_batched(count, samples, trajectory, tune_steps=0, grid="dense"):
             "timed_steps": [o for o in offs], "trajectory_steps": trajectory,
             "particle_steps_per_s": round(total * n / t_ens, 1),
             "trajectory_particle_steps_per_s": round(total * trajectory / traj_s, 1),
+            "trajectory_knobs": "the defaults (the pass that took the snapshots, before tuning)",
             "scene_by_scene_particle_steps_per_s": round(total * n / t_seq, 1),
             "speedup_vs_scene_by_scene": round(t_seq / t_ens, 3),
             "concurrent_streams_particle_steps_per_s": round(total * n / t_streams, 1),
             "speedup_vs_concurrent_streams": round(t_streams / t_ens, 3), "tuning": best}
+
+
+def bench_splash(samples, warmup, tune_steps=0, grid="hashed"):
+    """The 8M splash (two 100x100x400 columns, r = 0.004 m, 4 iterations, dt = 1/240,
+    2^24-slot hashed grid) over its configured 300 steps: a graph-replay pass that keeps
+    `samples` stratified snapshots, optionally the tuner over two of them, the trajectory
+    again from the end of the warm-up with the chosen knobs (its mean is the trajectory
+    figure), then one graph-replayed step from each snapshot."""
+    from pbfx.sim import Simulator
+    scene = P.splash_8m(grid=grid)
+    T = scene.steps
+    offs = sample_offsets(T, samples)
+    sim = Simulator.from_scene(scene, grid=grid)
+
+    def trajectory(keep):
+        snaps, done, t = [], 0, 0.0
+        for o in offs + [T]:
+            t0 = time.perf_counter()
+            sim.step(o - done)
+            sim.synchronize()
+            t += time.perf_counter() - t0
+            done = o
+            if keep and o < T:
+                snaps.append(sim.state_device())
+        return snaps, t
+
+    try:
+        n = sim.n
+        sim.step(warmup)
+        sim.synchronize()
+        warm = sim.state_device()
+        snaps, _ = trajectory(True)
+        best = None
+        if tune_steps:
+            from pbfx import tune
+            pick = [snaps[len(snaps) // 4], snaps[(3 * len(snaps)) // 4]]
+            best, _, _ = tune.autotune(sim, steps=tune_steps, key=tune.workload_key(sim, "splash_8m"), states=pick)
+        sim.set_state_device(*warm)
+        sim.step(1)  # (re)captures the graphs for the chosen knobs
+        sim.set_state_device(*warm)
+        sim.synchronize()
+        del warm
+        _, traj_s = trajectory(False)
+        t = sum(graph_step_s(sim, st) for st in snaps)
+        overflow = sim.hash_overflow()
+    finally:
+        sim.close()
+    print(f"[bench] splash {n} particles: {len(snaps)} sampled steps in {t:.4f} s; {T}-step trajectory "
+          f"{traj_s:.3f} s", file=sys.stderr, flush=True)
+    return {"particles": n, "grid": f"{grid} (2^{scene.table_log2} slots)" if grid == "hashed" else grid,
+            "solver_iters": scene.solver_iters, "trajectory_steps": f"{warmup}..{warmup + T}",
+            "timed_steps": [warmup + o for o in offs], "particle_steps_per_s": round(n * len(snaps) / t, 1),
+            "ms_per_step": round(t * 1e3 / len(snaps), 4),
+            "trajectory_particle_steps_per_s": round(n * T / traj_s, 1),
+            "trajectory_ms_per_step": round(traj_s * 1e3 / T, 4), "tuning": best, "hash_overflow": overflow,
+            "timed": "one graph-replayed step from each stratified state (graphs captured beforehand); the "
+                     "trajectory figure is a graph-replay pass over the whole run with the same knobs"}
 
 
 def Simulator_state(sim):
@@ -504,6 +563,8 @@ def main():
     ap.add_argument("--no-drop-in", action="store_true", help="skip timing the drop-in launchers' path")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-samples", type=int, default=5)
+    ap.add_argument("--no-splash", action="store_true", help="skip the 8M splash section")
+    ap.add_argument("--splash-samples", type=int, default=10)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel that "
@@ -663,6 +724,9 @@ def main():
 
     if not a.no_batched:
         out["batched"] = bench_batched(a.batch_scenes, a.batch_samples, 200, 0 if a.no_tune else a.tune_steps)
+
+    if not a.no_splash:
+        out["splash_8m"] = bench_splash(a.splash_samples, W, 0 if (a.no_tune or a.tuning) else a.tune_steps)
 
     if not a.no_adjoint:
         # the dam break's tuned walk knobs as the start; each rollout then tunes the walk

@@ -42,6 +42,10 @@ struct StepTape {
 };
 
 // Scratch of the reverse pass (sorted order unless noted).
+// Words between the per-scene maxima of AdjointScratch::gmax: one 128-byte line per
+// scene, so the atomics of different scenes do not queue on one cache line.
+constexpr uint32_t kGmaxStride = 32;
+
 struct AdjointScratch {
     float4* gPf = nullptr;    // dL/d(final position)
     float4* gVo = nullptr;    // dL/d(velocity after XSPH); w: the recorded XSPH factor
@@ -52,7 +56,7 @@ struct AdjointScratch {
     uint2* lamE = nullptr;    // lambda adjoint: (bits of eps = gLam e0, cap cut-off slot)
     double* partial = nullptr;  // per-block partial sums of dL/dc (one scene)
     double* gcSlot = nullptr;   // ensembles: dL/dc contribution of each sorted slot
-    uint32_t* gmax = nullptr;   // renormalised reverse: bits of max |dL/dX_t|, |dL/dV_t| (one word per scene)
+    uint32_t* gmax = nullptr;   // renormalised reverse: bits of max |dL/dX_t|, |dL/dV_t| (scene q at q * kGmaxStride)
     uint32_t blocks = 0;
 };
 

@@ -119,15 +119,15 @@ __device__ __forceinline__ float3 hess_apply(const Spiky& s, float3 r, float3 v)
 
 // gVo carries the slot's recorded XSPH factor gm = gate m / rho0 in w, so the XSPH
 // adjoint gathers a neighbour's gradient and factor with one load.
-// gmax (renormalised form): the step's running maxima (one per scene), cleared here
-// before k_adj_predict of the same step raises them.
+// gmax (renormalised form): the step's running maxima (one per scene, kGmaxStride
+// words apart), cleared here before k_adj_predict of the same step raises them.
 __global__ __launch_bounds__(kBS) void k_adj_gather(const uint32_t* __restrict__ idxS, const float4* __restrict__ gX,
                                                     const float4* __restrict__ gV, const float* __restrict__ gm,
                                                     float4* __restrict__ gPf, float4* __restrict__ gVo, uint32_t N,
                                                     uint32_t* __restrict__ gmax, uint32_t scenes) {
     const uint32_t s = blockIdx.x * kBS + threadIdx.x;
     if (gmax && s == 0)
-        for (uint32_t q = 0; q < scenes; ++q) gmax[q] = 0u;
+        for (uint32_t q = 0; q < scenes; ++q) gmax[q * kGmaxStride] = 0u;
     if (s >= N) return;
     const uint32_t i = idxS[s];
     gPf[s] = gX[i];
@@ -426,9 +426,9 @@ __global__ __launch_bounds__(kBS) void k_adj_predict(const uint32_t* __restrict_
     const uint32_t q = (sceneOf && s < N) ? sceneOf[s] : first;
     if (__all(q == first)) {
         m = wave_max_u32(m);
-        if ((threadIdx.x & 63u) == 0 && m) atomicMax(gmax + first, m);
+        if ((threadIdx.x & 63u) == 0 && m) atomicMax(gmax + first * kGmaxStride, m);
     } else if (m) {
-        atomicMax(gmax + q, m);  // a wave across a scene boundary
+        atomicMax(gmax + q * kGmaxStride, m);  // a wave across a scene boundary
     }
 }
 
@@ -445,7 +445,7 @@ __global__ __launch_bounds__(kBS) void k_adj_renorm(float4* __restrict__ gX, flo
     const uint32_t i = blockIdx.x * kBS + threadIdx.x;
     if (i >= N) return;
     const uint32_t q = sc.params ? sc.sceneOf[i] : 0u;
-    const uint32_t mb = gmax[q];
+    const uint32_t mb = gmax[q * kGmaxStride];
     if (mb == 0u || mb >= 0x7F800000u) return;
     const int e = min(126, max(-126, (int)(mb >> 23) - 126));
     if (e == 0) return;

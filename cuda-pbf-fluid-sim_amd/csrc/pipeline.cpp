@@ -967,13 +967,13 @@ bool Simulator::adjointInit(int slots) {
     m_adj.blocks = pbfx::ceil_div(cap, 256);
     ok = ok && dalloc(m_adj.gPf, cap) && dalloc(m_adj.gVo, cap) && dalloc(m_adj.gVn, cap) && dalloc(m_adj.gP, cap) &&
          dalloc(m_adj.a, cap) && dalloc(m_adj.lamQ, cap) && dalloc(m_adj.lamE, cap) &&
-         dalloc(m_adj.partial, m_adj.blocks) && dalloc(m_adj.gmax, std::max<size_t>(1, m_scenes.size()));
+         dalloc(m_adj.partial, m_adj.blocks) && dalloc(m_adj.gmax, std::max<size_t>(1, m_scenes.size()) * pbfx::kGmaxStride);
     if (!m_scenes.empty()) ok = ok && dalloc(m_adj.gcSlot, cap);
     if (!ok) {
         releaseAdjoint();
         return false;
     }
-    (void)hipMemset(m_adj.gmax, 0, sizeof(uint32_t) * std::max<size_t>(1, m_scenes.size()));
+    (void)hipMemset(m_adj.gmax, 0, sizeof(uint32_t) * std::max<size_t>(1, m_scenes.size()) * pbfx::kGmaxStride);
     return true;
 }
 
