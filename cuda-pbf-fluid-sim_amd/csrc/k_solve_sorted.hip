@@ -400,65 +400,61 @@ __global__ __launch_bounds__(kBS) void k_finalize_sorted(float4* __restrict__ Ps
     Vs[s] = make_float4((px - x.x) * invDt, (py - x.y) * invDt, (pz - x.z) * invDt, 0.0f);
 }
 
-// Eight waves per SIMD (64 VGPRs): the walk is latency-bound on its candidate and
-// velocity loads, and the extra waves hide more of it than the few spilled registers
-// cost (measured on the collapsed 1M dam break).
+// XSPH's pair term is a few products, so its walk evaluates every candidate in place
+// (inline_walk) rather than queueing the neighbours for a later pass: the sums take
+// the same neighbours in the same order, with every lane busy on its own candidates
+// and no queue to fill and drain.  Positions are read for every candidate of a round,
+// velocities for its neighbours only (the vector memory pipeline, not the VALU, bounds
+// a round that reads both for all four).  Both arrays carry kSlotPad padding slots past
+// the last particle.
+// Eight waves per SIMD (64 VGPRs).
 template <int BS>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8))) void k_xsph_scatter(const float4* __restrict__ Ps, const float4* __restrict__ Vs,
                                                       const uint32_t* __restrict__ segOf, SpanTable st,
-                                                      CandMask mk, int useMask, sim::DeviceParams dp, uint32_t N,
+                                                      sim::DeviceParams dp, uint32_t N,
                                                       const uint32_t* __restrict__ idxS, float4* __restrict__ Xn,
                                                       float4* __restrict__ V, Scenes sc, float* __restrict__ recGm) {
-    __shared__ uint32_t sQ[kQD * BS];
-    __shared__ float4 sBuf[BS];
     const uint32_t s = slot_of_thread<BS>();
-    const bool valid = s < N;
-    const uint32_t ss = valid ? s : N - 1;
-    const float4 pi = Ps[ss];
-    const float4 vi = Vs[ss];
+    if (s >= N) return;
+    const float4 pi = Ps[s];
+    const float4 vi = Vs[s];
     float vx = vi.x, vy = vi.y, vz = vi.z;
     float xc = dp.xsph_c, rest = dp.boundaryRestitution;
     sim::GridBounds grid = dp.grid;
     if (sc.params) {
-        const SceneParams& sp = sc.params[sc.sceneOfSlot[ss]];
+        const SceneParams& sp = sc.params[sc.sceneOfSlot[s]];
         xc = sp.xsph_c;
         rest = sp.restitution;
         grid = sp.grid;
     }
-    // the walk is a wave-level operation: every lane enters it when any lane smooths
-    const bool walk = xc > 0.f || recGm != nullptr;
-    if (__any(walk)) {
-        const uint32_t m = segOf[ss];
+    if (xc > 0.f || recGm != nullptr) {
         const float h2 = dp.kernel.h2, poly6 = dp.kernel.poly6;
-        const SlotBuf PS(Ps, N + kSlotPad), VS(Vs, N);
+        const SlotBuf PS(Ps, N + kSlotPad), VS(Vs, N + kSlotPad);
         float sx = 0.f, sy = 0.f, sz = 0.f;
         int n = 0;
-        auto dist2 = [&](const float4& pj) {
-            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            return pair_r2(rx, ry, rz);
-        };
-        auto apply = [&](uint32_t k) {
-            const float4 pj = PS[k];
-            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            const float hr2 = h2 - pair_r2(rx, ry, rz);
-            const float w = poly6 * hr2 * hr2 * hr2;
-            const float4 vj = VS[k];
-            xsph_pair(sx, sy, sz, vj, vi, w);
-            ++n;
-        };
-        if (useMask)
-            mask_walk<BS>(valid && walk, s, m, st, mk, PS, sQ + threadIdx.x, 0, h2, dist2, apply);
-        else
-            neighbor_walk<false, BS>(valid && walk, s, SpanRows(st, m), mk, SortedFetch{PS}, sBuf + (threadIdx.x & ~63u),
-                                 sQ + threadIdx.x, 0, h2, dist2, apply);
-        if (recGm && valid)
+        inline_walk(
+            true, s, SpanRows(st, segOf[s]), h2, [&](uint32_t k, uint32_t u) { return PS.at(k, u); },
+            [&](const float4& pj) { return pair_r2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z); },
+            // a non-neighbour's velocity read is sent past the end of the buffer, where the
+            // hardware returns zeros without touching memory (and take() discards it)
+            [&](uint32_t k, uint32_t u, bool in) { return VS.at(in ? k : kSlotBufBeyond, u); },
+            [&](const float4&, const float4& vj, float r2, bool in) {
+                const float hr2 = h2 - r2;
+                const float w = poly6 * hr2 * hr2 * hr2;
+                float tx = sx, ty = sy, tz = sz;
+                xsph_pair(tx, ty, tz, vj, vi, w);
+                sx = in ? tx : sx;
+                sy = in ? ty : sy;
+                sz = in ? tz : sz;
+                n += in ? 1 : 0;
+            });
+        if (recGm)
             recGm[s] = xsph_gate(n, dp) * ((dp.restDensity > 0.f) ? (dp.particleMass / dp.restDensity) : 0.f);
         if (xc > 0.f) {
             const float3 v = xsph_finish(vi.x, vi.y, vi.z, sx, sy, sz, n, dp, xc);
             vx = v.x; vy = v.y; vz = v.z;
         }
     }
-    if (!valid) return;
     // BoundaryOp with restitution (post_ops.cpp:25-29, boundary.cu:11-25)
     float px = pi.x, py = pi.y, pz = pi.z;
     clamp_box(px, py, pz, vx, vy, vz, grid, clamp_restitution(rest));
@@ -558,17 +554,13 @@ void launch_finalize_sorted(float4* Ps, const float4* X, const uint32_t* idxS, f
 }
 
 void launch_xsph_scatter(const float4* Ps, const float4* Vs, const uint32_t* segOf, const SpanTable& st,
-                         const CandMask& mk, bool useMask, const sim::DeviceParams& dp, uint32_t N,
-                         const uint32_t* idxS, float4* Xn, float4* V, const Scenes& sc, hipStream_t s,
-                         float* recGm) {
+                         const sim::DeviceParams& dp, uint32_t N, const uint32_t* idxS, float4* Xn, float4* V,
+                         const Scenes& sc, hipStream_t s, float* recGm) {
     if (N == 0) return;
-    const int um = useMask ? 1 : 0;
     if (st.block == 128)
-        k_xsph_scatter<128><<<ceil_div(N, 128u), 128, 0, s>>>(Ps, Vs, segOf, st, mk, um, dp, N, idxS, Xn, V, sc,
-                                                              recGm);
+        k_xsph_scatter<128><<<ceil_div(N, 128u), 128, 0, s>>>(Ps, Vs, segOf, st, dp, N, idxS, Xn, V, sc, recGm);
     else
-        k_xsph_scatter<256><<<ceil_div(N, 256u), 256, 0, s>>>(Ps, Vs, segOf, st, mk, um, dp, N, idxS, Xn, V, sc,
-                                                              recGm);
+        k_xsph_scatter<256><<<ceil_div(N, 256u), 256, 0, s>>>(Ps, Vs, segOf, st, dp, N, idxS, Xn, V, sc, recGm);
     check_launch("xsph_scatter");
 }
 

@@ -207,16 +207,13 @@ void SortedSolve(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream
 
 void SortedPost(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_t s) {
     const int fin = p.solverIters > 0 ? (p.solverIters & 1) : 0;
-    // XSPH walks its candidates directly: each of its pairs also gathers a velocity,
-    // and the scattered loads of the candidate-mask walk cost more than the
-    // consecutive candidates the mask would skip
-    pbfx::CandMask mk = g.mask;
-    const bool useMask = false;
+    // XSPH walks its candidates directly, evaluating each in place: each of its pairs
+    // also gathers a velocity, and the scattered loads of the candidate-mask walk cost
+    // more than the consecutive candidates the mask would skip
     const bool rec = b.rec.P[0] != nullptr;
     pbfx::launch_xsph_scatter(rec ? b.rec.P[p.solverIters] : b.d_pos_sorted[fin], rec ? b.rec.Vs : b.d_vel_sorted,
-                              g.d_segOf, g.spans, mk, useMask,
-                              MakeDeviceParams(p), p.numParticles, g.d_indices_sorted, b.d_pos_pred, b.d_vel,
-                              g.scenes, s, rec ? b.rec.xsphGm : nullptr);
+                              g.d_segOf, g.spans, MakeDeviceParams(p), p.numParticles, g.d_indices_sorted,
+                              b.d_pos_pred, b.d_vel, g.scenes, s, rec ? b.rec.xsphGm : nullptr);
     mark("xsph_scatter");
 }
 
@@ -250,7 +247,8 @@ bool DeviceBuffers::allocate(uint32_t cap) {
     bool ok = dalloc(d_pos_curr, cap) && dalloc(d_pos_next, cap) && dalloc(d_vel, cap) && dalloc(d_lambda, cap) &&
               dalloc(d_delta, cap) && dalloc(d_vel_snapshot, cap) &&
               dalloc(d_pos_sorted[0], (size_t)cap + pbfx::kSlotPad) &&
-              dalloc(d_pos_sorted[1], (size_t)cap + pbfx::kSlotPad) && dalloc(d_vel_sorted, cap);
+              dalloc(d_pos_sorted[1], (size_t)cap + pbfx::kSlotPad) &&
+              dalloc(d_vel_sorted, (size_t)cap + pbfx::kSlotPad);
     if (!ok) {
         release();
         return false;
@@ -263,6 +261,7 @@ bool DeviceBuffers::allocate(uint32_t cap) {
     (void)hipMemset(d_vel, 0, sizeof(float4) * cap);
     (void)hipMemset(d_pos_sorted[0], 0, sizeof(float4) * ((size_t)cap + pbfx::kSlotPad));
     (void)hipMemset(d_pos_sorted[1], 0, sizeof(float4) * ((size_t)cap + pbfx::kSlotPad));
+    (void)hipMemset(d_vel_sorted, 0, sizeof(float4) * ((size_t)cap + pbfx::kSlotPad));
     return true;
 }
 
@@ -954,9 +953,10 @@ bool Simulator::adjointInit(int slots) {
     bool ok = true;
     for (auto& t : m_tapes) {
         ok = ok && dalloc(t.X, cap) && dalloc(t.V, cap) && dalloc(t.idxS, cap) && dalloc(t.segOf, cap) &&
-             dalloc(t.spans, spanWords) && dalloc(t.Vs, cap) && dalloc(t.predClamp, cap) &&
+             dalloc(t.spans, spanWords) && dalloc(t.Vs, (size_t)cap + pbfx::kSlotPad) && dalloc(t.predClamp, cap) &&
              dalloc(t.finalClamp, cap) && dalloc(t.xsphGm, cap);
         if (!m_scenes.empty()) ok = ok && dalloc(t.sceneOfSlot, cap);
+        if (ok) (void)hipMemset(t.Vs, 0, sizeof(float4) * ((size_t)cap + pbfx::kSlotPad));
         for (int it = 0; ok && it <= m_params.solverIters; ++it) {
             ok = dalloc(t.P[it], (size_t)cap + pbfx::kSlotPad);
             if (ok) (void)hipMemset(t.P[it], 0, sizeof(float4) * ((size_t)cap + pbfx::kSlotPad));

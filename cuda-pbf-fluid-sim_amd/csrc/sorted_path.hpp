@@ -133,8 +133,7 @@ void launch_finalize_sorted(float4* Ps, const float4* X, const uint32_t* idxS, f
 // scatter Xn[idxS[s]] = (P, 1) and V[idxS[s]] = (v, 0).  Recording (non-null recGm):
 // recGm[s] = gate * m / rho0, walked also when xsph_c is 0.
 void launch_xsph_scatter(const float4* Ps, const float4* Vs, const uint32_t* segOf, const SpanTable& st,
-                         const CandMask& mk, bool useMask, const sim::DeviceParams& dp, uint32_t N,
-                         const uint32_t* idxS, float4* Xn, float4* V, const Scenes& sc, hipStream_t s,
-                         float* recGm = nullptr);
+                         const sim::DeviceParams& dp, uint32_t N, const uint32_t* idxS, float4* Xn, float4* V,
+                         const Scenes& sc, hipStream_t s, float* recGm = nullptr);
 
 }  // namespace pbfx

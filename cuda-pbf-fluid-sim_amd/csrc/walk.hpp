@@ -18,6 +18,9 @@ constexpr int kBS = 256;  // lanes per block of the kernels that walk (default)
 // slots become immediate offsets of one address.  Reads at or past `slots` return
 // zeros.  Arrays of up to kMaxSlotBufSlots slots (the offsets are 32-bit).
 constexpr uint32_t kMaxSlotBufSlots = 1u << 27;
+// A slot index whose reads fall past the end of any SlotBuf (its byte offset lies
+// beyond every array of at most kMaxSlotBufSlots slots).
+constexpr uint32_t kSlotBufBeyond = 0x0FFFFFF0u;
 typedef int SlotBufV4 __attribute__((ext_vector_type(4)));
 struct SlotBuf {
     __amdgpu_buffer_rsrc_t r;
@@ -387,6 +390,44 @@ __device__ __forceinline__ void mask_walk(bool valid, uint32_t s, uint32_t m, co
         }
     }
     pop_above(0);
+}
+
+// Walk without the pair queue, for a pair term cheap enough to evaluate on every
+// candidate (XSPH's: a few products against the neighbour's velocity).  Each lane
+// walks its own spans row by row, kUnroll candidates a round: load(k, u) reads the
+// positions of slots k + u together, then gather(k, u, in) reads what the term needs
+// besides, for the neighbours only (in: in the span, not the lane's own slot,
+// r2 <= h2), and take() gets every candidate in walk order with its flag and selects
+// the neighbours' terms into its sums.  No lane waits on another, so a lane whose
+// spans end early simply stops.  Slot k + u may lie up to kUnroll - 1 slots past the
+// span and past the last particle.
+template <class Rows, class Load, class Dist2, class Gather, class Take>
+__device__ __forceinline__ void inline_walk(bool valid, uint32_t s, const Rows& rows, float h2, Load&& load,
+                                            Dist2&& dist2, Gather&& gather, Take&& take) {
+    if (!valid) return;
+    const int nrows = rows.count();
+    for (int r = 0; r < nrows; ++r) {
+        uint32_t lo = 0, hi = 0;
+        rows.span(r, lo, hi);
+        for (uint32_t k = lo; k < hi; k += kUnroll) {
+            const uint32_t cnt = min(kUnroll, hi - k);
+            decltype(load(k, 0u)) c[kUnroll];
+            decltype(gather(k, 0u, true)) g[kUnroll];
+            float r2[kUnroll];
+            bool in[kUnroll];
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) c[u] = load(k, u);
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) {
+                r2[u] = dist2(c[u]);
+                in[u] = (u < cnt) & (k + u != s) & (r2[u] <= h2);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) g[u] = gather(k, u, in[u]);
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) take(c[u], g[u], r2[u], in[u]);
+        }
+    }
 }
 
 // Rows of a sorted slot's neighbourhood from the per-step span table.

