@@ -66,8 +66,8 @@ struct IndexedFetch {
 };
 
 // Every kernel below runs one lane per sorted slot and walks the slot's neighbours
-// with the shared wave walk; lanes past N take part in the walk's wave-level steps
-// with no rows.  The neighbour kernels deal their 256-slot tiles to the XCDs in
+// with the shared wave walk (the staged XSPH with the queue-free inline walk); lanes
+// past N take part in the walk's wave-level steps with no rows.  The neighbour kernels deal their 256-slot tiles to the XCDs in
 // contiguous runs (xcd_tile), so a tile and its neighbours' tiles share an L2: 12%
 // faster drop-in steps on the 1M dam break.
 //
@@ -174,8 +174,6 @@ __global__ __launch_bounds__(kBS) void k_xsph_ref(float4* __restrict__ Vout, con
                                                   const float4* __restrict__ Vs, const uint32_t* __restrict__ idxS,
                                                   const uint32_t* __restrict__ keyS, Lookup lookup,
                                                   sim::DeviceParams dp, uint32_t N) {
-    __shared__ uint32_t sQ[kQD * kBS];
-    __shared__ float4 sBuf[kBS];
     const uint32_t s = xcd_tile(blockIdx.x, gridDim.x) * kBS + threadIdx.x;
     const bool valid = s < N;
     const uint32_t ss = valid ? s : N - 1;
@@ -190,23 +188,38 @@ __global__ __launch_bounds__(kBS) void k_xsph_ref(float4* __restrict__ Vout, con
         return pair_r2(rx, ry, rz);
     };
     const CellRows<Lookup> rows(lookup, dp.grid, reach_of(dp.kernel.h, dp.grid.cellSize), keyS[ss]);
-    auto walk = [&](const auto& fetch) {
+    if constexpr (kStaged) {
+        // the staged copies are in slot order: XSPH's walk evaluates every candidate in
+        // place, as the cell-reordered kernel does (no queue)
+        const SlotBuf PS(Ps, N + kSlotPad), VS(Vs, N + kSlotPad);
+        inline_walk(
+            valid, s, rows, h2, [&](uint32_t k, uint32_t u) { return PS.at(k, u); }, dist2,
+            [&](uint32_t k, uint32_t u, bool in) { return VS.at(in ? k : kSlotBufBeyond, u); },
+            [&](const float4&, const float4& vj, float r2, bool in) {
+                const float hr2 = h2 - r2;
+                const float w = poly6 * hr2 * hr2 * hr2;
+                float tx = sx, ty = sy, tz = sz;
+                xsph_pair(tx, ty, tz, vj, vi, w);
+                sx = in ? tx : sx;
+                sy = in ? ty : sy;
+                sz = in ? tz : sz;
+                n += in ? 1 : 0;
+            });
+    } else {
+        __shared__ uint32_t sQ[kQD * kBS];
+        __shared__ float4 sBuf[kBS];
         auto apply = [&](uint32_t j) {
-            const float4 pj = fetch.at(j);
+            const float4 pj = P[j];
             const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
             const float hr2 = h2 - pair_r2(rx, ry, rz);
             const float w = poly6 * hr2 * hr2 * hr2;
-            const float4 vj = kStaged ? Vs[j] : Vin[j];
+            const float4 vj = Vin[j];
             xsph_pair(sx, sy, sz, vj, vi, w);
             ++n;
         };
-        neighbor_walk<false, kBS>(valid, s, rows, CandMask{}, fetch, sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x,
-                                  0, h2, dist2, apply);
-    };
-    if (kStaged)
-        walk(SortedFetch{SlotBuf(Ps, N + kSlotPad)});
-    else
-        walk(IndexedFetch{P, idxS});
+        neighbor_walk<false, kBS>(valid, s, rows, CandMask{}, IndexedFetch{P, idxS}, sBuf + (threadIdx.x & ~63u),
+                                  sQ + threadIdx.x, 0, h2, dist2, apply);
+    }
     if (!valid) return;
     const float3 v = xsph_finish(vi.x, vi.y, vi.z, sx, sy, sz, n, dp, dp.xsph_c);
     Vout[i] = make_float4(v.x, v.y, v.z, 0.0f);
