@@ -77,7 +77,7 @@ INSTR_FILE = os.path.join(REPO, "profiles", "instructions_dam_break_1m.json")
 # Launch knobs of the neighbour walk, which change what its kernels execute and read;
 # profiled counts hold only for the setting they were measured at.  (The sort's knobs
 # do not touch these kernels.)
-WALK_KNOBS = ("candidate_mask", "stream_ratio", "chunk_bits", "walk_block")
+WALK_KNOBS = ("candidate_mask", "stream_ratio", "chunk_bits", "walk_block", "lane_order")
 
 
 def sample_offsets(total, k):

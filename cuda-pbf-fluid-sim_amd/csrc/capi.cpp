@@ -65,7 +65,7 @@ extern "C" int pbfSimPublishVelocity(PbfSim* sim, uint32_t count) {
 extern "C" int pbfSimSetTuning(PbfSim* sim, const PbfSimTuning* t) {
     if (!sim || !t) return -1;
     const sim::Simulator::Tuning k{t->candidateMask != 0, t->streamRatio, t->chunkBits != 0, t->sortItems,
-                              t->sortDigitBits,      t->walkBlock};
+                              t->sortDigitBits,      t->walkBlock,  t->laneOrder != 0};
     return sim->s.setTuning(k) ? 0 : -1;
 }
 
@@ -78,6 +78,7 @@ extern "C" int pbfSimGetTuning(PbfSim* sim, PbfSimTuning* t) {
     t->sortItems = k.sortItems;
     t->sortDigitBits = k.sortDigitBits;
     t->walkBlock = k.walkBlock;
+    t->laneOrder = k.laneOrder ? 1 : 0;
     return 0;
 }
 

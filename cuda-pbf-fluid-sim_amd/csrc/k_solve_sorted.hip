@@ -282,14 +282,63 @@ struct HashRowsDev {
     }
 };
 
+// Lane order of the neighbour passes.  Pair pops serve every lane of a wave that holds
+// a queued pair, and a particle's neighbours arrive in bursts that follow its place
+// inside its cell (one near a cell's top meets most of them in the rows above).  Each
+// group of kLaneGroup slots is therefore dealt to its waves by that place (z, then y,
+// in quarters of a cell), stable in slot order within a class, so the lanes of a wave
+// burst together and a pop finds more of them busy.  perm[t] is the slot lane t of
+// the neighbour passes takes; every slot of a group stays in the group.  Only the
+// assignment of particles to lanes changes, never a particle's sums.  Groups of 128
+// slots (two waves) measured best on the collapsed 1M dam break: 256 let the waves'
+// candidate loads spread over more cells and slowed the lambda passes.
+constexpr uint32_t kLaneGroup = 128;
+template <uint32_t G>
+__global__ __launch_bounds__(G) void k_lane_order(const float4* __restrict__ Ps, uint32_t N, sim::GridBounds g,
+                                                  Scenes sc, uint32_t* __restrict__ perm) {
+    constexpr uint32_t W = G / 64u;
+    __shared__ uint32_t counts[W][16];
+    const uint32_t t = blockIdx.x * G + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    uint32_t cls = 16u;  // lanes past N sort after every class and are not placed
+    if (t < N) {
+        const float4 p = Ps[t];
+        const sim::GridBounds& gb = sc.params ? sc.params[sc.sceneOfSlot[t]].grid : g;
+        const float inv = 1.0f / gb.cellSize;
+        const float fz = (p.z - gb.mins.z) * inv, fy = (p.y - gb.mins.y) * inv;
+        const uint32_t qz = (uint32_t)((fz - floorf(fz)) * 4.0f) & 3u;
+        const uint32_t qy = (uint32_t)((fy - floorf(fy)) * 4.0f) & 3u;
+        cls = qz * 4u + qy;
+    }
+    uint32_t rank = 0;
+    for (uint32_t c = 0; c < 16u; ++c) {
+        const uint64_t b = __ballot(cls == c);
+        if (cls == c) rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+        if (lane == 0) counts[w][c] = (uint32_t)__popcll(b);
+    }
+    __syncthreads();
+    if (t >= N) return;
+    uint32_t off = 0;
+    for (uint32_t c = 0; c < cls; ++c)
+        for (uint32_t v = 0; v < W; ++v) off += counts[v][c];
+    for (uint32_t v = 0; v < w; ++v) off += counts[v][cls];
+    perm[blockIdx.x * G + off + rank] = t;
+}
+
+// The slot a lane of the neighbour passes takes: its own, or perm's.
+__device__ __forceinline__ uint32_t lane_slot(uint32_t t, uint32_t N, const uint32_t* __restrict__ perm) {
+    return (perm && t < N) ? perm[t] : t;
+}
+
 // Lambda of a sorted slot: the full walk (first iteration) or the mask walk.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_lambda_sorted(float4* Ps, const uint32_t* __restrict__ segOf, SpanTable st,
                                                        CandMask mk, int useMask, sim::DeviceParams dp, uint32_t N,
-                                                       float4* __restrict__ recA, uint2* __restrict__ recB) {
+                                                       float4* __restrict__ recA, uint2* __restrict__ recB,
+                                                       const uint32_t* __restrict__ perm) {
     __shared__ uint32_t sQ[kQD * BS];
     __shared__ float4 sBuf[BS];
-    const uint32_t s = slot_of_thread<BS>();
+    const uint32_t s = lane_slot(slot_of_thread<BS>(), N, perm);
     const bool valid = s < N;
     const uint32_t ss = valid ? s : N - 1;
     const float4 pi = Ps[ss];
@@ -335,10 +384,10 @@ __global__ __launch_bounds__(BS) void k_delta_sorted(const float4* __restrict__ 
                                                       int last, const float4* __restrict__ X,
                                                       const uint32_t* __restrict__ idxS, float4* __restrict__ Vs,
                                                       float invDt, Scenes sc, uint32_t* __restrict__ clampBits,
-                                                      float4* __restrict__ recU) {
+                                                      float4* __restrict__ recU, const uint32_t* __restrict__ perm) {
     __shared__ uint32_t sQ[kQD * BS];
     __shared__ float4 sBuf[BS];
-    const uint32_t s = slot_of_thread<BS>();
+    const uint32_t s = lane_slot(slot_of_thread<BS>(), N, perm);
     const bool valid = s < N;
     const uint32_t ss = valid ? s : N - 1;
     const float4 pi = Pa[ss];
@@ -516,7 +565,7 @@ void launch_spans_gather(CellLookup lookup, const float4* Xn, const uint32_t* id
 
 void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st, const CandMask& mk,
                           bool useMask, const sim::DeviceParams& dp, uint32_t N, hipStream_t s, float4* recA,
-                          uint2* recB) {
+                          uint2* recB, const uint32_t* perm) {
     if (N == 0) return;
     if ((recA == nullptr) != (recB == nullptr)) {
         set_error(PBFX_E_INVALID_ARGUMENT, "launch_lambda_sorted: recA and recB go together");
@@ -524,25 +573,32 @@ void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st
     }
     const int um = useMask ? 1 : 0;
     if (st.block == 128)
-        k_lambda_sorted<128><<<ceil_div(N, 128u), 128, 0, s>>>(Ps, segOf, st, mk, um, dp, N, recA, recB);
+        k_lambda_sorted<128><<<ceil_div(N, 128u), 128, 0, s>>>(Ps, segOf, st, mk, um, dp, N, recA, recB, perm);
     else
-        k_lambda_sorted<256><<<ceil_div(N, 256u), 256, 0, s>>>(Ps, segOf, st, mk, um, dp, N, recA, recB);
+        k_lambda_sorted<256><<<ceil_div(N, 256u), 256, 0, s>>>(Ps, segOf, st, mk, um, dp, N, recA, recB, perm);
     check_launch("lambda_sorted");
 }
 
 void launch_delta_sorted(const float4* Pa, float4* Pb, const uint32_t* segOf, const SpanTable& st,
                          const CandMask& mk, int maskMode, const sim::DeviceParams& dp, uint32_t N, bool last,
                          const float4* X, const uint32_t* idxS, float4* Vs, float invDt, const Scenes& sc,
-                         hipStream_t s, uint32_t* clampBits, float4* recU) {
+                         hipStream_t s, uint32_t* clampBits, float4* recU, const uint32_t* perm) {
     if (N == 0) return;
     const int lastPass = last ? 1 : 0;
     if (st.block == 128)
         k_delta_sorted<128><<<ceil_div(N, 128u), 128, 0, s>>>(Pa, Pb, segOf, st, mk, maskMode, dp, N, lastPass, X,
-                                                              idxS, Vs, invDt, sc, clampBits, recU);
+                                                              idxS, Vs, invDt, sc, clampBits, recU, perm);
     else
         k_delta_sorted<256><<<ceil_div(N, 256u), 256, 0, s>>>(Pa, Pb, segOf, st, mk, maskMode, dp, N, lastPass, X,
-                                                              idxS, Vs, invDt, sc, clampBits, recU);
+                                                              idxS, Vs, invDt, sc, clampBits, recU, perm);
     check_launch("delta_sorted");
+}
+
+void launch_lane_order(const float4* Ps, uint32_t N, const sim::GridBounds& grid, const Scenes& sc, uint32_t* perm,
+                       hipStream_t s) {
+    if (N == 0) return;
+    k_lane_order<kLaneGroup><<<ceil_div(N, kLaneGroup), kLaneGroup, 0, s>>>(Ps, N, grid, sc, perm);
+    check_launch("lane_order");
 }
 
 void launch_finalize_sorted(float4* Ps, const float4* X, const uint32_t* idxS, float4* Vs,

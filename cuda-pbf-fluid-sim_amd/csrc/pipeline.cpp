@@ -171,6 +171,10 @@ void SortedGridBuild(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipSt
                               g.d_cellOffsets, g.d_compactCount, g.d_cellStart, g.d_cellEnd, g.table, g.spans,
                               p.grid, g.scenes, s);
     mark("spans_gather");
+    if (g.laneOrder) {
+        pbfx::launch_lane_order(b.rec.P[0] ? b.rec.P[0] : b.d_pos_sorted[0], N, p.grid, g.scenes, g.d_laneOrder, s);
+        mark("lane_order");
+    }
 }
 
 void SortedSolve(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_t s) {
@@ -188,18 +192,19 @@ void SortedSolve(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream
     mk.rs2 = pbfx::mask_skin_radius2(p);
     const bool useMask = g.candidateMask && g.mask.words != nullptr && mk.rs2 > 0.f;
     const bool rec = b.rec.P[0] != nullptr;
+    const uint32_t* perm = g.laneOrder ? g.d_laneOrder : nullptr;
     int cur = 0;
     for (int it = 0; it < p.solverIters; ++it) {
         float4* Pa = rec ? b.rec.P[it] : b.d_pos_sorted[cur];
         float4* Pb = rec ? b.rec.P[it + 1] : b.d_pos_sorted[cur ^ 1];
         const bool last = it == p.solverIters - 1;
         pbfx::launch_lambda_sorted(Pa, g.d_segOf, g.spans, mk, useMask && it > 0, dp, N, s,
-                                   rec ? b.rec.lamA[it] : nullptr, rec ? b.rec.lamB[it] : nullptr);
+                                   rec ? b.rec.lamA[it] : nullptr, rec ? b.rec.lamB[it] : nullptr, perm);
         mark("lambda");
         const int maskMode = !useMask ? 0 : (it == 0 ? 1 : 2);
         pbfx::launch_delta_sorted(Pa, Pb, g.d_segOf, g.spans, mk, maskMode, dp, N, last, b.d_pos, g.d_indices_sorted,
                                   rec ? b.rec.Vs : b.d_vel_sorted, invDt, g.scenes, s,
-                                  last ? b.rec.finalClamp : nullptr, rec ? b.rec.dispU[it] : nullptr);
+                                  last ? b.rec.finalClamp : nullptr, rec ? b.rec.dispU[it] : nullptr, perm);
         mark(last ? "delta_last" : "delta");
         cur ^= 1;
     }
@@ -317,7 +322,7 @@ bool GridBuffers::allocate(uint32_t cap, uint32_t nCells, GridKind k, uint32_t t
          dalloc(spans.spans, (size_t)spans.rows * cap);
     if (reordered) {
         mask.stride = cap;
-        ok = ok && dalloc(mask.words, (size_t)pbfx::kCandMaskWords * cap);
+        ok = ok && dalloc(mask.words, (size_t)pbfx::kCandMaskWords * cap) && dalloc(d_laneOrder, cap);
         if (k == GridKind::Hashed) {
             tableLog2 = tLog2;
             const size_t slots = size_t(1) << tLog2;
@@ -344,7 +349,7 @@ bool GridBuffers::allocate(uint32_t cap, uint32_t nCells, GridKind k, uint32_t t
 void GridBuffers::release() {
     dfree(d_cellKeys); dfree(d_cellKeys_sorted); dfree(d_indices); dfree(d_indices_sorted);
     dfree(d_cellStart); dfree(d_cellEnd); dfree(d_cellUniqueKeys); dfree(d_cellOffsets); dfree(d_compactCount);
-    dfree(d_segOf); dfree(d_blockCounts); dfree(d_blockOffsets);
+    dfree(d_segOf); dfree(d_blockCounts); dfree(d_blockOffsets); dfree(d_laneOrder);
     if (d_sortTemp) (void)hipFree(d_sortTemp);
     d_sortTemp = nullptr;
     sortTempBytes = 0;
@@ -628,7 +633,7 @@ bool Simulator::restructure(const SimParams& p) {
         }
     }
     const pbfx::SpanTable keepSpans = m_grid.spans;
-    const bool keepCandidate = m_grid.candidateMask;
+    const bool keepCandidate = m_grid.candidateMask, keepLaneOrder = m_grid.laneOrder;
     const int items = m_grid.sortItems, digits = m_grid.sortDigitBits;
     if (!m_grid.allocate(cap, (uint32_t)cells, m_opt.grid, m_opt.tableLog2, reach_for(p),
                          m_opt.path == PathKind::Reordered)) {
@@ -637,6 +642,7 @@ bool Simulator::restructure(const SimParams& p) {
     }
     // launch knobs survive the re-allocation
     m_grid.candidateMask = keepCandidate;
+    m_grid.laneOrder = keepLaneOrder;
     m_grid.sortItems = items;
     m_grid.sortDigitBits = digits;
     m_grid.spans.streamRatio = keepSpans.streamRatio;
@@ -836,6 +842,7 @@ bool Simulator::setTuning(const Tuning& t) {
     m_grid.sortItems = t.sortItems;
     m_grid.sortDigitBits = t.sortDigitBits == 0 ? 8 : t.sortDigitBits;
     m_grid.candidateMask = t.candidateMask;
+    m_grid.laneOrder = t.laneOrder;
     m_grid.spans.streamRatio = t.streamRatio;
     m_grid.spans.chunkBits = t.chunkBits ? 1u : 0u;
     m_grid.spans.block = t.walkBlock == 0 ? 256u : (uint32_t)t.walkBlock;

@@ -81,6 +81,8 @@ struct GridBuffers {
     pbfx::SpanTable spans{};
     pbfx::CandMask mask{};  // per-step candidate mask (reordered path)
     bool candidateMask = true;  // later passes walk the mask (tuning knob)
+    uint32_t* d_laneOrder = nullptr;  // lane -> slot of the lambda and delta passes (reordered path)
+    bool laneOrder = true;            // use d_laneOrder (tuning knob)
     int sortItems = 0;          // radix sort keys per thread (0: by N; tuning knob)
     int sortDigitBits = 8;      // radix sort widest digit of a pass (4..8 bits; tuning knob)
     pbfx::HashTable table{};
@@ -217,11 +219,13 @@ public:
         int sortItems;
         int sortDigitBits;
         int walkBlock;
+        bool laneOrder;
     };
     bool setTuning(const Tuning& t);
     Tuning tuning() const {
         return Tuning{m_grid.candidateMask, m_grid.spans.streamRatio, m_grid.spans.chunkBits != 0,
-                      m_grid.sortItems,     m_grid.sortDigitBits,     (int)m_grid.spans.block};
+                      m_grid.sortItems,     m_grid.sortDigitBits,     (int)m_grid.spans.block,
+                      m_grid.laneOrder};
     }
     uint32_t sceneCount() const { return (uint32_t)m_scenes.size(); }
     const pbfx::SceneParams* sceneParams(uint32_t scene) const {

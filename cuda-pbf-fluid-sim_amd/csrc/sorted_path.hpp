@@ -114,7 +114,7 @@ void launch_spans_gather(CellLookup lookup, const float4* Xn, const uint32_t* id
 // coefficients (lambda_adjoint_coefs).
 void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st, const CandMask& mk,
                           bool useMask, const sim::DeviceParams& dp, uint32_t N, hipStream_t s,
-                          float4* recA = nullptr, uint2* recB = nullptr);
+                          float4* recA = nullptr, uint2* recB = nullptr, const uint32_t* perm = nullptr);
 
 // Pb[s] = Pa[s] + delta(s).  On the last iteration also clamps to the box and
 // writes Vs[s] = (Pb[s] - X[idxS[s]]) * invDt.  maskMode 0: full walk; 1: full
@@ -123,7 +123,13 @@ void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st
 void launch_delta_sorted(const float4* Pa, float4* Pb, const uint32_t* segOf, const SpanTable& st,
                          const CandMask& mk, int maskMode, const sim::DeviceParams& dp, uint32_t N, bool last,
                          const float4* X, const uint32_t* idxS, float4* Vs, float invDt, const Scenes& sc,
-                         hipStream_t s, uint32_t* clampBits = nullptr, float4* recU = nullptr);
+                         hipStream_t s, uint32_t* clampBits = nullptr, float4* recU = nullptr,
+                         const uint32_t* perm = nullptr);
+
+// perm[t] = the slot lane t of the lambda and delta passes takes: each 128-slot group
+// dealt to its two waves by the particles' place inside their cells (positions Ps).
+void launch_lane_order(const float4* Ps, uint32_t N, const sim::GridBounds& grid, const Scenes& sc, uint32_t* perm,
+                       hipStream_t s);
 
 // No solver iterations: clamp Ps and compute Vs (the post-solve boundary and velocity phases).
 void launch_finalize_sorted(float4* Ps, const float4* X, const uint32_t* idxS, float4* Vs,

@@ -12,7 +12,8 @@ PATHS = {"reordered": 0, "reference": 1}
 
 class _Tuning(ctypes.Structure):
     _fields_ = [("candidateMask", ctypes.c_int), ("streamRatio", ctypes.c_uint32), ("chunkBits", ctypes.c_int),
-                ("sortItems", ctypes.c_int), ("sortDigitBits", ctypes.c_int), ("walkBlock", ctypes.c_int)]
+                ("sortItems", ctypes.c_int), ("sortDigitBits", ctypes.c_int), ("walkBlock", ctypes.c_int),
+                ("laneOrder", ctypes.c_int)]
 
 
 class _CudaArray:
@@ -112,7 +113,7 @@ class Simulator:
             raise _lib.PbfError("publish_velocity failed")
 
     def set_tuning(self, candidate_mask=None, stream_ratio=None, chunk_bits=None, sort_items=None,
-                   sort_digit_bits=None, walk_block=None):
+                   sort_digit_bits=None, walk_block=None, lane_order=None):
         """Change launch knobs (results are bit-identical for every setting)."""
         t = self.tuning()
         if candidate_mask is not None:
@@ -127,8 +128,10 @@ class Simulator:
             t["sort_digit_bits"] = int(sort_digit_bits)
         if walk_block is not None:
             t["walk_block"] = int(walk_block)
+        if lane_order is not None:
+            t["lane_order"] = bool(lane_order)
         c = _Tuning(int(t["candidate_mask"]), t["stream_ratio"], int(t["chunk_bits"]), t["sort_items"],
-                    t["sort_digit_bits"], t["walk_block"])
+                    t["sort_digit_bits"], t["walk_block"], int(t["lane_order"]))
         if _lib.lib().pbfSimSetTuning(self._h, ctypes.byref(c)) != 0:
             _lib.check("pbfSimSetTuning")
             raise _lib.PbfError("set_tuning failed")
@@ -139,7 +142,8 @@ class Simulator:
             raise _lib.PbfError("get_tuning failed")
         return {"candidate_mask": bool(c.candidateMask), "stream_ratio": int(c.streamRatio),
                 "chunk_bits": bool(c.chunkBits), "sort_items": int(c.sortItems),
-                "sort_digit_bits": int(c.sortDigitBits), "walk_block": int(c.walkBlock)}
+                "sort_digit_bits": int(c.sortDigitBits), "walk_block": int(c.walkBlock),
+                "lane_order": bool(c.laneOrder)}
 
     def positions_view(self):
         """(N, 4) float32 CUDA tensor viewing the current position buffer (no copy).  The

@@ -22,6 +22,7 @@ SPACE = {
     "sort_items": (0, 4, 8, 16),
     "sort_digit_bits": (8, 6, 4),
     "walk_block": (256, 128),
+    "lane_order": (True, False),
 }
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "pbfx", "tune.json")

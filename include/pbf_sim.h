@@ -140,6 +140,9 @@ typedef struct {
                            // are spread evenly over ceil(bits / sortDigitBits) passes
     int walkBlock;         // lanes per block of the neighbour kernels: 128 or 256 (0 = 256)
                            // (pbfSimGetTuning reports the effective values: 8 and 256 for 0)
+    int laneOrder;         // 1: the lambda and delta passes deal each 128-slot group to its two
+                           // waves by the particles' place inside their cells, so a wave's
+                           // lanes meet their neighbours together; 0: lane i takes slot i
 } PbfSimTuning;
 int pbfSimSetTuning(PbfSim* sim, const PbfSimTuning* tuning);
 int pbfSimGetTuning(PbfSim* sim, PbfSimTuning* tuning);

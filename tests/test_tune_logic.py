@@ -11,7 +11,7 @@ class FakeSim:
 
     def __init__(self):
         self.knobs = {"candidate_mask": True, "stream_ratio": 3, "chunk_bits": True, "sort_items": 0,
-                      "sort_digit_bits": 8, "walk_block": 256}
+                      "sort_digit_bits": 8, "walk_block": 256, "lane_order": False}
         self.t = 0.0
         self.state_value = 123
         self.n = 10
@@ -25,7 +25,8 @@ class FakeSim:
         k = self.knobs
         return (1.0 + (0.0 if k["candidate_mask"] else 0.3) + {1: 0.0, 2: 0.1, 3: 0.2, 6: 0.5}[k["stream_ratio"]]
                 + (0.05 if k["chunk_bits"] else 0.0) + {0: 0.02, 4: 0.0, 8: 0.01, 16: 0.03}[k["sort_items"]]
-                + {8: 0.0, 6: 0.01, 4: 0.02}[k["sort_digit_bits"]] + {256: 0.01, 128: 0.0}[k["walk_block"]])
+                + {8: 0.0, 6: 0.01, 4: 0.02}[k["sort_digit_bits"]] + {256: 0.01, 128: 0.0}[k["walk_block"]]
+                + (0.0 if k["lane_order"] else 0.04))
 
     def tuning(self):
         return dict(self.knobs)
@@ -54,7 +55,7 @@ def test_autotune_finds_the_fastest_setting_and_restores(tmp_path):
     path = str(tmp_path / "t.json")
     best, ms, table = tune.autotune(sim, steps=2, key="k", path=path, snapshot=FakeSnapshot, clock=lambda: sim.t)
     assert best == {"candidate_mask": True, "stream_ratio": 1, "chunk_bits": False, "sort_items": 4,
-                    "sort_digit_bits": 8, "walk_block": 128}
+                    "sort_digit_bits": 8, "walk_block": 128, "lane_order": True}
     assert abs(ms - 1.0) < 1e-9
     assert sim.tuning() == best
     assert sim.state_value == 123
