@@ -54,6 +54,10 @@ void dfree(T*& p) {
     p = nullptr;
 }
 
+// Waits for the null stream, where the blocking hipMemset calls of the allocation
+// routines run (they may return before the memory is cleared).
+bool null_stream_sync() { return hipStreamSynchronize(nullptr) == hipSuccess; }
+
 int reach_for(const SimParams& p) {
     const float cs = p.grid.cellSize;
     if (!(cs > 0.f)) return 1;
@@ -261,13 +265,15 @@ bool DeviceBuffers::allocate(uint32_t cap) {
     capacity = cap;
     d_pos = d_pos_curr;
     d_pos_pred = d_pos_next;
-    (void)hipMemset(d_pos_curr, 0, sizeof(float4) * cap);
-    (void)hipMemset(d_pos_next, 0, sizeof(float4) * cap);
-    (void)hipMemset(d_vel, 0, sizeof(float4) * cap);
-    (void)hipMemset(d_pos_sorted[0], 0, sizeof(float4) * ((size_t)cap + pbfx::kSlotPad));
-    (void)hipMemset(d_pos_sorted[1], 0, sizeof(float4) * ((size_t)cap + pbfx::kSlotPad));
-    (void)hipMemset(d_vel_sorted, 0, sizeof(float4) * ((size_t)cap + pbfx::kSlotPad));
-    return true;
+    ok = hipMemset(d_pos_curr, 0, sizeof(float4) * cap) == hipSuccess &&
+         hipMemset(d_pos_next, 0, sizeof(float4) * cap) == hipSuccess &&
+         hipMemset(d_vel, 0, sizeof(float4) * cap) == hipSuccess &&
+         hipMemset(d_pos_sorted[0], 0, sizeof(float4) * ((size_t)cap + pbfx::kSlotPad)) == hipSuccess &&
+         hipMemset(d_pos_sorted[1], 0, sizeof(float4) * ((size_t)cap + pbfx::kSlotPad)) == hipSuccess &&
+         hipMemset(d_vel_sorted, 0, sizeof(float4) * ((size_t)cap + pbfx::kSlotPad)) == hipSuccess;
+    // hipMemset of device memory may return before it lands, and the simulator's
+    // stream does not wait for the null stream
+    return null_stream_sync() && ok;
 }
 
 void DeviceBuffers::release() {
@@ -331,19 +337,17 @@ bool GridBuffers::allocate(uint32_t cap, uint32_t nCells, GridKind k, uint32_t t
             // lookup compares full keys, so results never depend on it)
             table.scatter = (uint64_t)nCells > (uint64_t)slots ? 1u : 0u;
             ok = ok && dalloc(table.slots, slots) && dalloc(table.epoch, 1) && dalloc(table.overflow, 1);
-            if (ok) {
-                (void)hipMemset(table.slots, 0, sizeof(pbfx::HashSlot) * slots);
-                (void)hipMemset(table.epoch, 0, sizeof(uint32_t));
-                (void)hipMemset(table.overflow, 0, sizeof(uint32_t));
-            }
+            ok = ok && hipMemset(table.slots, 0, sizeof(pbfx::HashSlot) * slots) == hipSuccess &&
+                 hipMemset(table.epoch, 0, sizeof(uint32_t)) == hipSuccess &&
+                 hipMemset(table.overflow, 0, sizeof(uint32_t)) == hipSuccess;
         }
     }
     if (!ok) {
         release();
         return false;
     }
-    (void)hipMemset(d_compactCount, 0, sizeof(uint32_t));
-    return true;
+    ok = hipMemset(d_compactCount, 0, sizeof(uint32_t)) == hipSuccess;
+    return null_stream_sync() && ok;
 }
 
 void GridBuffers::release() {
@@ -460,10 +464,10 @@ bool Simulator::initializeEnsemble(const SimParams& base, const std::vector<Scen
                   sceneOf.begin() + m_scenes[b].particleBase + m_scenes[b].numParticles, (uint32_t)b);
     bool ok = dalloc(d_sceneParams, B) && dalloc(d_sceneOf, parts) && dalloc(d_sceneOfSlot, parts) &&
               dalloc(d_cellBase, B + 1);
-    ok = ok && hipMemcpy(d_sceneParams, m_scenes.data(), sizeof(pbfx::SceneParams) * B, hipMemcpyHostToDevice) ==
-                   hipSuccess;
-    ok = ok && hipMemcpy(d_sceneOf, sceneOf.data(), sizeof(uint32_t) * parts, hipMemcpyHostToDevice) == hipSuccess;
-    ok = ok && hipMemcpy(d_cellBase, cellBase.data(), sizeof(uint32_t) * (B + 1), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && copyOnStream(d_sceneParams, m_scenes.data(), sizeof(pbfx::SceneParams) * B, hipMemcpyHostToDevice);
+    ok = ok && copyOnStream(d_sceneOf, sceneOf.data(), sizeof(uint32_t) * parts, hipMemcpyHostToDevice);
+    ok = ok && copyOnStream(d_cellBase, cellBase.data(), sizeof(uint32_t) * (B + 1), hipMemcpyHostToDevice);
+    ok = ok && syncStream();
     if (!ok) {
         set_error(PBFX_E_CAPACITY, "initializeEnsemble: scene table upload failed");
         shutdown();
@@ -617,13 +621,13 @@ bool Simulator::restructure(const SimParams& p) {
             dfree(pos);
             return false;
         }
-        bool ok = hipMemcpy(pos, m_bufs.d_pos, sizeof(float4) * keep, hipMemcpyDeviceToDevice) == hipSuccess &&
-                  hipMemcpy(vel, m_bufs.d_vel, sizeof(float4) * keep, hipMemcpyDeviceToDevice) == hipSuccess;
+        bool ok = copyOnStream(pos, m_bufs.d_pos, sizeof(float4) * keep, hipMemcpyDeviceToDevice) &&
+                  copyOnStream(vel, m_bufs.d_vel, sizeof(float4) * keep, hipMemcpyDeviceToDevice) && syncStream();
         ok = ok && m_bufs.allocate(cap);
         m_parity = 0;
-        ok = ok && hipMemcpy(m_bufs.d_pos, pos, sizeof(float4) * keep, hipMemcpyDeviceToDevice) == hipSuccess &&
-             hipMemcpy(m_bufs.d_pos_pred, pos, sizeof(float4) * keep, hipMemcpyDeviceToDevice) == hipSuccess &&
-             hipMemcpy(m_bufs.d_vel, vel, sizeof(float4) * keep, hipMemcpyDeviceToDevice) == hipSuccess;
+        ok = ok && copyOnStream(m_bufs.d_pos, pos, sizeof(float4) * keep, hipMemcpyDeviceToDevice) &&
+             copyOnStream(m_bufs.d_pos_pred, pos, sizeof(float4) * keep, hipMemcpyDeviceToDevice) &&
+             copyOnStream(m_bufs.d_vel, vel, sizeof(float4) * keep, hipMemcpyDeviceToDevice) && syncStream();
         dfree(pos);
         dfree(vel);
         if (!ok) {
@@ -747,9 +751,9 @@ bool Simulator::getSceneState(uint32_t scene, float4* pos, float4* vel) const {
     const pbfx::SceneParams& sp = m_scenes[scene];
     const size_t bytes = sizeof(float4) * sp.numParticles;
     bool ok = true;
-    if (pos) ok = ok && hipMemcpy(pos, m_bufs.d_pos + sp.particleBase, bytes, hipMemcpyDeviceToHost) == hipSuccess;
-    if (vel) ok = ok && hipMemcpy(vel, m_bufs.d_vel + sp.particleBase, bytes, hipMemcpyDeviceToHost) == hipSuccess;
-    return ok;
+    if (pos) ok = ok && copyOnStream(pos, m_bufs.d_pos + sp.particleBase, bytes, hipMemcpyDeviceToHost);
+    if (vel) ok = ok && copyOnStream(vel, m_bufs.d_vel + sp.particleBase, bytes, hipMemcpyDeviceToHost);
+    return syncStream() && ok;
 }
 
 bool Simulator::fillLattice(const GridBounds& g, uint32_t first, uint32_t nx, uint32_t ny, uint32_t nz, float3 o,
@@ -786,15 +790,22 @@ bool Simulator::fillLattice(const GridBounds& g, uint32_t first, uint32_t nx, ui
     return uploadBlock(first, h, "seedBoxLattice: copy failed");
 }
 
+bool Simulator::copyOnStream(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) const {
+    return bytes == 0 || hipMemcpyAsync(dst, src, bytes, kind, m_stream) == hipSuccess;
+}
+
+bool Simulator::syncStream() const { return hipStreamSynchronize(m_stream) == hipSuccess; }
+
 bool Simulator::uploadBlock(uint32_t first, const std::vector<float4>& h, const char* where) {
     const size_t bytes = sizeof(float4) * h.size();
     if (hipStreamSynchronize(m_stream) != hipSuccess) {
         set_error(PBFX_E_INVALID_ARGUMENT, where);
         return false;
     }
-    bool ok = hipMemcpy(m_bufs.d_pos + first, h.data(), bytes, hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(m_bufs.d_pos_pred + first, h.data(), bytes, hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemset(m_bufs.d_vel + first, 0, bytes) == hipSuccess;
+    bool ok = copyOnStream(m_bufs.d_pos + first, h.data(), bytes, hipMemcpyHostToDevice) &&
+              copyOnStream(m_bufs.d_pos_pred + first, h.data(), bytes, hipMemcpyHostToDevice) &&
+              hipMemsetAsync(m_bufs.d_vel + first, 0, bytes, m_stream) == hipSuccess;
+    ok = syncStream() && ok;
     if (!ok) set_error(PBFX_E_INVALID_ARGUMENT, where);
     return ok;
 }
@@ -889,10 +900,11 @@ bool Simulator::setState(const float4* pos, const float4* vel, uint32_t N, bool 
     const size_t bytes = sizeof(float4) * N;
     bool ok = hipStreamSynchronize(m_stream) == hipSuccess;  // no step may still use the buffers
     if (pos) {
-        ok = ok && hipMemcpy(m_bufs.d_pos, pos, bytes, k) == hipSuccess;
-        ok = ok && hipMemcpy(m_bufs.d_pos_pred, pos, bytes, k) == hipSuccess;
+        ok = ok && copyOnStream(m_bufs.d_pos, pos, bytes, k);
+        ok = ok && copyOnStream(m_bufs.d_pos_pred, pos, bytes, k);
     }
-    if (vel) ok = ok && hipMemcpy(m_bufs.d_vel, vel, bytes, k) == hipSuccess;
+    if (vel) ok = ok && copyOnStream(m_bufs.d_vel, vel, bytes, k);
+    ok = syncStream() && ok;
     if (!ok) set_error(PBFX_E_INVALID_ARGUMENT, "setState: copy failed");
     return ok;
 }
@@ -906,9 +918,9 @@ bool Simulator::getState(float4* pos, float4* vel, bool toDevice) const {
     const hipMemcpyKind k = toDevice ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
     const size_t bytes = sizeof(float4) * m_params.numParticles;
     bool ok = true;
-    if (pos) ok = ok && hipMemcpy(pos, m_bufs.d_pos, bytes, k) == hipSuccess;
-    if (vel) ok = ok && hipMemcpy(vel, m_bufs.d_vel, bytes, k) == hipSuccess;
-    return ok;
+    if (pos) ok = ok && copyOnStream(pos, m_bufs.d_pos, bytes, k);
+    if (vel) ok = ok && copyOnStream(vel, m_bufs.d_vel, bytes, k);
+    return syncStream() && ok;
 }
 
 bool Simulator::densityStats(double* out5) {
@@ -963,10 +975,10 @@ bool Simulator::adjointInit(int slots) {
              dalloc(t.spans, spanWords) && dalloc(t.Vs, (size_t)cap + pbfx::kSlotPad) && dalloc(t.predClamp, cap) &&
              dalloc(t.finalClamp, cap) && dalloc(t.xsphGm, cap);
         if (!m_scenes.empty()) ok = ok && dalloc(t.sceneOfSlot, cap);
-        if (ok) (void)hipMemset(t.Vs, 0, sizeof(float4) * ((size_t)cap + pbfx::kSlotPad));
+        ok = ok && hipMemset(t.Vs, 0, sizeof(float4) * ((size_t)cap + pbfx::kSlotPad)) == hipSuccess;
         for (int it = 0; ok && it <= m_params.solverIters; ++it) {
-            ok = dalloc(t.P[it], (size_t)cap + pbfx::kSlotPad);
-            if (ok) (void)hipMemset(t.P[it], 0, sizeof(float4) * ((size_t)cap + pbfx::kSlotPad));
+            ok = dalloc(t.P[it], (size_t)cap + pbfx::kSlotPad) &&
+                 hipMemset(t.P[it], 0, sizeof(float4) * ((size_t)cap + pbfx::kSlotPad)) == hipSuccess;
         }
         for (int it = 0; ok && it < m_params.solverIters; ++it)
             ok = dalloc(t.lamA[it], cap) && dalloc(t.lamB[it], cap) && dalloc(t.dispU[it], cap);
@@ -980,7 +992,12 @@ bool Simulator::adjointInit(int slots) {
         releaseAdjoint();
         return false;
     }
-    (void)hipMemset(m_adj.gmax, 0, sizeof(uint32_t) * std::max<size_t>(1, m_scenes.size()) * pbfx::kGmaxStride);
+    ok = hipMemset(m_adj.gmax, 0, sizeof(uint32_t) * std::max<size_t>(1, m_scenes.size()) * pbfx::kGmaxStride) ==
+         hipSuccess;
+    if (!(null_stream_sync() && ok)) {
+        releaseAdjoint();
+        return false;
+    }
     return true;
 }
 
@@ -1038,8 +1055,7 @@ bool Simulator::backwardStep(int slot, const float4* gXin, const float4* gVin, f
 bool Simulator::hashOverflowed() const {
     if (!m_grid.table.overflow) return false;
     uint32_t v = 0;
-    (void)hipStreamSynchronize(m_stream);
-    (void)hipMemcpy(&v, m_grid.table.overflow, sizeof(v), hipMemcpyDeviceToHost);
+    if (!copyOnStream(&v, m_grid.table.overflow, sizeof(v), hipMemcpyDeviceToHost) || !syncStream()) return false;
     return v != 0;
 }
 

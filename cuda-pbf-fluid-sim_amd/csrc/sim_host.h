@@ -265,6 +265,12 @@ private:
     bool fillLattice(const GridBounds& g, uint32_t first, uint32_t nx, uint32_t ny, uint32_t nz, float3 o,
                      float spacing, float amp, uint64_t seed);
     bool uploadBlock(uint32_t first, const std::vector<float4>& h, const char* where);
+    // Copies enqueued on the simulator's stream.  That stream does not wait for the
+    // null stream, and a blocking hipMemcpy between device buffers (or a pageable
+    // upload, or hipMemset) may return before its data lands, so state moves in and
+    // out through these and the caller then synchronises the stream.
+    bool copyOnStream(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) const;
+    bool syncStream() const;
     void* m_extVel = nullptr;
     size_t m_extVelBytes = 0;
     uint32_t m_extVelStride = 0;

@@ -128,6 +128,38 @@ def test_graph_replay_equals_eager(dev):
         np.testing.assert_array_equal(Va, Vb)
 
 
+@pytest.mark.timeout(300)
+def test_restored_state_steps_alike_eager_and_graph(dev):
+    """A state restored from device tensors (set_state_device) is in place before the
+    next step starts: graph replays and eager steps from it give the same bits, every
+    time.  The copies run on the simulator's stream: a blocking device-to-device copy
+    on the null stream may return before it lands, and the simulator's stream does not
+    wait for the null stream, so a graph launched right after it read half-copied
+    positions.  The 1M dam break after 60 steps (collapsed, every step distinct)."""
+    scene = P.dam_break_1m()
+    graph = _sim(scene, "hashed", use_graph=True)
+    graph.step(60)
+    st = graph.state_device()
+    st = (st[0].clone(), st[1].clone())
+    eager = _sim(scene, "hashed", use_graph=False)
+    eager.set_state_device(*st)
+    eager.step(1)
+    Xe, Ve = eager.state()
+    eager.close()
+    for _ in range(3):
+        graph.set_state_device(*st)
+        graph.step(1)
+        Xg, Vg = graph.state()
+        np.testing.assert_array_equal(Xg, Xe)
+        np.testing.assert_array_equal(Vg, Ve)
+        graph.set_state_device(*st)
+        graph.profile_launches()
+        Xp, Vp = graph.state()
+        np.testing.assert_array_equal(Xp, Xe)
+        np.testing.assert_array_equal(Vp, Ve)
+    graph.close()
+
+
 def test_paths_agree_bitwise(dev):
     scene = P.small_block((20, 20, 20))
     a = _sim(scene, "dense", "reordered")
