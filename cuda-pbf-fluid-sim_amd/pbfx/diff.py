@@ -4,7 +4,8 @@
 The reverse pass walks the segments from the last one back. For each segment it
 restores the checkpoint, re-runs the segment's forward steps while recording them
 (sort permutation, spans, clamp bits, every iteration's positions and lambdas), and
-applies the reverse kernel of each recorded step, last first.
+applies the reverse kernel of each recorded step, last first.  The forward pass records
+its last segment as it runs, so the reverse pass starts from those tapes.
 ``PBFRollout`` exposes X_T as a differentiable function of the initial velocity
 and of the XSPH coefficient c. Gradients reach x0 as well.
 
@@ -54,6 +55,7 @@ class Rollout:
             raise _lib.PbfError("adjoint init failed")
         self.n = int(sim.n)
         self._ckpts = []
+        self._tail_taped = False  # the tape slots hold the last segment of the latest forward()
         self.timed = False    # when set, backward() synchronises between its phases and fills `timing`
         self.timing = {}
 
@@ -107,6 +109,7 @@ class Rollout:
         X, V = _as4(x0), _as4(v0)
         self._set_state(X, V)
         self._ckpts = []
+        self._tail_taped = False
         # backward() re-records the segments with these parameters, whatever changes later
         self._fwd_params = self._params_now()
         L = _lib.lib()
@@ -114,9 +117,19 @@ class Rollout:
         while done < self.steps:
             self._ckpts.append(self._get_state())
             k = min(self.ck, self.steps - done)
-            if L.pbfSimStep(self.sim._h, k) != 0:
-                _lib.check("pbfSimStep")
-                raise _lib.PbfError("step failed")
+            if done + k < self.steps:
+                if L.pbfSimStep(self.sim._h, k) != 0:
+                    _lib.check("pbfSimStep")
+                    raise _lib.PbfError("step failed")
+            else:
+                # the last segment is recorded as it runs (a recorded step is the same
+                # step, with its tape written beside it), so backward() starts from these
+                # tapes instead of re-running the segment
+                for t in range(k):
+                    if L.pbfSimStepRecord(self.sim._h, t) != 0:
+                        _lib.check("pbfSimStepRecord")
+                        raise _lib.PbfError("record failed")
+                self._tail_taped = True
             done += k
         return self._get_state()
 
@@ -146,15 +159,24 @@ class Rollout:
             self._apply_params(self._fwd_params)
         starts = list(range(0, self.steps, self.ck))
         t_rec = t_rev = 0.0
+        # the reverse kernels run on the simulator's stream and read what torch's stream
+        # wrote above (the seeds, the zeroed dL/dc and exponents)
+        torch.cuda.synchronize()
         try:
             for seg in reversed(range(len(starts))):
                 k = min(self.ck, self.steps - starts[seg])
-                self._set_state(*self._ckpts[seg])
                 t0 = self._clock()
-                for t in range(k):
-                    if L.pbfSimStepRecord(self.sim._h, t) != 0:
-                        _lib.check("pbfSimStepRecord")
-                        raise _lib.PbfError("record failed")
+                if seg == len(starts) - 1 and self._tail_taped:
+                    # forward() left this segment's tapes in the slots
+                    self._tail_taped = False
+                else:
+                    self._tail_taped = False  # the slots are overwritten from here on
+                    self._set_state(*self._ckpts[seg])
+                    t0 = self._clock()
+                    for t in range(k):
+                        if L.pbfSimStepRecord(self.sim._h, t) != 0:
+                            _lib.check("pbfSimStepRecord")
+                            raise _lib.PbfError("record failed")
                 t1 = self._clock()
                 for t in reversed(range(k)):
                     if L.pbfSimBackwardStepScaled(self.sim._h, t, _p(gX), _p(gV), _p(oX), _p(oV), _p(gc),
