@@ -17,6 +17,12 @@ void compact_segments(uint32_t* uniq, uint32_t* offsets, uint32_t* count, uint32
                       const uint32_t* keysSorted, uint32_t N, uint32_t* blockCounts, uint32_t* blockOffsets,
                       hipStream_t s);
 
+// Slot pairs of the cells after compact_segments (segOf, offsets): each cell's slots two
+// at a time, in slot order, pairs[p] = (first, second or ~0 when the cell's count is
+// odd); *count = number of pairs.  Same scratch as compact_segments.
+void build_slot_pairs(uint2* pairs, uint32_t* count, const uint32_t* segOf, const uint32_t* offsets, uint32_t N,
+                      uint32_t* blockCounts, uint32_t* blockOffsets, hipStream_t s);
+
 // Compaction scratch of one stream, for LaunchCellRangesCompact calls on it (reserved
 // and released with the stream's drop-in staging).  The reference keeps one scratch per
 // process (cell_compact.cu:50-56), which two streams compacting at once would share.

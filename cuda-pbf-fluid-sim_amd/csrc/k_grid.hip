@@ -173,6 +173,51 @@ __global__ __launch_bounds__(kBS) void k_compact_scatter(const uint32_t* __restr
     }
 }
 
+// Slot pairs: slot i opens a pair when its place inside its cell (i - off[segOf[i]])
+// is even; the pair is (i, i + 1) when i + 1 is in the same cell, else (i, ~0).
+// Four slots a lane, as in the compaction: count per tile, scan, scatter.
+__device__ __forceinline__ uint32_t pair_start_bits(const uint32_t* __restrict__ segOf,
+                                                    const uint32_t* __restrict__ off, uint64_t first, uint32_t N) {
+    uint32_t bits = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint64_t i = first + j;
+        if (i < N) bits |= (((uint32_t)i - off[segOf[i]]) & 1u) ? 0u : (1u << j);
+    }
+    return bits;
+}
+
+__global__ __launch_bounds__(kBS) void k_pair_count(const uint32_t* __restrict__ segOf,
+                                                    const uint32_t* __restrict__ off, uint32_t N,
+                                                    uint32_t* __restrict__ blockCounts) {
+    __shared__ uint32_t sh[kBS / 64];
+    const uint64_t first = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kItems;
+    uint32_t total;
+    block_excl_scan(__popc(pair_start_bits(segOf, off, first, N)), sh, total);
+    if (threadIdx.x == 0) blockCounts[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kBS) void k_pair_scatter(const uint32_t* __restrict__ segOf,
+                                                      const uint32_t* __restrict__ off, uint32_t N,
+                                                      const uint32_t* __restrict__ blockOffsets,
+                                                      const uint32_t* __restrict__ blockCounts,
+                                                      uint2* __restrict__ pairs, uint32_t* __restrict__ count) {
+    __shared__ uint32_t sh[kBS / 64];
+    const uint64_t first = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kItems;
+    const uint32_t bits = pair_start_bits(segOf, off, first, N);
+    uint32_t total;
+    uint32_t p = blockOffsets[blockIdx.x] + block_excl_scan(__popc(bits), sh, total);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if ((bits >> j) & 1u) {
+            const uint32_t i = (uint32_t)(first + j);
+            const uint32_t end = off[segOf[i] + 1u];
+            pairs[p++] = make_uint2(i, i + 1u < end ? i + 1u : 0xFFFFFFFFu);
+        }
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *count = blockOffsets[blockIdx.x] + blockCounts[blockIdx.x];
+}
+
 __global__ void k_compact_empty(uint32_t* off, uint32_t* count) {
     off[0] = 0;
     *count = 0;
@@ -281,6 +326,19 @@ void compact_segments(uint32_t* uniq, uint32_t* offsets, uint32_t* count, uint32
     k_scan_tiles<<<1, kBS, 0, s>>>(blockCounts, blockOffsets, tiles);
     k_compact_scatter<<<tiles, kBS, 0, s>>>(keysSorted, N, blockOffsets, blockCounts, uniq, offsets, count, segOf);
     check_launch("compact_segments");
+}
+
+void build_slot_pairs(uint2* pairs, uint32_t* count, const uint32_t* segOf, const uint32_t* offsets, uint32_t N,
+                      uint32_t* blockCounts, uint32_t* blockOffsets, hipStream_t s) {
+    if (N == 0) {
+        fill_u32(count, 0u, 1, s);
+        return;
+    }
+    const uint32_t tiles = compact_scratch_elems(N);
+    k_pair_count<<<tiles, kBS, 0, s>>>(segOf, offsets, N, blockCounts);
+    k_scan_tiles<<<1, kBS, 0, s>>>(blockCounts, blockOffsets, tiles);
+    k_pair_scatter<<<tiles, kBS, 0, s>>>(segOf, offsets, N, blockOffsets, blockCounts, pairs, count);
+    check_launch("build_slot_pairs");
 }
 
 }  // namespace pbfx

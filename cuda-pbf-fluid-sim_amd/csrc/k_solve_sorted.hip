@@ -450,28 +450,41 @@ __global__ __launch_bounds__(kBS) void k_finalize_sorted(float4* __restrict__ Ps
 }
 
 // XSPH's pair term is a few products, so its walk evaluates every candidate in place
-// (inline_walk) rather than queueing the neighbours for a later pass: the sums take
-// the same neighbours in the same order, with every lane busy on its own candidates
-// and no queue to fill and drain.  Positions are read for every candidate of a round,
-// velocities for its neighbours only (the vector memory pipeline, not the VALU, bounds
-// a round that reads both for all four).  Both arrays carry kSlotPad padding slots past
-// the last particle.
-// Eight waves per SIMD (64 VGPRs).
+// rather than queueing the neighbours for a later pass: the sums take the same
+// neighbours in the same order, with no queue to fill and drain.  The vector memory
+// pipeline, not the VALU, bounds that walk, so each lane takes two particles of one
+// cell (a slot pair: the cell's slots two at a time), which share the cell's neighbour
+// rows: a round's four candidate positions are read once for both, and a candidate's
+// velocity once when it is a neighbour of either (a non-neighbour's read is sent past
+// the end of the buffer, where the hardware returns zeros without touching memory).
+// Both arrays carry kSlotPad padding slots past the last particle.  The grid covers N
+// pairs, the most there can be; blocks past the step's pair count return at once, and
+// the XCD-aware order is taken over the blocks that have pairs.
 template <int BS>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8))) void k_xsph_scatter(const float4* __restrict__ Ps, const float4* __restrict__ Vs,
-                                                      const uint32_t* __restrict__ segOf, SpanTable st,
-                                                      sim::DeviceParams dp, uint32_t N,
-                                                      const uint32_t* __restrict__ idxS, float4* __restrict__ Xn,
-                                                      float4* __restrict__ V, Scenes sc, float* __restrict__ recGm) {
-    const uint32_t s = slot_of_thread<BS>();
-    if (s >= N) return;
-    const float4 pi = Ps[s];
-    const float4 vi = Vs[s];
-    float vx = vi.x, vy = vi.y, vz = vi.z;
+__global__ __launch_bounds__(BS) void k_xsph_scatter(const float4* __restrict__ Ps, const float4* __restrict__ Vs,
+                                                     const uint32_t* __restrict__ segOf, SpanTable st,
+                                                     sim::DeviceParams dp, uint32_t N,
+                                                     const uint2* __restrict__ pairs,
+                                                     const uint32_t* __restrict__ pairCount,
+                                                     const uint32_t* __restrict__ idxS, float4* __restrict__ Xn,
+                                                     float4* __restrict__ V, Scenes sc, float* __restrict__ recGm) {
+    const uint32_t np = *pairCount;
+    const uint32_t nb = (np + (uint32_t)BS - 1u) / (uint32_t)BS;
+    if (blockIdx.x >= nb) return;
+    const uint32_t p = xcd_tile(blockIdx.x, nb) * BS + threadIdx.x;
+    if (p >= np) return;
+    const uint2 ab = pairs[p];
+    const uint32_t a = ab.x;
+    const bool hasB = ab.y != 0xFFFFFFFFu;
+    const uint32_t b = hasB ? ab.y : a;
+    const float4 pa = Ps[a], pb = Ps[b];
+    const float4 va = Vs[a], vb = Vs[b];
+    float vax = va.x, vay = va.y, vaz = va.z, vbx = vb.x, vby = vb.y, vbz = vb.z;
     float xc = dp.xsph_c, rest = dp.boundaryRestitution;
     sim::GridBounds grid = dp.grid;
     if (sc.params) {
-        const SceneParams& sp = sc.params[sc.sceneOfSlot[s]];
+        // a cell never spans two scenes
+        const SceneParams& sp = sc.params[sc.sceneOfSlot[a]];
         xc = sp.xsph_c;
         rest = sp.restitution;
         grid = sp.grid;
@@ -479,37 +492,59 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8))) void k_
     if (xc > 0.f || recGm != nullptr) {
         const float h2 = dp.kernel.h2, poly6 = dp.kernel.poly6;
         const SlotBuf PS(Ps, N + kSlotPad), VS(Vs, N + kSlotPad);
-        float sx = 0.f, sy = 0.f, sz = 0.f;
-        int n = 0;
-        inline_walk(
-            true, s, SpanRows(st, segOf[s]), h2, [&](uint32_t k, uint32_t u) { return PS.at(k, u); },
-            [&](const float4& pj) { return pair_r2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z); },
-            // a non-neighbour's velocity read is sent past the end of the buffer, where the
-            // hardware returns zeros without touching memory (and take() discards it)
+        float sax = 0.f, say = 0.f, saz = 0.f, sbx = 0.f, sby = 0.f, sbz = 0.f;
+        int na = 0, nb2 = 0;
+        inline_walk_pair(
+            a, b, hasB, SpanRows(st, segOf[a]), h2, [&](uint32_t k, uint32_t u) { return PS.at(k, u); },
+            [&](const float4& pj) { return pair_r2(pa.x - pj.x, pa.y - pj.y, pa.z - pj.z); },
+            [&](const float4& pj) { return pair_r2(pb.x - pj.x, pb.y - pj.y, pb.z - pj.z); },
             [&](uint32_t k, uint32_t u, bool in) { return VS.at(in ? k : kSlotBufBeyond, u); },
-            [&](const float4&, const float4& vj, float r2, bool in) {
+            [&](const float4& vj, float r2, bool in) {
                 const float hr2 = h2 - r2;
                 const float w = poly6 * hr2 * hr2 * hr2;
-                float tx = sx, ty = sy, tz = sz;
-                xsph_pair(tx, ty, tz, vj, vi, w);
-                sx = in ? tx : sx;
-                sy = in ? ty : sy;
-                sz = in ? tz : sz;
-                n += in ? 1 : 0;
+                float tx = sax, ty = say, tz = saz;
+                xsph_pair(tx, ty, tz, vj, va, w);
+                sax = in ? tx : sax;
+                say = in ? ty : say;
+                saz = in ? tz : saz;
+                na += in ? 1 : 0;
+            },
+            [&](const float4& vj, float r2, bool in) {
+                const float hr2 = h2 - r2;
+                const float w = poly6 * hr2 * hr2 * hr2;
+                float tx = sbx, ty = sby, tz = sbz;
+                xsph_pair(tx, ty, tz, vj, vb, w);
+                sbx = in ? tx : sbx;
+                sby = in ? ty : sby;
+                sbz = in ? tz : sbz;
+                nb2 += in ? 1 : 0;
             });
-        if (recGm)
-            recGm[s] = xsph_gate(n, dp) * ((dp.restDensity > 0.f) ? (dp.particleMass / dp.restDensity) : 0.f);
+        if (recGm) {
+            const float mor = (dp.restDensity > 0.f) ? (dp.particleMass / dp.restDensity) : 0.f;
+            recGm[a] = xsph_gate(na, dp) * mor;
+            if (hasB) recGm[b] = xsph_gate(nb2, dp) * mor;
+        }
         if (xc > 0.f) {
-            const float3 v = xsph_finish(vi.x, vi.y, vi.z, sx, sy, sz, n, dp, xc);
-            vx = v.x; vy = v.y; vz = v.z;
+            const float3 u = xsph_finish(va.x, va.y, va.z, sax, say, saz, na, dp, xc);
+            vax = u.x; vay = u.y; vaz = u.z;
+            const float3 w = xsph_finish(vb.x, vb.y, vb.z, sbx, sby, sbz, nb2, dp, xc);
+            vbx = w.x; vby = w.y; vbz = w.z;
         }
     }
     // BoundaryOp with restitution (post_ops.cpp:25-29, boundary.cu:11-25)
-    float px = pi.x, py = pi.y, pz = pi.z;
-    clamp_box(px, py, pz, vx, vy, vz, grid, clamp_restitution(rest));
-    const uint32_t i = idxS[s];
-    Xn[i] = make_float4(px, py, pz, 1.0f);
-    V[i] = make_float4(vx, vy, vz, 0.0f);
+    const float e = clamp_restitution(rest);
+    float px = pa.x, py = pa.y, pz = pa.z;
+    clamp_box(px, py, pz, vax, vay, vaz, grid, e);
+    const uint32_t ia = idxS[a];
+    Xn[ia] = make_float4(px, py, pz, 1.0f);
+    V[ia] = make_float4(vax, vay, vaz, 0.0f);
+    if (hasB) {
+        px = pb.x; py = pb.y; pz = pb.z;
+        clamp_box(px, py, pz, vbx, vby, vbz, grid, e);
+        const uint32_t ib = idxS[b];
+        Xn[ib] = make_float4(px, py, pz, 1.0f);
+        V[ib] = make_float4(vbx, vby, vbz, 0.0f);
+    }
 }
 
 inline uint32_t blocks_for(uint32_t n) { return ceil_div(n > 0 ? n : 1, kBS); }
@@ -610,13 +645,16 @@ void launch_finalize_sorted(float4* Ps, const float4* X, const uint32_t* idxS, f
 }
 
 void launch_xsph_scatter(const float4* Ps, const float4* Vs, const uint32_t* segOf, const SpanTable& st,
-                         const sim::DeviceParams& dp, uint32_t N, const uint32_t* idxS, float4* Xn, float4* V,
-                         const Scenes& sc, hipStream_t s, float* recGm) {
+                         const sim::DeviceParams& dp, uint32_t N, const uint2* pairs, const uint32_t* pairCount,
+                         const uint32_t* idxS, float4* Xn, float4* V, const Scenes& sc, hipStream_t s,
+                         float* recGm) {
     if (N == 0) return;
     if (st.block == 128)
-        k_xsph_scatter<128><<<ceil_div(N, 128u), 128, 0, s>>>(Ps, Vs, segOf, st, dp, N, idxS, Xn, V, sc, recGm);
+        k_xsph_scatter<128><<<ceil_div(N, 128u), 128, 0, s>>>(Ps, Vs, segOf, st, dp, N, pairs, pairCount, idxS, Xn,
+                                                              V, sc, recGm);
     else
-        k_xsph_scatter<256><<<ceil_div(N, 256u), 256, 0, s>>>(Ps, Vs, segOf, st, dp, N, idxS, Xn, V, sc, recGm);
+        k_xsph_scatter<256><<<ceil_div(N, 256u), 256, 0, s>>>(Ps, Vs, segOf, st, dp, N, pairs, pairCount, idxS, Xn,
+                                                              V, sc, recGm);
     check_launch("xsph_scatter");
 }
 

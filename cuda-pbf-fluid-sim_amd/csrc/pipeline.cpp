@@ -179,6 +179,10 @@ void SortedGridBuild(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipSt
         pbfx::launch_lane_order(b.rec.P[0] ? b.rec.P[0] : b.d_pos_sorted[0], N, p.grid, g.scenes, g.d_laneOrder, s);
         mark("lane_order");
     }
+    // XSPH's slot pairs (the compaction's tile scratch is free again)
+    pbfx::build_slot_pairs(g.d_slotPairs, g.d_slotPairCount, g.d_segOf, g.d_cellOffsets, N, g.d_blockCounts,
+                           g.d_blockOffsets, s);
+    mark("slot_pairs");
 }
 
 void SortedSolve(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_t s) {
@@ -216,13 +220,14 @@ void SortedSolve(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream
 
 void SortedPost(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_t s) {
     const int fin = p.solverIters > 0 ? (p.solverIters & 1) : 0;
-    // XSPH walks its candidates directly, evaluating each in place: each of its pairs
-    // also gathers a velocity, and the scattered loads of the candidate-mask walk cost
-    // more than the consecutive candidates the mask would skip
+    // XSPH walks its candidates directly, evaluating each in place, two particles of a
+    // cell a lane: each of its pairs also gathers a velocity, and the scattered loads of
+    // the candidate-mask walk cost more than the consecutive candidates the mask would skip
     const bool rec = b.rec.P[0] != nullptr;
     pbfx::launch_xsph_scatter(rec ? b.rec.P[p.solverIters] : b.d_pos_sorted[fin], rec ? b.rec.Vs : b.d_vel_sorted,
-                              g.d_segOf, g.spans, MakeDeviceParams(p), p.numParticles, g.d_indices_sorted,
-                              b.d_pos_pred, b.d_vel, g.scenes, s, rec ? b.rec.xsphGm : nullptr);
+                              g.d_segOf, g.spans, MakeDeviceParams(p), p.numParticles, g.d_slotPairs,
+                              g.d_slotPairCount, g.d_indices_sorted, b.d_pos_pred, b.d_vel, g.scenes, s,
+                              rec ? b.rec.xsphGm : nullptr);
     mark("xsph_scatter");
 }
 
@@ -328,7 +333,8 @@ bool GridBuffers::allocate(uint32_t cap, uint32_t nCells, GridKind k, uint32_t t
          dalloc(spans.spans, (size_t)spans.rows * cap);
     if (reordered) {
         mask.stride = cap;
-        ok = ok && dalloc(mask.words, (size_t)pbfx::kCandMaskWords * cap) && dalloc(d_laneOrder, cap);
+        ok = ok && dalloc(mask.words, (size_t)pbfx::kCandMaskWords * cap) && dalloc(d_laneOrder, cap) &&
+             dalloc(d_slotPairs, cap) && dalloc(d_slotPairCount, 1);
         if (k == GridKind::Hashed) {
             tableLog2 = tLog2;
             const size_t slots = size_t(1) << tLog2;
@@ -354,6 +360,7 @@ void GridBuffers::release() {
     dfree(d_cellKeys); dfree(d_cellKeys_sorted); dfree(d_indices); dfree(d_indices_sorted);
     dfree(d_cellStart); dfree(d_cellEnd); dfree(d_cellUniqueKeys); dfree(d_cellOffsets); dfree(d_compactCount);
     dfree(d_segOf); dfree(d_blockCounts); dfree(d_blockOffsets); dfree(d_laneOrder);
+    dfree(d_slotPairs); dfree(d_slotPairCount);
     if (d_sortTemp) (void)hipFree(d_sortTemp);
     d_sortTemp = nullptr;
     sortTempBytes = 0;

@@ -82,6 +82,8 @@ struct GridBuffers {
     pbfx::CandMask mask{};  // per-step candidate mask (reordered path)
     bool candidateMask = true;  // later passes walk the mask (tuning knob)
     uint32_t* d_laneOrder = nullptr;  // lane -> slot of the lambda and delta passes (reordered path)
+    uint2* d_slotPairs = nullptr;     // each cell's slots two at a time, for XSPH (reordered path)
+    uint32_t* d_slotPairCount = nullptr;
     bool laneOrder = true;            // use d_laneOrder (tuning knob)
     int sortItems = 0;          // radix sort keys per thread (0: by N; tuning knob)
     int sortDigitBits = 8;      // radix sort widest digit of a pass (4..8 bits; tuning knob)

@@ -136,10 +136,12 @@ void launch_finalize_sorted(float4* Ps, const float4* X, const uint32_t* idxS, f
                             const sim::GridBounds& grid, float invDt, uint32_t N, const Scenes& sc, hipStream_t s);
 
 // XSPH (Jacobi over Vs, skipped when xsph_c <= 0), boundary with restitution,
-// scatter Xn[idxS[s]] = (P, 1) and V[idxS[s]] = (v, 0).  Recording (non-null recGm):
+// scatter Xn[idxS[s]] = (P, 1) and V[idxS[s]] = (v, 0), one slot pair a lane
+// (pairs / pairCount from build_slot_pairs).  Recording (non-null recGm):
 // recGm[s] = gate * m / rho0, walked also when xsph_c is 0.
 void launch_xsph_scatter(const float4* Ps, const float4* Vs, const uint32_t* segOf, const SpanTable& st,
-                         const sim::DeviceParams& dp, uint32_t N, const uint32_t* idxS, float4* Xn, float4* V,
-                         const Scenes& sc, hipStream_t s, float* recGm = nullptr);
+                         const sim::DeviceParams& dp, uint32_t N, const uint2* pairs, const uint32_t* pairCount,
+                         const uint32_t* idxS, float4* Xn, float4* V, const Scenes& sc, hipStream_t s,
+                         float* recGm = nullptr);
 
 }  // namespace pbfx

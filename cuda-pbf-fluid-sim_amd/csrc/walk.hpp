@@ -430,6 +430,44 @@ __device__ __forceinline__ void inline_walk(bool valid, uint32_t s, const Rows& 
     }
 }
 
+// inline_walk for two particles of one cell (slots a and b; b valid only when hasB):
+// they share the cell's neighbour rows, so each round's positions are read once for
+// both, and a candidate's gather() once when it is a neighbour of either.  Each
+// particle's take gets every candidate in walk order with its own flag.
+template <class Rows, class Load, class Dist2A, class Dist2B, class Gather, class TakeA, class TakeB>
+__device__ __forceinline__ void inline_walk_pair(uint32_t a, uint32_t b, bool hasB, const Rows& rows, float h2,
+                                                 Load&& load, Dist2A&& dist2a, Dist2B&& dist2b, Gather&& gather,
+                                                 TakeA&& takeA, TakeB&& takeB) {
+    const int nrows = rows.count();
+    for (int r = 0; r < nrows; ++r) {
+        uint32_t lo = 0, hi = 0;
+        rows.span(r, lo, hi);
+        for (uint32_t k = lo; k < hi; k += kUnroll) {
+            const uint32_t cnt = min(kUnroll, hi - k);
+            decltype(load(k, 0u)) c[kUnroll];
+            decltype(gather(k, 0u, true)) g[kUnroll];
+            float ra[kUnroll], rb[kUnroll];
+            bool ia[kUnroll], ib[kUnroll];
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) c[u] = load(k, u);
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) {
+                ra[u] = dist2a(c[u]);
+                rb[u] = dist2b(c[u]);
+                ia[u] = (u < cnt) & (k + u != a) & (ra[u] <= h2);
+                ib[u] = hasB & (u < cnt) & (k + u != b) & (rb[u] <= h2);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) g[u] = gather(k, u, ia[u] | ib[u]);
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) {
+                takeA(g[u], ra[u], ia[u]);
+                takeB(g[u], rb[u], ib[u]);
+            }
+        }
+    }
+}
+
 // Rows of a sorted slot's neighbourhood from the per-step span table.
 struct SpanRows {
     const SpanTable& st;
