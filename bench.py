@@ -79,6 +79,15 @@ INSTR_FILE = os.path.join(REPO, "profiles", "instructions_dam_break_1m.json")
 # do not touch these kernels.)
 WALK_KNOBS = ("candidate_mask", "stream_ratio", "chunk_bits", "walk_block", "lane_order")
 
+# The dam break's tuning space: the walk settings its counter summaries cover (the
+# candidate mask on or off, 128 or 256 lanes a block), the rest of the walk fixed at
+# what every measurement of the collapsed fluid chose (per-lane walk, chunk masks, lane
+# order), and the sort's knobs, which do not touch the profiled kernels.
+BENCH_FIXED = {"stream_ratio": 1, "chunk_bits": True, "lane_order": True}
+BENCH_SPACE = {"candidate_mask": (True, False), "stream_ratio": (1,), "chunk_bits": (True,),
+               "sort_items": (0, 4, 8, 16), "sort_digit_bits": (8, 6, 4), "walk_block": (256, 128),
+               "lane_order": (True,)}
+
 
 def sample_offsets(total, k):
     """k stratified steps of a total-step trajectory: the midpoints of k equal strata."""
@@ -614,7 +623,9 @@ def main():
     elif not a.no_tune:
         from pbfx import tune
         pick = sorted({(2 * j + 1) * len(snaps) // (2 * a.tune_states) for j in range(a.tune_states)})
+        sim.set_tuning(**BENCH_FIXED)
         best, best_ms, table = tune.autotune(sim, steps=a.tune_steps, key=tune.workload_key(sim, "dam_break_1m"),
+                                             space=BENCH_SPACE,
                                              log=lambda m: print(m, file=sys.stderr, flush=True),
                                              states=[snaps[i] for i in pick])
         tuning = {"chosen": best, "ms_per_step_at_tuning": round(best_ms, 4), "settings_tried": len(table),
