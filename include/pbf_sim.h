@@ -36,6 +36,9 @@ void pbfSimDestroy(PbfSim* sim);
 int pbfSimSeedLattice(PbfSim* sim, uint32_t first, uint32_t nx, uint32_t ny, uint32_t nz, float ox, float oy,
                       float oz, float spacing, float jitterAmp, uint64_t seed);
 // Copy state in/out (host or device pointers, N = params->numParticles on output).
+// Both wait for the simulator's stream, copy on it and wait again, so the copy is
+// complete on return and ordered with the steps before and after it; device memory
+// written on another stream must be complete before the call.
 int pbfSimSetState(PbfSim* sim, const float4* pos, const float4* vel, uint32_t N, int fromDevice);
 int pbfSimGetState(PbfSim* sim, float4* pos, float4* vel, int toDevice);
 
