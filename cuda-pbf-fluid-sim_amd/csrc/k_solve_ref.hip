@@ -225,6 +225,67 @@ __global__ __launch_bounds__(kBS) void k_xsph_ref(float4* __restrict__ Vout, con
     Vout[i] = make_float4(v.x, v.y, v.z, 0.0f);
 }
 
+// Staged XSPH, two particles of one cell a lane (the slot pairs of build_slot_pairs):
+// they share the cell's neighbour rows, so a round's candidate positions are read once
+// for both and a candidate's velocity once when it is a neighbour of either; each
+// particle's sum takes its own neighbours in walk order, as in k_xsph_ref.  Blocks
+// past the pair count return at once; the XCD-aware order is taken over the others.
+template <class Lookup>
+__global__ __launch_bounds__(kBS) void k_xsph_ref_pairs(float4* __restrict__ Vout, const float4* __restrict__ Ps,
+                                                        const float4* __restrict__ Vs,
+                                                        const uint32_t* __restrict__ idxS,
+                                                        const uint32_t* __restrict__ keyS, Lookup lookup,
+                                                        sim::DeviceParams dp, uint32_t N,
+                                                        const uint2* __restrict__ pairs,
+                                                        const uint32_t* __restrict__ pairCount) {
+    const uint32_t np = *pairCount;
+    const uint32_t nb = (np + (uint32_t)kBS - 1u) / (uint32_t)kBS;
+    if (blockIdx.x >= nb) return;
+    const uint32_t p = xcd_tile(blockIdx.x, nb) * kBS + threadIdx.x;
+    if (p >= np) return;
+    const uint2 ab = pairs[p];
+    const uint32_t a = ab.x;
+    const bool hasB = ab.y != 0xFFFFFFFFu;
+    const uint32_t b = hasB ? ab.y : a;
+    const float4 pa = Ps[a], pb = Ps[b], va = Vs[a], vb = Vs[b];
+    const float h2 = dp.kernel.h2, poly6 = dp.kernel.poly6;
+    float sax = 0.f, say = 0.f, saz = 0.f, sbx = 0.f, sby = 0.f, sbz = 0.f;
+    int na = 0, nbr = 0;
+    const CellRows<Lookup> rows(lookup, dp.grid, reach_of(dp.kernel.h, dp.grid.cellSize), keyS[a]);
+    const SlotBuf PS(Ps, N + kSlotPad), VS(Vs, N + kSlotPad);
+    inline_walk_pair(
+        a, b, hasB, rows, h2, [&](uint32_t k, uint32_t u) { return PS.at(k, u); },
+        [&](const float4& pj) { return pair_r2(pa.x - pj.x, pa.y - pj.y, pa.z - pj.z); },
+        [&](const float4& pj) { return pair_r2(pb.x - pj.x, pb.y - pj.y, pb.z - pj.z); },
+        [&](uint32_t k, uint32_t u, bool in) { return VS.at(in ? k : kSlotBufBeyond, u); },
+        [&](const float4& vj, float r2, bool in) {
+            const float hr2 = h2 - r2;
+            const float w = poly6 * hr2 * hr2 * hr2;
+            float tx = sax, ty = say, tz = saz;
+            xsph_pair(tx, ty, tz, vj, va, w);
+            sax = in ? tx : sax;
+            say = in ? ty : say;
+            saz = in ? tz : saz;
+            na += in ? 1 : 0;
+        },
+        [&](const float4& vj, float r2, bool in) {
+            const float hr2 = h2 - r2;
+            const float w = poly6 * hr2 * hr2 * hr2;
+            float tx = sbx, ty = sby, tz = sbz;
+            xsph_pair(tx, ty, tz, vj, vb, w);
+            sbx = in ? tx : sbx;
+            sby = in ? ty : sby;
+            sbz = in ? tz : sbz;
+            nbr += in ? 1 : 0;
+        });
+    const float3 u = xsph_finish(va.x, va.y, va.z, sax, say, saz, na, dp, dp.xsph_c);
+    Vout[idxS[a]] = make_float4(u.x, u.y, u.z, 0.0f);
+    if (hasB) {
+        const float3 v = xsph_finish(vb.x, vb.y, vb.z, sbx, sby, sbz, nbr, dp, dp.xsph_c);
+        Vout[idxS[b]] = make_float4(v.x, v.y, v.z, 0.0f);
+    }
+}
+
 // Staging gather: Ps[s] = (P[i].xyz, lambda[i] or 0), Vs[s] = V[i] (when V), i = idxS[s].
 __global__ __launch_bounds__(kBS) void k_stage(float4* __restrict__ Ps, float4* __restrict__ Vs,
                                                const float4* __restrict__ P, const float* __restrict__ lambda,
@@ -239,12 +300,29 @@ __global__ __launch_bounds__(kBS) void k_stage(float4* __restrict__ Ps, float4* 
 }
 
 // Staging buffers of one stream, sized by EnsureDropInStaging: N + kSlotPad slots each,
-// the pad zeroed once (the walk's padded loads may read it).
+// the pad zeroed once (the walk's padded loads may read it); and what the staged XSPH
+// builds its slot pairs in (the cells of the sorted keys, then their slots two at a
+// time).
 struct Staging {
     float4* P = nullptr;
     float4* V = nullptr;
+    uint32_t* uniq = nullptr;   // N cell keys
+    uint32_t* off = nullptr;    // N + 1 cell offsets
+    uint32_t* count = nullptr;  // cells
+    uint32_t* segOf = nullptr;  // N: cell of each slot
+    uint2* pairs = nullptr;     // N slot pairs
+    uint32_t* pairCount = nullptr;
+    uint32_t* tileCounts = nullptr;   // compact_scratch_elems(N) each
+    uint32_t* tileOffsets = nullptr;
     uint32_t capacity = 0;
 };
+
+void free_staging(Staging& st) {
+    for (void* p : {(void*)st.P, (void*)st.V, (void*)st.uniq, (void*)st.off, (void*)st.count, (void*)st.segOf,
+                    (void*)st.pairs, (void*)st.pairCount, (void*)st.tileCounts, (void*)st.tileOffsets})
+        if (p) (void)hipFree(p);
+    st = Staging{};
+}
 std::mutex g_stage_mu;
 std::unordered_map<hipStream_t, Staging> g_stage;
 
@@ -317,7 +395,10 @@ void launch_xsph(float4* Vout, const float4* Vin, const float4* P, const uint32_
     Staging st;
     if (staging_for(s, N, st)) {
         k_stage<<<ceil_div(N, kBS), kBS, 0, s>>>(st.P, st.V, P, nullptr, Vin, idxS, N);
-        k_xsph_ref<Lookup, true><<<ceil_div(N, kBS), kBS, 0, s>>>(Vout, Vin, P, st.P, st.V, idxS, keyS, lk, dp, N);
+        compact_segments(st.uniq, st.off, st.count, st.segOf, keyS, N, st.tileCounts, st.tileOffsets, s);
+        build_slot_pairs(st.pairs, st.pairCount, st.segOf, st.off, N, st.tileCounts, st.tileOffsets, s);
+        k_xsph_ref_pairs<Lookup><<<ceil_div(N, kBS), kBS, 0, s>>>(Vout, st.P, st.V, idxS, keyS, lk, dp, N, st.pairs,
+                                                                  st.pairCount);
     } else {
         k_xsph_ref<Lookup, false><<<ceil_div(N, kBS), kBS, 0, s>>>(Vout, Vin, P, nullptr, nullptr, idxS, keyS, lk,
                                                                     dp, N);
@@ -331,16 +412,24 @@ extern "C" bool EnsureDropInStaging(uint32_t N, hipStream_t stream) {
     std::lock_guard<std::mutex> lk(g_stage_mu);
     Staging& st = g_stage[stream];
     if (N <= st.capacity) return reserve_stream_compact_scratch(N, stream);
-    if (st.P) (void)hipFree(st.P);
-    if (st.V) (void)hipFree(st.V);
-    st = Staging{};
+    free_staging(st);
     const size_t slots = (size_t)N + kSlotPad;
+    const size_t tiles = compact_scratch_elems(N);
+    // the pad is cleared on the stream the launchers use, then waited for
     if (hipMalloc(&st.P, sizeof(float4) * slots) != hipSuccess ||
         hipMalloc(&st.V, sizeof(float4) * slots) != hipSuccess ||
-        hipMemset(st.P, 0, sizeof(float4) * slots) != hipSuccess ||
-        hipMemset(st.V, 0, sizeof(float4) * slots) != hipSuccess) {
-        if (st.P) (void)hipFree(st.P);
-        if (st.V) (void)hipFree(st.V);
+        hipMalloc(&st.uniq, sizeof(uint32_t) * N) != hipSuccess ||
+        hipMalloc(&st.off, sizeof(uint32_t) * ((size_t)N + 1)) != hipSuccess ||
+        hipMalloc(&st.count, sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&st.segOf, sizeof(uint32_t) * N) != hipSuccess ||
+        hipMalloc(&st.pairs, sizeof(uint2) * N) != hipSuccess ||
+        hipMalloc(&st.pairCount, sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&st.tileCounts, sizeof(uint32_t) * tiles) != hipSuccess ||
+        hipMalloc(&st.tileOffsets, sizeof(uint32_t) * tiles) != hipSuccess ||
+        hipMemsetAsync(st.P, 0, sizeof(float4) * slots, stream) != hipSuccess ||
+        hipMemsetAsync(st.V, 0, sizeof(float4) * slots, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess) {
+        free_staging(st);
         g_stage.erase(stream);
         set_error(PBFX_E_CAPACITY, "EnsureDropInStaging: allocation failed");
         return false;
@@ -354,8 +443,7 @@ extern "C" void ReleaseDropInStaging(hipStream_t stream) {
     std::lock_guard<std::mutex> lk(g_stage_mu);
     const auto it = g_stage.find(stream);
     if (it == g_stage.end()) return;
-    if (it->second.P) (void)hipFree(it->second.P);
-    if (it->second.V) (void)hipFree(it->second.V);
+    free_staging(it->second);
     g_stage.erase(it);
 }
 

@@ -102,10 +102,12 @@ void LaunchXSPHCompact(float4* vel_out, const float4* vel_in, const float4* pos_
 
 // ---- staging for the neighbour launchers (no counterpart in the reference) -----
 // EnsureDropInStaging sizes two cell-sorted staging buffers of N particles for
-// launches on `stream`.  With them, LaunchLambda / LaunchDeltaApply / LaunchXSPH and
-// their compact forms on that stream first gather their inputs into slot order (one
-// extra short kernel) and walk the gathered copy, instead of reading every candidate
-// through indicesSorted: the same pairs in the same order, the same bits, faster.
+// launches on `stream` (and the cell table and slot pairs the staged XSPH builds,
+// about 56 bytes a particle in all).  With them, LaunchLambda / LaunchDeltaApply /
+// LaunchXSPH and their compact forms on that stream first gather their inputs into slot
+// order (one extra short kernel) and walk the gathered copy, instead of reading every
+// candidate through indicesSorted; XSPH also pairs each cell's slots two to a lane
+// (six more short kernels): the same pairs in the same order, the same bits, faster.
 // Launches on other streams, or with more particles than reserved, read through
 // indicesSorted as before.  Like EnsureCellCompactScratch it allocates, so call it
 // before graph capture; growing the reservation frees the old buffers, so a graph
