@@ -18,14 +18,21 @@ using pbfx::set_error;
 
 namespace {
 
-// Event marks recorded after every launch while a profiled (eager) step runs.
+// Event marks recorded after every launch while a profiled (eager) step runs.  The
+// events come from the simulator's pool, created once: creating one per launch kept
+// the host behind the short grid-build launches at the start of a step.
 struct LaunchTimer {
     hipStream_t stream = nullptr;
+    std::vector<hipEvent_t>* pool = nullptr;
     std::vector<const char*> names;
     std::vector<hipEvent_t> events;
     void mark(const char* n) {
-        hipEvent_t e = nullptr;
-        if (hipEventCreate(&e) != hipSuccess) return;
+        if (pool->size() <= events.size()) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreate(&e) != hipSuccess) return;
+            pool->push_back(e);
+        }
+        hipEvent_t e = (*pool)[events.size()];
         (void)hipEventRecord(e, stream);
         names.push_back(n);
         events.push_back(e);
@@ -543,6 +550,9 @@ void Simulator::destroyGraphs() {
 }
 
 void Simulator::shutdown() {
+    if (m_stream) (void)hipStreamSynchronize(m_stream);
+    for (auto& e : m_eventPool) (void)hipEventDestroy(e);
+    m_eventPool.clear();
     destroyGraphs();
     releaseAdjoint();
     releaseScenes();
@@ -720,6 +730,7 @@ bool Simulator::profilePhases(std::vector<std::pair<std::string, float>>& out) {
     if (!m_initialized) return false;
     LaunchTimer timer;
     timer.stream = m_stream;
+    timer.pool = &m_eventPool;
     timer.mark("start");
     t_timer = &timer;
     runPipeline(m_stream);
@@ -730,7 +741,6 @@ bool Simulator::profilePhases(std::vector<std::pair<std::string, float>>& out) {
         (void)hipEventElapsedTime(&ms, timer.events[i - 1], timer.events[i]);
         out.emplace_back(timer.names[i], ms);
     }
-    for (auto& e : timer.events) (void)hipEventDestroy(e);
     if (!ok) return false;
     m_bufs.swapPositionPingPong();
     m_parity ^= 1;

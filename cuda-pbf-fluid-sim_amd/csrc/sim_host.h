@@ -297,6 +297,7 @@ private:
     SimulationContext m_ctx;
     hipStream_t m_stream = nullptr;
     hipGraphExec_t m_exec[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> m_eventPool;  // events of profilePhases, reused step after step
     int m_parity = 0;
     uint64_t m_frameIndex = 0;
     bool m_graphDirty = true;
