@@ -147,11 +147,14 @@ class Simulator:
 
     def positions_view(self):
         """(N, 4) float32 CUDA tensor viewing the current position buffer (no copy).  The
-        buffer alternates between steps; take a new view after stepping."""
+        buffer alternates between steps; take a new view after stepping.  Steps run on the
+        simulator's own stream: synchronize() before reading the view on another stream,
+        and finish writes to it there before the next step."""
         return _device_view(_lib.lib().pbfSimDevicePositions(self._h), self.n, self)
 
     def velocities_view(self):
-        """(N, 4) float32 CUDA tensor viewing the velocity buffer (no copy)."""
+        """(N, 4) float32 CUDA tensor viewing the velocity buffer (no copy; the same stream
+        rule as positions_view)."""
         return _device_view(_lib.lib().pbfSimDeviceVelocities(self._h), self.n, self)
 
     def sorted_grid(self):
