@@ -320,6 +320,13 @@ void DeviceBuffers::swapPositionPingPong() {
 bool GridBuffers::allocate(uint32_t cap, uint32_t nCells, GridKind k, uint32_t tLog2, int reach, bool reordered) {
     release();
     if (cap == 0) cap = 1;
+    // The walks index the span table (rows x capacity entries) with 32-bit integers; a cell
+    // size far below h (reach = ceil(h / cellSize) cells a side) would need more rows.
+    const uint64_t rows = (2 * (uint64_t)std::max(reach, 1) + 1) * (2 * (uint64_t)std::max(reach, 1) + 1);
+    if (rows * cap >= (uint64_t(1) << 32) || rows > 0x7FFFFFFFull) {
+        set_error(PBFX_E_CAPACITY, "grid: neighbour rows x particles exceed 2^32 (cell size too small for h)");
+        return false;
+    }
     kind = k;
     capacity = cap;
     numCells = nCells;
@@ -334,7 +341,7 @@ bool GridBuffers::allocate(uint32_t cap, uint32_t nCells, GridKind k, uint32_t t
     // on both paths, by the density statistics
     const uint32_t tiles = pbfx::compact_scratch_elems(cap);
     spans.reach = reach;
-    spans.rows = (2 * reach + 1) * (2 * reach + 1);
+    spans.rows = (int)rows;
     spans.stride = cap;
     ok = ok && dalloc(d_segOf, cap) && dalloc(d_blockCounts, tiles) && dalloc(d_blockOffsets, tiles) &&
          dalloc(spans.spans, (size_t)spans.rows * cap);

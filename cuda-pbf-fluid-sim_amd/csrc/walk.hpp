@@ -303,27 +303,22 @@ __device__ __forceinline__ void mask_walk(bool valid, uint32_t s, uint32_t m, co
                 pop_one();
             } while (__any(qn > keep));
     };
+    // Span-table entries are read at 32-bit indices r * stride + m: the table holds fewer
+    // than 2^32 entries (GridBuffers::allocate refuses larger ones).
     uint32_t total = 0;
     if (valid) {
-        for (int r = 0; r < st.rows; ++r) {
-            const uint2 sp = st.spans[(uint64_t)r * st.stride + m];
+        uint32_t e = m;
+        for (int r = 0; r < st.rows; ++r, e += st.stride) {
+            const uint2 sp = st.spans[e];
             total += sp.y - sp.x;
         }
     }
-    // row cursor: candidate indices [rowStart, rowStart + rowLen) are slots rowLo + (ci - rowStart);
-    // candidate indices only grow, so the cursor only moves forward
-    int r = -1;
-    uint32_t rowStart = 0, rowLen = 0, rowLo = 0;
-    auto slot_of = [&](uint32_t ci) {
-        while (ci >= rowStart + rowLen) {
-            rowStart += rowLen;
-            ++r;
-            const uint2 sp = st.spans[(uint64_t)r * st.stride + m];
-            rowLo = sp.x;
-            rowLen = sp.y - sp.x;
-        }
-        return rowLo + (ci - rowStart);
-    };
+    // Row cursor: the current row's candidate indices end at rowEnd, and its candidate ci
+    // is slot ci + delta; spanAt indexes the next row's span.  A round takes candidates of
+    // one row only, so the cursor moves before a round's candidates are taken, once per
+    // round at most, instead of being checked for each of them.  A round that reaches
+    // the end of its row takes fewer than kUnroll candidates.
+    uint32_t rowEnd = 0, delta = 0, spanAt = m;
     const uint32_t nw = valid ? min((total + 31u) >> 5, kMaskWords) : 0u;
     const uint32_t covered = kMaskWords * 32u;
     uint32_t w = 0;
@@ -342,8 +337,21 @@ __device__ __forceinline__ void mask_walk(bool valid, uint32_t s, uint32_t m, co
                 word = next;
                 next = (w + 1 < nw) ? mk.words[(uint64_t)(w + 1) * mk.stride + s] : 0u;
             }
+            // move the cursor to the row of the round's first candidate (every candidate
+            // index is below total, the end of the last row, so the cursor stops in a row)
+            const uint32_t first = word != 0u ? w * 32u + (uint32_t)__ffs((int)word) - 1u : tail;
+            if (first < total) {
+                while (first >= rowEnd) {
+                    const uint2 sp = st.spans[spanAt];
+                    spanAt += st.stride;
+                    delta = sp.x - rowEnd;
+                    rowEnd += sp.y - sp.x;
+                }
+            }
         }
         const bool fromTail = word == 0u;
+        const uint32_t wbase = w * 32u;
+        const uint32_t tailEnd = min(total, rowEnd);
         // ks[u] for u >= cnt repeats the last taken slot (slot 0 when none was taken), so
         // every load below is in bounds and branch-free; taken candidates come first
         uint32_t ks[kUnroll];
@@ -353,16 +361,16 @@ __device__ __forceinline__ void mask_walk(bool valid, uint32_t s, uint32_t m, co
             bool take;
             uint32_t ci;
             if (!fromTail) {
-                take = word != 0u;
-                ci = w * 32u + (uint32_t)__ffs((int)word) - 1u;
-                word &= word - 1u;
+                ci = wbase + (uint32_t)__ffs((int)word) - 1u;
+                take = (word != 0u) & (ci < rowEnd);
+                if (take) word &= word - 1u;
             } else {
-                take = tail < total;
                 ci = tail;
+                take = tail < tailEnd;
                 tail += take ? 1u : 0u;
             }
             if (active && take) {
-                prev = slot_of(ci);
+                prev = ci + delta;
                 ++cnt;
             }
             ks[u] = prev;
