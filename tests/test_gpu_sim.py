@@ -450,6 +450,26 @@ def test_set_params_structural_refusals(oracle, dev):
     del ta, tb
 
 
+@pytest.mark.parametrize("grid,path", [("dense", "reordered"), ("hashed", "reordered"), ("dense", "reference")])
+def test_cell_size_far_below_h_is_refused(dev, grid, path):
+    """A cell size 5000 times below h needs (2 * 5000 + 1)^2 neighbour rows a particle: the
+    span table would hold more than 2^32 entries, so creation fails with a capacity error
+    (no overflowed row count, no undersized table)."""
+    from pbfx import _lib
+    from pbfx.sim import Simulator
+    scene = P.small_block((8, 8, 8), table_log2=16)
+    sp = P.build_sim_params(scene)
+    sp.kernel.h = sp.grid.cellSize * 5000.0
+    with pytest.raises(_lib.PbfError, match="error 1003"):
+        Simulator(sp, grid=grid, table_log2=16, path=path)
+    # the library stays usable
+    sim = _sim(scene, grid, path=path)
+    try:
+        sim.step(1)
+    finally:
+        sim.close()
+
+
 @pytest.mark.parametrize("dims", [(16, 16, 16), (13, 11, 7)])
 @pytest.mark.parametrize("path,grid", [("reordered", "hashed"), ("reordered", "dense"), ("reference", "compact")])
 def test_simulator_grid_build_matches_oracle(oracle, dev, dims, path, grid):
