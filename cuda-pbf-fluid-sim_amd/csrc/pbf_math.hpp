@@ -198,7 +198,7 @@ struct LambdaAcc {
     int n = 0;
 };
 
-// One in-range neighbour (r2 <= h2 already tested) of the lambda sum.  kFixed: the
+// One in-range neighbour (range already tested: r2 <= h2, or NaN) of the lambda sum.  kFixed: the
 // caller has established pair_fixed(c).
 template <bool kFixed = false>
 __device__ __forceinline__ void lambda_pair(LambdaAcc& a, float rx, float ry, float rz, float r2,

@@ -71,6 +71,12 @@ constexpr int kQD = 16;          // FIFO depth per lane
 constexpr uint32_t kUnroll = 4;  // candidates per round of a lane's own walk
 static_assert(kSlotPad + 1 >= kUnroll, "a round may load kUnroll - 1 slots past the last particle");
 
+// The reference's range test: a candidate is skipped only when r2 > h2
+// (pbf_lambda.cu:69, pbf_delta_apply.cu:68, xsph.cu:64), so a NaN distance (a NaN
+// position) counts as in range and its NaN enters the sums.  Every range test of the
+// walks, and the candidate mask's skin test, takes this form.
+__device__ __forceinline__ bool in_range(float r2, float h2) { return !(r2 > h2); }
+
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o));
@@ -88,7 +94,7 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 // positions.  The passes that read the mask (the lambda and delta passes of
 // iterations 1 .. I-1) see positions at most I - 1 corrections later, each clamped
 // to maxDisp, so a pair outside the skin cannot have come within h.  Those passes
-// visit only the set bits, in walk order, and still test r2 <= h2.  (XSPH, after
+// visit only the set bits, in walk order, and still test the range.  (XSPH, after
 // the last correction and the box clamp, walks its candidates directly.)
 // The mask covers the first 32 * kMaskWords candidates; candidates past that are
 // tested directly.
@@ -141,7 +147,7 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, const Rows
     // one own candidate: record its skin bit, queue it when in range
     auto visit = [&](uint32_t k, float r2, uint32_t& hi, uint32_t lo) {
         if (kBuild) {
-            if (r2 <= mk.rs2) word |= 1u << (cidx & 31);
+            if (in_range(r2, mk.rs2)) word |= 1u << (cidx & 31);
             ++cidx;
             if ((cidx & 31) == 0) {
                 const uint32_t w = (cidx >> 5) - 1;
@@ -149,7 +155,7 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, const Rows
                 word = 0;
             }
         }
-        if (k != s && r2 <= h2) {
+        if (k != s && in_range(r2, h2)) {
             push(fetch.id(k));
             if (cap > 0 && found >= cap) hi = lo;
         }
@@ -186,8 +192,8 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, const Rows
 #pragma unroll
                 for (uint32_t u = 0; u < kUnroll; ++u) {
                     const float r2 = dist2(pj[u]);
-                    bits |= (r2 <= h2) ? 1u << u : 0u;
-                    if (kBuild) skin |= (r2 <= mk.rs2) ? 1u << u : 0u;
+                    bits |= in_range(r2, h2) ? 1u << u : 0u;
+                    if (kBuild) skin |= in_range(r2, mk.rs2) ? 1u << u : 0u;
                 }
                 bits &= inSpan & ~self;
                 skin &= inSpan;
@@ -245,8 +251,8 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, const Rows
                 const uint32_t n = min(64u, uhi - base);
                 const uint32_t n0 = min(n, 32u);
                 uint32_t m0 = 0u, m1 = 0u;
-                for (uint32_t t = 0; t < n0; ++t) m0 |= (dist2(buf[t]) <= h2) ? (1u << t) : 0u;
-                for (uint32_t t = 32; t < n; ++t) m1 |= (dist2(buf[t]) <= h2) ? (1u << (t - 32)) : 0u;
+                for (uint32_t t = 0; t < n0; ++t) m0 |= in_range(dist2(buf[t]), h2) ? (1u << t) : 0u;
+                for (uint32_t t = 32; t < n; ++t) m1 |= in_range(dist2(buf[t]), h2) ? (1u << (t - 32)) : 0u;
                 __builtin_amdgcn_wave_barrier();
                 const uint32_t a = lo > base ? min(lo - base, n) : 0u;
                 const uint32_t b = hi > base ? min(hi - base, n) : 0u;
@@ -382,7 +388,7 @@ __device__ __forceinline__ void mask_walk(bool valid, uint32_t s, uint32_t m, co
         uint32_t bits = 0u;
 #pragma unroll
         for (uint32_t u = 0; u < kUnroll; ++u)
-            bits |= ((uint32_t)(u < cnt) & (uint32_t)(ks[u] != s) & (uint32_t)(dist2(pj[u]) <= h2)) << u;
+            bits |= ((uint32_t)(u < cnt) & (uint32_t)(ks[u] != s) & (uint32_t)in_range(dist2(pj[u]), h2)) << u;
         if (cap > 0) {
             const int room = cap - found;
             while (__popc(bits) > room) bits &= ~(1u << (31 - __clz(bits)));
@@ -405,7 +411,7 @@ __device__ __forceinline__ void mask_walk(bool valid, uint32_t s, uint32_t m, co
 // walks its own spans row by row, kUnroll candidates a round: load(k, u) reads the
 // positions of slots k + u together, then gather(k, u, in) reads what the term needs
 // besides, for the neighbours only (in: in the span, not the lane's own slot,
-// r2 <= h2), and take() gets every candidate in walk order with its flag and selects
+// in_range), and take() gets every candidate in walk order with its flag and selects
 // the neighbours' terms into its sums.  No lane waits on another, so a lane whose
 // spans end early simply stops.  Slot k + u may lie up to kUnroll - 1 slots past the
 // span and past the last particle.
@@ -428,7 +434,7 @@ __device__ __forceinline__ void inline_walk(bool valid, uint32_t s, const Rows& 
 #pragma unroll
             for (uint32_t u = 0; u < kUnroll; ++u) {
                 r2[u] = dist2(c[u]);
-                in[u] = (u < cnt) & (k + u != s) & (r2[u] <= h2);
+                in[u] = (u < cnt) & (k + u != s) & in_range(r2[u], h2);
             }
 #pragma unroll
             for (uint32_t u = 0; u < kUnroll; ++u) g[u] = gather(k, u, in[u]);
@@ -462,8 +468,8 @@ __device__ __forceinline__ void inline_walk_pair(uint32_t a, uint32_t b, bool ha
             for (uint32_t u = 0; u < kUnroll; ++u) {
                 ra[u] = dist2a(c[u]);
                 rb[u] = dist2b(c[u]);
-                ia[u] = (u < cnt) & (k + u != a) & (ra[u] <= h2);
-                ib[u] = hasB & (u < cnt) & (k + u != b) & (rb[u] <= h2);
+                ia[u] = (u < cnt) & (k + u != a) & in_range(ra[u], h2);
+                ib[u] = hasB & (u < cnt) & (k + u != b) & in_range(rb[u], h2);
             }
 #pragma unroll
             for (uint32_t u = 0; u < kUnroll; ++u) g[u] = gather(k, u, ia[u] | ib[u]);
