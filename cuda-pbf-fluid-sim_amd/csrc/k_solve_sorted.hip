@@ -31,13 +31,29 @@ __device__ __forceinline__ uint32_t slot_of_thread() {
     return xcd_tile(blockIdx.x, gridDim.x) * BS + threadIdx.x;
 }
 
+// A velocity whose components are all below 2^126 in magnitude (NaN fails the test):
+// the difference of two such velocities is finite.  The kernels that write the step's
+// velocities set kStepVelocityUnbounded for any other; XSPH reads it.
+__device__ __forceinline__ bool velocity_bounded(float x, float y, float z) {
+    return fabsf(x) < 0x1p126f && fabsf(y) < 0x1p126f && fabsf(z) < 0x1p126f;
+}
+__device__ __forceinline__ void note_velocity(uint32_t* stepFlags, float x, float y, float z) {
+    if (stepFlags && !velocity_bounded(x, y, z)) atomicOr(stepFlags, kStepVelocityUnbounded);
+}
+// Whether the step's flags say a solver pass wrote a NaN position (the candidate mask
+// then no longer bounds the neighbours; see kStepNanPosition).
+__device__ __forceinline__ bool nan_position_seen(const uint32_t* stepFlags) {
+    return stepFlags != nullptr && (*stepFlags & kStepNanPosition) != 0u;
+}
+
 __global__ __launch_bounds__(kBS) void k_predict_hash(const float4* __restrict__ X, const float4* __restrict__ V,
                                                       float4* __restrict__ Xn, uint32_t* __restrict__ keys,
                                                       uint32_t* __restrict__ epoch, float3 g, float dt,
                                                       sim::GridBounds grid, uint32_t N, Scenes sc,
-                                                      uint32_t* __restrict__ clampBits) {
+                                                      uint32_t* __restrict__ clampBits, uint32_t* __restrict__ stepFlags) {
     const uint32_t i = blockIdx.x * kBS + threadIdx.x;
     if (epoch && i == 0) epoch[0] = epoch[0] + 1u;
+    if (stepFlags && i == 0) stepFlags[0] = 0u;  // the step's flags start clear
     if (i >= N) return;
     uint32_t keyBase = 0;
     if (sc.params) {
@@ -335,7 +351,8 @@ template <int BS>
 __global__ __launch_bounds__(BS) void k_lambda_sorted(float4* Ps, const uint32_t* __restrict__ segOf, SpanTable st,
                                                        CandMask mk, int useMask, sim::DeviceParams dp, uint32_t N,
                                                        float4* __restrict__ recA, uint2* __restrict__ recB,
-                                                       const uint32_t* __restrict__ perm) {
+                                                       const uint32_t* __restrict__ perm,
+                                                       const uint32_t* __restrict__ stepFlags) {
     __shared__ uint32_t sQ[kQD * BS];
     __shared__ float4 sBuf[BS];
     const uint32_t s = lane_slot(slot_of_thread<BS>(), N, perm);
@@ -359,7 +376,7 @@ __global__ __launch_bounds__(BS) void k_lambda_sorted(float4* Ps, const uint32_t
             lambda_pair<decltype(form)::value>(a, rx, ry, rz, pair_r2(rx, ry, rz), c);
             if (recB && a.n == cap) cut = k;
         };
-        if (useMask)
+        if (useMask && !nan_position_seen(stepFlags))
             mask_walk<BS>(valid, s, m, st, mk, PS, sQ + threadIdx.x, dp.maxNeighbors, c.h2, dist2, apply);
         else
             neighbor_walk<false, BS>(valid, s, SpanRows(st, m), mk, SortedFetch{PS}, sBuf + (threadIdx.x & ~63u),
@@ -384,7 +401,8 @@ __global__ __launch_bounds__(BS) void k_delta_sorted(const float4* __restrict__ 
                                                       int last, const float4* __restrict__ X,
                                                       const uint32_t* __restrict__ idxS, float4* __restrict__ Vs,
                                                       float invDt, Scenes sc, uint32_t* __restrict__ clampBits,
-                                                      float4* __restrict__ recU, const uint32_t* __restrict__ perm) {
+                                                      float4* __restrict__ recU, const uint32_t* __restrict__ perm,
+                                                      uint32_t* __restrict__ stepFlags) {
     __shared__ uint32_t sQ[kQD * BS];
     __shared__ float4 sBuf[BS];
     const uint32_t s = lane_slot(slot_of_thread<BS>(), N, perm);
@@ -406,12 +424,12 @@ __global__ __launch_bounds__(BS) void k_delta_sorted(const float4* __restrict__ 
             const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
             delta_pair<decltype(form)::value>(dx, dy, dz, rx, ry, rz, pair_r2(rx, ry, rz), pi.w, pj.w, c);
         };
-        if (maskMode == 2)
+        if (maskMode == 2 && !nan_position_seen(stepFlags))
             mask_walk<BS>(valid, s, m, st, mk, PA, sQ + threadIdx.x, 0, c.h2, dist2, apply);
         else if (maskMode == 1)
             neighbor_walk<true, BS>(valid, s, SpanRows(st, m), mk, SortedFetch{PA}, buf, sQ + threadIdx.x, 0, c.h2,
                                     dist2, apply);
-        else
+        else  // maskMode 0, or 2 after a NaN position
             neighbor_walk<false, BS>(valid, s, SpanRows(st, m), mk, SortedFetch{PA}, buf, sQ + threadIdx.x, 0, c.h2,
                                      dist2, apply);
     });
@@ -423,6 +441,8 @@ __global__ __launch_bounds__(BS) void k_delta_sorted(const float4* __restrict__ 
     float px = pi.x + d.x, py = pi.y + d.y, pz = pi.z + d.z;
     if (!last) {
         Pb[s] = make_float4(px, py, pz, 0.0f);
+        if (stepFlags && (__builtin_isnan(px) || __builtin_isnan(py) || __builtin_isnan(pz)))
+            atomicOr(stepFlags, kStepNanPosition);
         return;
     }
     // post-solve boundary (phases.cpp:94) and velocity (phases.cpp:102-105)
@@ -432,12 +452,15 @@ __global__ __launch_bounds__(BS) void k_delta_sorted(const float4* __restrict__ 
     clamp_box(px, py, pz, ux, uy, uz, gb, 0.0f);
     Pb[s] = make_float4(px, py, pz, 1.0f);
     const float4 x = X[idxS[s]];
-    Vs[s] = make_float4((px - x.x) * invDt, (py - x.y) * invDt, (pz - x.z) * invDt, 0.0f);
+    const float vx = (px - x.x) * invDt, vy = (py - x.y) * invDt, vz = (pz - x.z) * invDt;
+    Vs[s] = make_float4(vx, vy, vz, 0.0f);
+    note_velocity(stepFlags, vx, vy, vz);
 }
 
 __global__ __launch_bounds__(kBS) void k_finalize_sorted(float4* __restrict__ Ps, const float4* __restrict__ X,
                                                          const uint32_t* __restrict__ idxS, float4* __restrict__ Vs,
-                                                         sim::GridBounds g, float invDt, uint32_t N, Scenes sc) {
+                                                         sim::GridBounds g, float invDt, uint32_t N, Scenes sc,
+                                                         uint32_t* __restrict__ stepFlags) {
     const uint32_t s = blockIdx.x * kBS + threadIdx.x;
     if (s >= N) return;
     if (sc.params) g = sc.params[sc.sceneOfSlot[s]].grid;
@@ -446,7 +469,9 @@ __global__ __launch_bounds__(kBS) void k_finalize_sorted(float4* __restrict__ Ps
     clamp_box(px, py, pz, ux, uy, uz, g, 0.0f);
     Ps[s] = make_float4(px, py, pz, 1.0f);
     const float4 x = X[idxS[s]];
-    Vs[s] = make_float4((px - x.x) * invDt, (py - x.y) * invDt, (pz - x.z) * invDt, 0.0f);
+    const float vx = (px - x.x) * invDt, vy = (py - x.y) * invDt, vz = (pz - x.z) * invDt;
+    Vs[s] = make_float4(vx, vy, vz, 0.0f);
+    note_velocity(stepFlags, vx, vy, vz);
 }
 
 // XSPH's pair term is a few products, so its walk evaluates every candidate in place
@@ -460,6 +485,13 @@ __global__ __launch_bounds__(kBS) void k_finalize_sorted(float4* __restrict__ Ps
 // Both arrays carry kSlotPad padding slots past the last particle.  The grid covers N
 // pairs, the most there can be; blocks past the step's pair count return at once, and
 // the XCD-aware order is taken over the blocks that have pairs.
+//
+// A candidate that is not a neighbour must leave the sums as they are.  In general each
+// sum selects its old value (three selects a particle per candidate).  When the step's
+// flags have kStepVelocityUnbounded clear (every velocity of the step has components below
+// 2^126, so every difference v_j - v_i is finite), the weight is selected instead, W or +0,
+// and the fused update runs on every candidate: fma(d, +0, s) == s for finite d, since the
+// sums start at +0 and a round-to-nearest sum is -0 only when both of its terms are.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_xsph_scatter(const float4* __restrict__ Ps, const float4* __restrict__ Vs,
                                                      const uint32_t* __restrict__ segOf, SpanTable st,
@@ -467,7 +499,8 @@ __global__ __launch_bounds__(BS) void k_xsph_scatter(const float4* __restrict__ 
                                                      const uint2* __restrict__ pairs,
                                                      const uint32_t* __restrict__ pairCount,
                                                      const uint32_t* __restrict__ idxS, float4* __restrict__ Xn,
-                                                     float4* __restrict__ V, Scenes sc, float* __restrict__ recGm) {
+                                                     float4* __restrict__ V, Scenes sc, float* __restrict__ recGm,
+                                                     const uint32_t* __restrict__ stepFlags) {
     const uint32_t np = *pairCount;
     const uint32_t nb = (np + (uint32_t)BS - 1u) / (uint32_t)BS;
     if (blockIdx.x >= nb) return;
@@ -494,31 +527,36 @@ __global__ __launch_bounds__(BS) void k_xsph_scatter(const float4* __restrict__ 
         const SlotBuf PS(Ps, N + kSlotPad), VS(Vs, N + kSlotPad);
         float sax = 0.f, say = 0.f, saz = 0.f, sbx = 0.f, sby = 0.f, sbz = 0.f;
         int na = 0, nb2 = 0;
-        inline_walk_pair(
-            a, b, hasB, SpanRows(st, segOf[a]), h2, [&](uint32_t k, uint32_t u) { return PS.at(k, u); },
-            [&](const float4& pj) { return pair_r2(pa.x - pj.x, pa.y - pj.y, pa.z - pj.z); },
-            [&](const float4& pj) { return pair_r2(pb.x - pj.x, pb.y - pj.y, pb.z - pj.z); },
-            [&](uint32_t k, uint32_t u, bool in) { return VS.at(in ? k : kSlotBufBeyond, u); },
-            [&](const float4& vj, float r2, bool in) {
-                const float hr2 = h2 - r2;
-                const float w = poly6 * hr2 * hr2 * hr2;
-                float tx = sax, ty = say, tz = saz;
-                xsph_pair(tx, ty, tz, vj, va, w);
-                sax = in ? tx : sax;
-                say = in ? ty : say;
-                saz = in ? tz : saz;
-                na += in ? 1 : 0;
-            },
-            [&](const float4& vj, float r2, bool in) {
-                const float hr2 = h2 - r2;
-                const float w = poly6 * hr2 * hr2 * hr2;
-                float tx = sbx, ty = sby, tz = sbz;
-                xsph_pair(tx, ty, tz, vj, vb, w);
-                sbx = in ? tx : sbx;
-                sby = in ? ty : sby;
-                sbz = in ? tz : sbz;
-                nb2 += in ? 1 : 0;
-            });
+        // one neighbour's term into the sums (sx, sy, sz), by the selected weight or
+        // by selecting the old sums
+        auto take = [&](auto weighted, float& sx, float& sy, float& sz, int& n, const float4& vj, const float4& vi,
+                        float r2, bool in) {
+            const float hr2 = h2 - r2;
+            const float w = poly6 * hr2 * hr2 * hr2;
+            if constexpr (decltype(weighted)::value) {
+                xsph_pair(sx, sy, sz, vj, vi, in ? w : 0.0f);
+            } else {
+                float tx = sx, ty = sy, tz = sz;
+                xsph_pair(tx, ty, tz, vj, vi, w);
+                sx = in ? tx : sx;
+                sy = in ? ty : sy;
+                sz = in ? tz : sz;
+            }
+            n += in ? 1 : 0;
+        };
+        auto walk = [&](auto weighted) {
+            inline_walk_pair(
+                a, b, hasB, SpanRows(st, segOf[a]), h2, [&](uint32_t k, uint32_t u) { return PS.at(k, u); },
+                [&](const float4& pj) { return pair_r2(pa.x - pj.x, pa.y - pj.y, pa.z - pj.z); },
+                [&](const float4& pj) { return pair_r2(pb.x - pj.x, pb.y - pj.y, pb.z - pj.z); },
+                [&](uint32_t k, uint32_t u, bool in) { return VS.at(in ? k : kSlotBufBeyond, u); },
+                [&](const float4& vj, float r2, bool in) { take(weighted, sax, say, saz, na, vj, va, r2, in); },
+                [&](const float4& vj, float r2, bool in) { take(weighted, sbx, sby, sbz, nb2, vj, vb, r2, in); });
+        };
+        if (stepFlags != nullptr && (*stepFlags & kStepVelocityUnbounded) == 0u)
+            walk(std::true_type{});
+        else
+            walk(std::false_type{});
         if (recGm) {
             const float mor = (dp.restDensity > 0.f) ? (dp.particleMass / dp.restDensity) : 0.f;
             recGm[a] = xsph_gate(na, dp) * mor;
@@ -563,10 +601,11 @@ float mask_skin_radius2(const sim::SimParams& p) {
 }
 
 void launch_predict_hash(const float4* X, const float4* V, float4* Xn, uint32_t* keys, uint32_t* epoch,
-                         const sim::SimParams& p, const Scenes& sc, hipStream_t s, uint32_t* clampBits) {
+                         const sim::SimParams& p, const Scenes& sc, hipStream_t s, uint32_t* clampBits,
+                         uint32_t* stepFlags) {
     const uint32_t N = p.numParticles;
     k_predict_hash<<<blocks_for(N), kBS, 0, s>>>(X, V, Xn, keys, epoch, p.gravity, p.dt, p.grid, N, sc,
-                                                 clampBits);
+                                                 clampBits, stepFlags);
     check_launch("predict_hash");
 }
 
@@ -600,7 +639,7 @@ void launch_spans_gather(CellLookup lookup, const float4* Xn, const uint32_t* id
 
 void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st, const CandMask& mk,
                           bool useMask, const sim::DeviceParams& dp, uint32_t N, hipStream_t s, float4* recA,
-                          uint2* recB, const uint32_t* perm) {
+                          uint2* recB, const uint32_t* perm, const uint32_t* stepFlags) {
     if (N == 0) return;
     if ((recA == nullptr) != (recB == nullptr)) {
         set_error(PBFX_E_INVALID_ARGUMENT, "launch_lambda_sorted: recA and recB go together");
@@ -608,24 +647,27 @@ void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st
     }
     const int um = useMask ? 1 : 0;
     if (st.block == 128)
-        k_lambda_sorted<128><<<ceil_div(N, 128u), 128, 0, s>>>(Ps, segOf, st, mk, um, dp, N, recA, recB, perm);
+        k_lambda_sorted<128><<<ceil_div(N, 128u), 128, 0, s>>>(Ps, segOf, st, mk, um, dp, N, recA, recB, perm,
+                                                               stepFlags);
     else
-        k_lambda_sorted<256><<<ceil_div(N, 256u), 256, 0, s>>>(Ps, segOf, st, mk, um, dp, N, recA, recB, perm);
+        k_lambda_sorted<256><<<ceil_div(N, 256u), 256, 0, s>>>(Ps, segOf, st, mk, um, dp, N, recA, recB, perm,
+                                                               stepFlags);
     check_launch("lambda_sorted");
 }
 
 void launch_delta_sorted(const float4* Pa, float4* Pb, const uint32_t* segOf, const SpanTable& st,
                          const CandMask& mk, int maskMode, const sim::DeviceParams& dp, uint32_t N, bool last,
                          const float4* X, const uint32_t* idxS, float4* Vs, float invDt, const Scenes& sc,
-                         hipStream_t s, uint32_t* clampBits, float4* recU, const uint32_t* perm) {
+                         hipStream_t s, uint32_t* clampBits, float4* recU, const uint32_t* perm,
+                         uint32_t* stepFlags) {
     if (N == 0) return;
     const int lastPass = last ? 1 : 0;
     if (st.block == 128)
         k_delta_sorted<128><<<ceil_div(N, 128u), 128, 0, s>>>(Pa, Pb, segOf, st, mk, maskMode, dp, N, lastPass, X,
-                                                              idxS, Vs, invDt, sc, clampBits, recU, perm);
+                                                              idxS, Vs, invDt, sc, clampBits, recU, perm, stepFlags);
     else
         k_delta_sorted<256><<<ceil_div(N, 256u), 256, 0, s>>>(Pa, Pb, segOf, st, mk, maskMode, dp, N, lastPass, X,
-                                                              idxS, Vs, invDt, sc, clampBits, recU, perm);
+                                                              idxS, Vs, invDt, sc, clampBits, recU, perm, stepFlags);
     check_launch("delta_sorted");
 }
 
@@ -638,23 +680,23 @@ void launch_lane_order(const float4* Ps, uint32_t N, const sim::GridBounds& grid
 
 void launch_finalize_sorted(float4* Ps, const float4* X, const uint32_t* idxS, float4* Vs,
                             const sim::GridBounds& grid, float invDt, uint32_t N, const Scenes& sc,
-                            hipStream_t s) {
+                            hipStream_t s, uint32_t* stepFlags) {
     if (N == 0) return;
-    k_finalize_sorted<<<blocks_for(N), kBS, 0, s>>>(Ps, X, idxS, Vs, grid, invDt, N, sc);
+    k_finalize_sorted<<<blocks_for(N), kBS, 0, s>>>(Ps, X, idxS, Vs, grid, invDt, N, sc, stepFlags);
     check_launch("finalize_sorted");
 }
 
 void launch_xsph_scatter(const float4* Ps, const float4* Vs, const uint32_t* segOf, const SpanTable& st,
                          const sim::DeviceParams& dp, uint32_t N, const uint2* pairs, const uint32_t* pairCount,
                          const uint32_t* idxS, float4* Xn, float4* V, const Scenes& sc, hipStream_t s,
-                         float* recGm) {
+                         float* recGm, const uint32_t* stepFlags) {
     if (N == 0) return;
     if (st.block == 128)
         k_xsph_scatter<128><<<ceil_div(N, 128u), 128, 0, s>>>(Ps, Vs, segOf, st, dp, N, pairs, pairCount, idxS, Xn,
-                                                              V, sc, recGm);
+                                                              V, sc, recGm, stepFlags);
     else
         k_xsph_scatter<256><<<ceil_div(N, 256u), 256, 0, s>>>(Ps, Vs, segOf, st, dp, N, pairs, pairCount, idxS, Xn,
-                                                              V, sc, recGm);
+                                                              V, sc, recGm, stepFlags);
     check_launch("xsph_scatter");
 }
 

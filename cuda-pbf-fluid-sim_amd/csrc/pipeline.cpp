@@ -156,7 +156,7 @@ void RefPost(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_t s
 void SortedPredict(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_t s) {
     pbfx::launch_predict_hash(b.d_pos, b.d_vel, b.d_pos_pred, g.d_cellKeys,
                               g.kind == GridKind::Hashed ? g.table.epoch : nullptr, p, g.scenes, s,
-                              b.rec.predClamp);
+                              b.rec.predClamp, g.d_stepFlags);
     mark("predict_hash");
 }
 
@@ -198,7 +198,7 @@ void SortedSolve(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream
     const float invDt = 1.0f / p.dt;
     if (p.solverIters <= 0) {
         pbfx::launch_finalize_sorted(b.d_pos_sorted[0], b.d_pos, g.d_indices_sorted, b.d_vel_sorted, p.grid, invDt,
-                                     N, g.scenes, s);
+                                     N, g.scenes, s, g.d_stepFlags);
         mark("finalize");
         return;
     }
@@ -214,12 +214,14 @@ void SortedSolve(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream
         float4* Pb = rec ? b.rec.P[it + 1] : b.d_pos_sorted[cur ^ 1];
         const bool last = it == p.solverIters - 1;
         pbfx::launch_lambda_sorted(Pa, g.d_segOf, g.spans, mk, useMask && it > 0, dp, N, s,
-                                   rec ? b.rec.lamA[it] : nullptr, rec ? b.rec.lamB[it] : nullptr, perm);
+                                   rec ? b.rec.lamA[it] : nullptr, rec ? b.rec.lamB[it] : nullptr, perm,
+                                   g.d_stepFlags);
         mark("lambda");
         const int maskMode = !useMask ? 0 : (it == 0 ? 1 : 2);
         pbfx::launch_delta_sorted(Pa, Pb, g.d_segOf, g.spans, mk, maskMode, dp, N, last, b.d_pos, g.d_indices_sorted,
                                   rec ? b.rec.Vs : b.d_vel_sorted, invDt, g.scenes, s,
-                                  last ? b.rec.finalClamp : nullptr, rec ? b.rec.dispU[it] : nullptr, perm);
+                                  last ? b.rec.finalClamp : nullptr, rec ? b.rec.dispU[it] : nullptr, perm,
+                                  g.d_stepFlags);
         mark(last ? "delta_last" : "delta");
         cur ^= 1;
     }
@@ -234,7 +236,7 @@ void SortedPost(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream_
     pbfx::launch_xsph_scatter(rec ? b.rec.P[p.solverIters] : b.d_pos_sorted[fin], rec ? b.rec.Vs : b.d_vel_sorted,
                               g.d_segOf, g.spans, MakeDeviceParams(p), p.numParticles, g.d_slotPairs,
                               g.d_slotPairCount, g.d_indices_sorted, b.d_pos_pred, b.d_vel, g.scenes, s,
-                              rec ? b.rec.xsphGm : nullptr);
+                              rec ? b.rec.xsphGm : nullptr, g.d_stepFlags);
     mark("xsph_scatter");
 }
 
@@ -348,7 +350,8 @@ bool GridBuffers::allocate(uint32_t cap, uint32_t nCells, GridKind k, uint32_t t
     if (reordered) {
         mask.stride = cap;
         ok = ok && dalloc(mask.words, (size_t)pbfx::kCandMaskWords * cap) && dalloc(d_laneOrder, cap) &&
-             dalloc(d_slotPairs, cap) && dalloc(d_slotPairCount, 1);
+             dalloc(d_slotPairs, cap) && dalloc(d_slotPairCount, 1) && dalloc(d_stepFlags, 1) &&
+             hipMemset(d_stepFlags, 0xFF, sizeof(uint32_t)) == hipSuccess;
         if (k == GridKind::Hashed) {
             tableLog2 = tLog2;
             const size_t slots = size_t(1) << tLog2;
@@ -374,7 +377,7 @@ void GridBuffers::release() {
     dfree(d_cellKeys); dfree(d_cellKeys_sorted); dfree(d_indices); dfree(d_indices_sorted);
     dfree(d_cellStart); dfree(d_cellEnd); dfree(d_cellUniqueKeys); dfree(d_cellOffsets); dfree(d_compactCount);
     dfree(d_segOf); dfree(d_blockCounts); dfree(d_blockOffsets); dfree(d_laneOrder);
-    dfree(d_slotPairs); dfree(d_slotPairCount);
+    dfree(d_slotPairs); dfree(d_slotPairCount); dfree(d_stepFlags);
     if (d_sortTemp) (void)hipFree(d_sortTemp);
     d_sortTemp = nullptr;
     sortTempBytes = 0;

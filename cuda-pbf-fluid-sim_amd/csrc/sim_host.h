@@ -84,6 +84,9 @@ struct GridBuffers {
     uint32_t* d_laneOrder = nullptr;  // lane -> slot of the lambda and delta passes (reordered path)
     uint2* d_slotPairs = nullptr;     // each cell's slots two at a time, for XSPH (reordered path)
     uint32_t* d_slotPairCount = nullptr;
+    // the step's flags word (reordered path; bits pbfx::kStepVelocityUnbounded and
+    // pbfx::kStepNanPosition): cleared by the predict kernel, set by the solver passes
+    uint32_t* d_stepFlags = nullptr;
     bool laneOrder = true;            // use d_laneOrder (tuning knob)
     int sortItems = 0;          // radix sort keys per thread (0: by N; tuning knob)
     int sortDigitBits = 8;      // radix sort widest digit of a pass (4..8 bits; tuning knob)

@@ -92,11 +92,23 @@ struct Scenes {
     uint32_t count = 0;
 };
 
+// Bits of the step's flags word (reordered path), which the predict kernel clears:
+//  - kStepVelocityUnbounded: a velocity the step wrote is NaN or has a component of
+//    magnitude 2^126 or more (XSPH then takes its general form);
+//  - kStepNanPosition: a solver pass before the last wrote a NaN position.  The
+//    candidate mask's skin argument does not cover a NaN position (the reference's
+//    range test admits it at any distance), so the later lambda and delta passes then
+//    walk every candidate instead of the mask.
+constexpr uint32_t kStepVelocityUnbounded = 1u;
+constexpr uint32_t kStepNanPosition = 2u;
+
 // P = X + (V + g dt) dt, AABB clamp, write Xn = (P, 1) and its cell key.
-// Thread 0 also advances the hash-table epoch when `epoch` is not null.
+// Thread 0 also advances the hash-table epoch when `epoch` is not null, and clears the
+// step's flags when `stepFlags` is not null.
 // clampBits (optional, particle order): axes moved by the clamp, for the adjoint.
 void launch_predict_hash(const float4* X, const float4* V, float4* Xn, uint32_t* keys, uint32_t* epoch,
-                         const sim::SimParams& p, const Scenes& sc, hipStream_t s, uint32_t* clampBits = nullptr);
+                         const sim::SimParams& p, const Scenes& sc, hipStream_t s, uint32_t* clampBits = nullptr,
+                         uint32_t* stepFlags = nullptr);
 
 // Insert the M occupied cells (count on device) into the table.
 void launch_hash_insert(const HashTable& t, const uint32_t* uniq, const uint32_t* off, const uint32_t* count,
@@ -112,36 +124,43 @@ void launch_spans_gather(CellLookup lookup, const float4* Xn, const uint32_t* id
 // Recording (non-null recA/recB): recA[s] = (sum of spiky gradients, b0) and recB[s] =
 // (bits of e0, slot of the neighbour that reached the cap or ~0), the reverse pass's
 // coefficients (lambda_adjoint_coefs).
+// stepFlags: with kStepNanPosition set, the full walk replaces the mask's.
 void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st, const CandMask& mk,
                           bool useMask, const sim::DeviceParams& dp, uint32_t N, hipStream_t s,
-                          float4* recA = nullptr, uint2* recB = nullptr, const uint32_t* perm = nullptr);
+                          float4* recA = nullptr, uint2* recB = nullptr, const uint32_t* perm = nullptr,
+                          const uint32_t* stepFlags = nullptr);
 
 // Pb[s] = Pa[s] + delta(s).  On the last iteration also clamps to the box and
 // writes Vs[s] = (Pb[s] - X[idxS[s]]) * invDt.  maskMode 0: full walk; 1: full
 // walk that also builds the candidate mask; 2: walk of the mask.  Recording: clampBits[s]
 // (last iteration) and recU[s] = the scaled correction before the displacement clamp.
+// stepFlags: the last iteration sets kStepVelocityUnbounded, the others kStepNanPosition
+// (see there); with kStepNanPosition set, maskMode 2 walks every candidate instead.
 void launch_delta_sorted(const float4* Pa, float4* Pb, const uint32_t* segOf, const SpanTable& st,
                          const CandMask& mk, int maskMode, const sim::DeviceParams& dp, uint32_t N, bool last,
                          const float4* X, const uint32_t* idxS, float4* Vs, float invDt, const Scenes& sc,
                          hipStream_t s, uint32_t* clampBits = nullptr, float4* recU = nullptr,
-                         const uint32_t* perm = nullptr);
+                         const uint32_t* perm = nullptr, uint32_t* stepFlags = nullptr);
 
 // perm[t] = the slot lane t of the lambda and delta passes takes: each 128-slot group
 // dealt to its two waves by the particles' place inside their cells (positions Ps).
 void launch_lane_order(const float4* Ps, uint32_t N, const sim::GridBounds& grid, const Scenes& sc, uint32_t* perm,
                        hipStream_t s);
 
-// No solver iterations: clamp Ps and compute Vs (the post-solve boundary and velocity phases).
+// No solver iterations: clamp Ps and compute Vs (the post-solve boundary and velocity phases),
+// setting kStepVelocityUnbounded as launch_delta_sorted's last iteration does.
 void launch_finalize_sorted(float4* Ps, const float4* X, const uint32_t* idxS, float4* Vs,
-                            const sim::GridBounds& grid, float invDt, uint32_t N, const Scenes& sc, hipStream_t s);
+                            const sim::GridBounds& grid, float invDt, uint32_t N, const Scenes& sc, hipStream_t s,
+                            uint32_t* stepFlags = nullptr);
 
 // XSPH (Jacobi over Vs, skipped when xsph_c <= 0), boundary with restitution,
 // scatter Xn[idxS[s]] = (P, 1) and V[idxS[s]] = (v, 0), one slot pair a lane
 // (pairs / pairCount from build_slot_pairs).  Recording (non-null recGm):
-// recGm[s] = gate * m / rho0, walked also when xsph_c is 0.
+// recGm[s] = gate * m / rho0, walked also when xsph_c is 0.  stepFlags: the step's flags
+// (kStepVelocityUnbounded clear: the weighted-select form of the walk); null: the general form.
 void launch_xsph_scatter(const float4* Ps, const float4* Vs, const uint32_t* segOf, const SpanTable& st,
                          const sim::DeviceParams& dp, uint32_t N, const uint2* pairs, const uint32_t* pairCount,
                          const uint32_t* idxS, float4* Xn, float4* V, const Scenes& sc, hipStream_t s,
-                         float* recGm = nullptr);
+                         float* recGm = nullptr, const uint32_t* stepFlags = nullptr);
 
 }  // namespace pbfx

@@ -450,6 +450,34 @@ def test_set_params_structural_refusals(oracle, dev):
     del ta, tb
 
 
+@pytest.mark.parametrize("grid,path,use_graph", [("dense", "reordered", True), ("hashed", "reordered", False),
+                                                 ("compact", "reference", True)])
+def test_non_finite_velocity_steps_like_oracle(oracle, dev, grid, path, use_graph):
+    """A particle with a NaN velocity and one with a velocity of 3e38 (clamped to the box),
+    in the 32,768-particle dam break at step 20: two steps give the oracle's bits, NaN where
+    the oracle has NaN (the NaN spreads to a few hundred particles: the reference's range
+    test, a skip when r2 > h2, admits a NaN distance).  On the reordered path the step's
+    velocities then include NaN, so XSPH takes its general form, and positions that turn
+    NaN in the first iteration make the second walk every candidate instead of the mask."""
+    scene = P.dam_break_32k()
+    sp = P.build_sim_params(scene)
+    X0 = P.seed_scene_reference(scene, oracle.seed_lattice)
+    X, V = oracle.run(X0, np.zeros_like(X0), sp, 20)
+    V[100, :3] = [np.nan, 0.0, 0.0]
+    V[5000, :3] = [3e38, -3e38, 3e38]
+    sim = _sim(scene, grid, path=path, use_graph=use_graph)
+    try:
+        sim.set_state(X, V)
+        sim.step(2)
+        Xg, Vg = sim.state()
+    finally:
+        sim.close()
+    eX, eV = oracle.run(X, V, sp, 2, compact=(grid != "dense"))
+    assert np.isnan(eX[100, 0]) and np.isnan(eV).any()
+    np.testing.assert_array_equal(Xg, eX)
+    np.testing.assert_array_equal(Vg, eV)
+
+
 @pytest.mark.parametrize("grid,path", [("dense", "reordered"), ("hashed", "reordered"), ("dense", "reference")])
 def test_cell_size_far_below_h_is_refused(dev, grid, path):
     """A cell size 5000 times below h needs (2 * 5000 + 1)^2 neighbour rows a particle: the
