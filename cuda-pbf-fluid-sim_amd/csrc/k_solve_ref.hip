@@ -57,6 +57,11 @@ struct CellRows {
 
 // Positions in original particle order, reached through the sorted index array
 // (which has no slots past the last particle); a particle's id is its original index.
+// Lanes per block of the drop-in neighbour kernels (the staging and apply kernels run
+// kBS): on the collapsed 1M dam break, blocks of 128 lanes (8 KB of pair queue each)
+// ran the neighbour walks faster than blocks of 256.
+constexpr int kWalkBS = 128;
+
 struct IndexedFetch {
     static constexpr bool kPadded = false;
     const float4* P;
@@ -67,23 +72,24 @@ struct IndexedFetch {
 
 // Every kernel below runs one lane per sorted slot and walks the slot's neighbours
 // with the shared wave walk (the staged XSPH with the queue-free inline walk); lanes
-// past N take part in the walk's wave-level steps with no rows.  The neighbour kernels deal their 256-slot tiles to the XCDs in
-// contiguous runs (xcd_tile), so a tile and its neighbours' tiles share an L2: 12%
-// faster drop-in steps on the 1M dam break.
+// past N take part in the walk's wave-level steps with no rows.  The neighbour kernels
+// run kWalkBS lanes a block and deal their tiles to the XCDs in contiguous runs
+// (xcd_tile), so a tile and its neighbours' tiles share an L2: 12% faster drop-in
+// steps on the 1M dam break.
 //
 // kStaged: the launcher has gathered the inputs into cell-sorted staging buffers (Ps:
 // position with lambda or 0 in w; Vs: velocity), so the walk reads candidates there
 // in slot order (SortedFetch, queueing slots) instead of through the sorted index
 // array.  The pairs, their order and their arithmetic are the same either way.
 template <class Lookup, bool kStaged>
-__global__ __launch_bounds__(kBS) void k_lambda_ref(float* __restrict__ lambda, const float4* __restrict__ P,
+__global__ __launch_bounds__(kWalkBS) void k_lambda_ref(float* __restrict__ lambda, const float4* __restrict__ P,
                                                     const float4* __restrict__ Ps,
                                                     const uint32_t* __restrict__ idxS,
                                                     const uint32_t* __restrict__ keyS, Lookup lookup,
                                                     sim::DeviceParams dp, uint32_t N) {
-    __shared__ uint32_t sQ[kQD * kBS];
-    __shared__ float4 sBuf[kBS];
-    const uint32_t s = xcd_tile(blockIdx.x, gridDim.x) * kBS + threadIdx.x;
+    __shared__ uint32_t sQ[kQD * kWalkBS];
+    __shared__ float4 sBuf[kWalkBS];
+    const uint32_t s = xcd_tile(blockIdx.x, gridDim.x) * kWalkBS + threadIdx.x;
     const bool valid = s < N;
     const uint32_t ss = valid ? s : N - 1;
     const uint32_t i = idxS[ss];
@@ -102,7 +108,7 @@ __global__ __launch_bounds__(kBS) void k_lambda_ref(float* __restrict__ lambda, 
                 const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
                 lambda_pair<decltype(form)::value>(a, rx, ry, rz, pair_r2(rx, ry, rz), c);
             };
-            neighbor_walk<false, kBS>(valid, s, rows, CandMask{}, fetch, sBuf + (threadIdx.x & ~63u),
+            neighbor_walk<false, kWalkBS>(valid, s, rows, CandMask{}, fetch, sBuf + (threadIdx.x & ~63u),
                                       sQ + threadIdx.x, dp.maxNeighbors, c.h2, dist2, apply);
         });
     };
@@ -116,13 +122,13 @@ __global__ __launch_bounds__(kBS) void k_lambda_ref(float* __restrict__ lambda, 
 // Held to 72 VGPRs (seven waves per SIMD); at its natural 77 it ran six, and the
 // extra wave made the drop-in Δ pass 7% faster on the 1M dam break.
 template <class Lookup, bool kStaged>
-__global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k_delta_ref(
+__global__ __launch_bounds__(kWalkBS) __attribute__((amdgpu_waves_per_eu(7))) void k_delta_ref(
     float4* __restrict__ delta, const float4* __restrict__ P, const float4* __restrict__ Ps,
     const float* __restrict__ lambda, const uint32_t* __restrict__ idxS, const uint32_t* __restrict__ keyS,
     Lookup lookup, sim::DeviceParams dp, uint32_t N) {
-    __shared__ uint32_t sQ[kQD * kBS];
-    __shared__ float4 sBuf[kBS];
-    const uint32_t s = xcd_tile(blockIdx.x, gridDim.x) * kBS + threadIdx.x;
+    __shared__ uint32_t sQ[kQD * kWalkBS];
+    __shared__ float4 sBuf[kWalkBS];
+    const uint32_t s = xcd_tile(blockIdx.x, gridDim.x) * kWalkBS + threadIdx.x;
     const bool valid = s < N;
     const uint32_t ss = valid ? s : N - 1;
     const uint32_t i = idxS[ss];
@@ -144,7 +150,7 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k
                 delta_pair<decltype(form)::value>(dx, dy, dz, rx, ry, rz, pair_r2(rx, ry, rz), lam_i,
                                                   lam_j, c);
             };
-            neighbor_walk<false, kBS>(valid, s, rows, CandMask{}, fetch, sBuf + (threadIdx.x & ~63u),
+            neighbor_walk<false, kWalkBS>(valid, s, rows, CandMask{}, fetch, sBuf + (threadIdx.x & ~63u),
                                       sQ + threadIdx.x, 0, c.h2, dist2, apply);
         });
     };
@@ -169,12 +175,12 @@ __global__ __launch_bounds__(kBS) void k_apply_delta(float4* __restrict__ P, con
 }
 
 template <class Lookup, bool kStaged>
-__global__ __launch_bounds__(kBS) void k_xsph_ref(float4* __restrict__ Vout, const float4* __restrict__ Vin,
+__global__ __launch_bounds__(kWalkBS) void k_xsph_ref(float4* __restrict__ Vout, const float4* __restrict__ Vin,
                                                   const float4* __restrict__ P, const float4* __restrict__ Ps,
                                                   const float4* __restrict__ Vs, const uint32_t* __restrict__ idxS,
                                                   const uint32_t* __restrict__ keyS, Lookup lookup,
                                                   sim::DeviceParams dp, uint32_t N) {
-    const uint32_t s = xcd_tile(blockIdx.x, gridDim.x) * kBS + threadIdx.x;
+    const uint32_t s = xcd_tile(blockIdx.x, gridDim.x) * kWalkBS + threadIdx.x;
     const bool valid = s < N;
     const uint32_t ss = valid ? s : N - 1;
     const uint32_t i = idxS[ss];
@@ -206,8 +212,8 @@ __global__ __launch_bounds__(kBS) void k_xsph_ref(float4* __restrict__ Vout, con
                 n += in ? 1 : 0;
             });
     } else {
-        __shared__ uint32_t sQ[kQD * kBS];
-        __shared__ float4 sBuf[kBS];
+        __shared__ uint32_t sQ[kQD * kWalkBS];
+        __shared__ float4 sBuf[kWalkBS];
         auto apply = [&](uint32_t j) {
             const float4 pj = P[j];
             const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
@@ -217,7 +223,7 @@ __global__ __launch_bounds__(kBS) void k_xsph_ref(float4* __restrict__ Vout, con
             xsph_pair(sx, sy, sz, vj, vi, w);
             ++n;
         };
-        neighbor_walk<false, kBS>(valid, s, rows, CandMask{}, IndexedFetch{P, idxS}, sBuf + (threadIdx.x & ~63u),
+        neighbor_walk<false, kWalkBS>(valid, s, rows, CandMask{}, IndexedFetch{P, idxS}, sBuf + (threadIdx.x & ~63u),
                                   sQ + threadIdx.x, 0, h2, dist2, apply);
     }
     if (!valid) return;
@@ -231,7 +237,7 @@ __global__ __launch_bounds__(kBS) void k_xsph_ref(float4* __restrict__ Vout, con
 // particle's sum takes its own neighbours in walk order, as in k_xsph_ref.  Blocks
 // past the pair count return at once; the XCD-aware order is taken over the others.
 template <class Lookup>
-__global__ __launch_bounds__(kBS) void k_xsph_ref_pairs(float4* __restrict__ Vout, const float4* __restrict__ Ps,
+__global__ __launch_bounds__(kWalkBS) void k_xsph_ref_pairs(float4* __restrict__ Vout, const float4* __restrict__ Ps,
                                                         const float4* __restrict__ Vs,
                                                         const uint32_t* __restrict__ idxS,
                                                         const uint32_t* __restrict__ keyS, Lookup lookup,
@@ -239,9 +245,9 @@ __global__ __launch_bounds__(kBS) void k_xsph_ref_pairs(float4* __restrict__ Vou
                                                         const uint2* __restrict__ pairs,
                                                         const uint32_t* __restrict__ pairCount) {
     const uint32_t np = *pairCount;
-    const uint32_t nb = (np + (uint32_t)kBS - 1u) / (uint32_t)kBS;
+    const uint32_t nb = (np + (uint32_t)kWalkBS - 1u) / (uint32_t)kWalkBS;
     if (blockIdx.x >= nb) return;
-    const uint32_t p = xcd_tile(blockIdx.x, nb) * kBS + threadIdx.x;
+    const uint32_t p = xcd_tile(blockIdx.x, nb) * kWalkBS + threadIdx.x;
     if (p >= np) return;
     const uint2 ab = pairs[p];
     const uint32_t a = ab.x;
@@ -369,9 +375,10 @@ void launch_lambda(float* lambda, const float4* P, const uint32_t* idxS, const u
     Staging st;
     if (staging_for(s, N, st)) {
         k_stage<<<ceil_div(N, kBS), kBS, 0, s>>>(st.P, nullptr, P, nullptr, nullptr, idxS, N);
-        k_lambda_ref<Lookup, true><<<ceil_div(N, kBS), kBS, 0, s>>>(lambda, P, st.P, idxS, keyS, lk, dp, N);
+        k_lambda_ref<Lookup, true><<<ceil_div(N, kWalkBS), kWalkBS, 0, s>>>(lambda, P, st.P, idxS, keyS, lk, dp, N);
     } else {
-        k_lambda_ref<Lookup, false><<<ceil_div(N, kBS), kBS, 0, s>>>(lambda, P, nullptr, idxS, keyS, lk, dp, N);
+        k_lambda_ref<Lookup, false><<<ceil_div(N, kWalkBS), kWalkBS, 0, s>>>(lambda, P, nullptr, idxS, keyS, lk, dp,
+                                                                             N);
     }
 }
 
@@ -381,10 +388,11 @@ void launch_delta(float4* P, float4* delta, const float* lambda, const uint32_t*
     Staging st;
     if (staging_for(s, N, st)) {
         k_stage<<<ceil_div(N, kBS), kBS, 0, s>>>(st.P, nullptr, P, lambda, nullptr, idxS, N);
-        k_delta_ref<Lookup, true><<<ceil_div(N, kBS), kBS, 0, s>>>(delta, P, st.P, lambda, idxS, keyS, lk, dp, N);
+        k_delta_ref<Lookup, true><<<ceil_div(N, kWalkBS), kWalkBS, 0, s>>>(delta, P, st.P, lambda, idxS, keyS, lk,
+                                                                           dp, N);
     } else {
-        k_delta_ref<Lookup, false><<<ceil_div(N, kBS), kBS, 0, s>>>(delta, P, nullptr, lambda, idxS, keyS, lk, dp,
-                                                                     N);
+        k_delta_ref<Lookup, false><<<ceil_div(N, kWalkBS), kWalkBS, 0, s>>>(delta, P, nullptr, lambda, idxS, keyS,
+                                                                            lk, dp, N);
     }
     k_apply_delta<<<ceil_div(N, kBS), kBS, 0, s>>>(P, delta, idxS, N);
 }
@@ -397,10 +405,10 @@ void launch_xsph(float4* Vout, const float4* Vin, const float4* P, const uint32_
         k_stage<<<ceil_div(N, kBS), kBS, 0, s>>>(st.P, st.V, P, nullptr, Vin, idxS, N);
         compact_segments(st.uniq, st.off, st.count, st.segOf, keyS, N, st.tileCounts, st.tileOffsets, s);
         build_slot_pairs(st.pairs, st.pairCount, st.segOf, st.off, N, st.tileCounts, st.tileOffsets, s);
-        k_xsph_ref_pairs<Lookup><<<ceil_div(N, kBS), kBS, 0, s>>>(Vout, st.P, st.V, idxS, keyS, lk, dp, N, st.pairs,
+        k_xsph_ref_pairs<Lookup><<<ceil_div(N, kWalkBS), kWalkBS, 0, s>>>(Vout, st.P, st.V, idxS, keyS, lk, dp, N, st.pairs,
                                                                   st.pairCount);
     } else {
-        k_xsph_ref<Lookup, false><<<ceil_div(N, kBS), kBS, 0, s>>>(Vout, Vin, P, nullptr, nullptr, idxS, keyS, lk,
+        k_xsph_ref<Lookup, false><<<ceil_div(N, kWalkBS), kWalkBS, 0, s>>>(Vout, Vin, P, nullptr, nullptr, idxS, keyS, lk,
                                                                     dp, N);
     }
 }
