@@ -52,6 +52,37 @@ def test_ensemble_scenes_match_oracle(oracle, dev, grid, use_graph):
         ens.close()
 
 
+def test_ensemble_nan_in_one_scene(oracle, dev):
+    """A NaN velocity in one scene of an ensemble: that scene evolves as the oracle's run
+    of it alone (NaN where it has NaN), and every other scene as its own run, although
+    the scenes share the step's flags (the NaN turns the candidate mask off for all of
+    them: the full walk gives the same sums)."""
+    from pbfx import _lib
+    from pbfx.ensemble import Ensemble
+    scenes = _scenes()
+    ens = Ensemble(scenes, grid="hashed", use_graph=True)
+    try:
+        ens.step(6)
+        total = sum(sc.num_particles for sc in scenes)
+        X = np.zeros((total, 4), np.float32)
+        V = np.zeros_like(X)
+        assert _lib.lib().pbfSimGetState(ens._h, X.ctypes.data, V.ctypes.data, 0) == 0
+        first, n, _ = ens.scene_info(3)
+        V[first + n // 2, :3] = [np.nan, 0.0, 0.0]
+        assert _lib.lib().pbfSimSetState(ens._h, X.ctypes.data, V.ctypes.data, total, 0) == 0
+        ens.step(2)
+        for k, sc in enumerate(scenes):
+            f, n, _ = ens.scene_info(k)
+            eX, eV = oracle.run(X[f:f + n], V[f:f + n], P.build_sim_params(sc), 2, compact=True)
+            if k == 3:
+                assert np.isnan(eX).any()
+            gX, gV = ens.scene_state(k)
+            np.testing.assert_array_equal(gX, eX, err_msg=f"scene {k}")
+            np.testing.assert_array_equal(gV, eV, err_msg=f"scene {k}")
+    finally:
+        ens.close()
+
+
 def test_ensemble_equals_solo_runs(dev):
     """The same scenes as single-scene simulators give the same bits (scene-by-scene path)."""
     from pbfx.ensemble import Ensemble, solo
