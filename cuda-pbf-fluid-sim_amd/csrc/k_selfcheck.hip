@@ -6,6 +6,7 @@
 #include "pbf_common.hpp"
 #include "pbf_hip.h"
 #include "pbf_math.hpp"
+#include "sorted_path.hpp"
 
 namespace pbfx {
 namespace {
@@ -97,6 +98,30 @@ extern "C" int pbfDebugPairUnscaled(sim::DeviceParams dp) {
         return -1;
     }
     return h;
+}
+
+extern "C" int pbfDebugScorrRatio(sim::DeviceParams dp, float* out8) {
+    if (!out8) return -1;
+    pbfx::RatioCoef* d = nullptr;
+    uint32_t* bad = nullptr;
+    if (hipMalloc(&d, sizeof(pbfx::RatioCoef)) != hipSuccess || hipMalloc(&bad, sizeof(uint32_t)) != hipSuccess) {
+        if (d) PBFX_CHECK(hipFree(d), "pbfDebugScorrRatio");
+        pbfx::set_error((int)hipErrorOutOfMemory, "pbfDebugScorrRatio");
+        return -1;
+    }
+    pbfx::launch_ratio_check(dp, d, bad, nullptr);
+    hipError_t e = hipGetLastError();
+    pbfx::RatioCoef h{};
+    if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof(h), hipMemcpyDeviceToHost);
+    PBFX_CHECK(hipFree(d), "pbfDebugScorrRatio");
+    PBFX_CHECK(hipFree(bad), "pbfDebugScorrRatio");
+    if (e != hipSuccess) {
+        pbfx::set_error((int)e, "pbfDebugScorrRatio");
+        return -1;
+    }
+    const float v[8] = {h.zh, h.zl, h.ok, h.w_q, h.poly6, h.h2, h.neg_k, h.smin};
+    for (int i = 0; i < 8; ++i) out8[i] = v[i];
+    return 0;
 }
 
 extern "C" int pbfDebugCheckUnscaledMath(uint64_t divSamples, uint64_t seed, unsigned long long* out5) {

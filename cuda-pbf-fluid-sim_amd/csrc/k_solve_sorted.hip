@@ -40,6 +40,58 @@ __device__ __forceinline__ bool velocity_bounded(float x, float y, float z) {
 __device__ __forceinline__ void note_velocity(uint32_t* stepFlags, float x, float y, float z) {
     if (stepFlags && !velocity_bounded(x, y, z)) atomicOr(stepFlags, kStepVelocityUnbounded);
 }
+// The two-instruction s_corr ratio, when the record holds the check's verdict for this
+// launch's constants (a record left from other parameters keeps the division).
+__device__ __forceinline__ void take_ratio_coef(PairConsts& c, const RatioCoef* __restrict__ rc) {
+    if (rc && rc->ok != 0.f && rc->w_q == c.w_q && rc->poly6 == c.poly6 && rc->h2 == c.h2 &&
+        rc->neg_k == c.neg_scorr_k && rc->smin == c.scorr_min) {
+        c.rq_hi = rc->zh;
+        c.rq_lo = rc->zl;
+        c.ratio2 = true;
+    }
+}
+
+// Every float W from +0 to the largest a pair can have (poly6 h2^3: r = 0), in a
+// grid-stride loop: any W whose s_corr term from the two-instruction ratio differs from
+// the division's sets *bad.
+__global__ __launch_bounds__(kBS) void k_ratio_check(sim::DeviceParams dp, uint32_t* __restrict__ bad) {
+    const PairConsts c = make_pair_consts(dp);
+    if (!pair_fixed(c)) return;
+    const float zh = 1.0f / c.w_q;                               // RN(1 / w_q)
+    const float zl = __builtin_fmaf(-c.w_q, zh, 1.0f) / c.w_q;  // 1 - w_q zh is exact
+    const float wmax = c.poly6 * c.h2 * c.h2 * c.h2;
+    if (!(wmax >= 0.f && wmax < 0x1p127f)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(bad, 1u);
+        return;
+    }
+    const uint64_t top = __float_as_uint(wmax);
+    uint32_t diff = 0;
+    for (uint64_t u = (uint64_t)blockIdx.x * kBS + threadIdx.x; u <= top; u += (uint64_t)gridDim.x * kBS) {
+        const float w = __uint_as_float((uint32_t)u);
+        const float a = scorr_fixed(scorr_ratio2(w, zh, zl), c);
+        const float b = scorr_fixed(div_unscaled(w, c.w_q, c.rcp_w_q), c);
+        diff |= __float_as_uint(a) ^ __float_as_uint(b);
+    }
+    if (diff) atomicOr(bad, 1u);
+}
+
+__global__ void k_ratio_record(sim::DeviceParams dp, const uint32_t* __restrict__ bad, RatioCoef* __restrict__ rc) {
+    const PairConsts c = make_pair_consts(dp);
+    const float zh = 1.0f / c.w_q;
+    RatioCoef r;
+    r.zh = zh;
+    r.zl = __builtin_fmaf(-c.w_q, zh, 1.0f) / c.w_q;
+    r.ok = (pair_fixed(c) && *bad == 0u) ? 1.0f : 0.0f;
+    r.w_q = c.w_q;
+    r.poly6 = c.poly6;
+    r.h2 = c.h2;
+    r.neg_k = c.neg_scorr_k;
+    r.smin = c.scorr_min;
+    *rc = r;
+}
+
+__global__ void k_clear_word(uint32_t* __restrict__ w) { *w = 0u; }
+
 // Whether the step's flags say a solver pass wrote a NaN position (the candidate mask
 // then no longer bounds the neighbours; see kStepNanPosition).
 __device__ __forceinline__ bool nan_position_seen(const uint32_t* stepFlags) {
@@ -402,7 +454,8 @@ __global__ __launch_bounds__(BS) void k_delta_sorted(const float4* __restrict__ 
                                                       const uint32_t* __restrict__ idxS, float4* __restrict__ Vs,
                                                       float invDt, Scenes sc, uint32_t* __restrict__ clampBits,
                                                       float4* __restrict__ recU, const uint32_t* __restrict__ perm,
-                                                      uint32_t* __restrict__ stepFlags) {
+                                                      uint32_t* __restrict__ stepFlags,
+                                                      const RatioCoef* __restrict__ ratioCoef) {
     __shared__ uint32_t sQ[kQD * BS];
     __shared__ float4 sBuf[BS];
     const uint32_t s = lane_slot(slot_of_thread<BS>(), N, perm);
@@ -410,7 +463,8 @@ __global__ __launch_bounds__(BS) void k_delta_sorted(const float4* __restrict__ 
     const uint32_t ss = valid ? s : N - 1;
     const float4 pi = Pa[ss];
     const uint32_t m = segOf[ss];
-    const PairConsts c = make_pair_consts(dp);
+    PairConsts c = make_pair_consts(dp);
+    take_ratio_coef(c, ratioCoef);
     const SlotBuf PA(Pa, N + kSlotPad);
     float dx = 0.f, dy = 0.f, dz = 0.f;
     auto dist2 = [&](const float4& pj) {
@@ -418,11 +472,12 @@ __global__ __launch_bounds__(BS) void k_delta_sorted(const float4* __restrict__ 
         return pair_r2(rx, ry, rz);
     };
     float4* buf = sBuf + (threadIdx.x & ~63u);
-    with_pair_form(c, [&](auto form) {
+    with_delta_form(c, [&](auto form) {
         auto apply = [&](uint32_t k) {
             const float4 pj = PA[k];
             const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            delta_pair<decltype(form)::value>(dx, dy, dz, rx, ry, rz, pair_r2(rx, ry, rz), pi.w, pj.w, c);
+            delta_pair<decltype(form)::kFixed, decltype(form)::kRatio2>(dx, dy, dz, rx, ry, rz,
+                                                                        pair_r2(rx, ry, rz), pi.w, pj.w, c);
         };
         if (maskMode == 2 && !nan_position_seen(stepFlags))
             mask_walk<BS>(valid, s, m, st, mk, PA, sQ + threadIdx.x, 0, c.h2, dist2, apply);
@@ -659,16 +714,25 @@ void launch_delta_sorted(const float4* Pa, float4* Pb, const uint32_t* segOf, co
                          const CandMask& mk, int maskMode, const sim::DeviceParams& dp, uint32_t N, bool last,
                          const float4* X, const uint32_t* idxS, float4* Vs, float invDt, const Scenes& sc,
                          hipStream_t s, uint32_t* clampBits, float4* recU, const uint32_t* perm,
-                         uint32_t* stepFlags) {
+                         uint32_t* stepFlags, const RatioCoef* ratioCoef) {
     if (N == 0) return;
     const int lastPass = last ? 1 : 0;
     if (st.block == 128)
         k_delta_sorted<128><<<ceil_div(N, 128u), 128, 0, s>>>(Pa, Pb, segOf, st, mk, maskMode, dp, N, lastPass, X,
-                                                              idxS, Vs, invDt, sc, clampBits, recU, perm, stepFlags);
+                                                              idxS, Vs, invDt, sc, clampBits, recU, perm, stepFlags,
+                                                              ratioCoef);
     else
         k_delta_sorted<256><<<ceil_div(N, 256u), 256, 0, s>>>(Pa, Pb, segOf, st, mk, maskMode, dp, N, lastPass, X,
-                                                              idxS, Vs, invDt, sc, clampBits, recU, perm, stepFlags);
+                                                              idxS, Vs, invDt, sc, clampBits, recU, perm, stepFlags,
+                                                              ratioCoef);
     check_launch("delta_sorted");
+}
+
+void launch_ratio_check(const sim::DeviceParams& dp, RatioCoef* coef, uint32_t* bad, hipStream_t s) {
+    k_clear_word<<<1, 1, 0, s>>>(bad);
+    k_ratio_check<<<8192, kBS, 0, s>>>(dp, bad);
+    k_ratio_record<<<1, 1, 0, s>>>(dp, bad, coef);
+    check_launch("ratio_check");
 }
 
 void launch_lane_order(const float4* Ps, uint32_t N, const sim::GridBounds& grid, const Scenes& sc, uint32_t* perm,

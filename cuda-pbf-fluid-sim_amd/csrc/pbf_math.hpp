@@ -123,7 +123,26 @@ struct PairConsts {
     int scorr_enable, scorr_ni;
     float rcp_w_q;  // rcp_refined(w_q)
     bool unscaled;  // pair_unscaled_ok: the pair arithmetic uses the unscaled sequences
+    // the two-instruction s_corr ratio, when checked for these constants (scorr_ratio2)
+    float rq_hi = 0.f, rq_lo = 0.f;
+    bool ratio2 = false;
 };
+
+// s_corr's ratio W / w_q with the divisor fixed for a launch, from the reciprocal's two
+// parts zh = RN(1 / w_q) and zl = RN(1 / w_q - zh): two instructions against
+// div_unscaled's five.  It is not the correctly rounded quotient for every divisor and
+// dividend (with the benchmark's w_q it differs below W = 3e-26, where W^4 underflows),
+// so it is used only where the simulator has checked, on the device, that the s_corr
+// term it gives equals the division's for every float W from +0 to the largest W a pair
+// can have (launch_ratio_check).
+__device__ __forceinline__ float scorr_ratio2(float w, float zh, float zl) {
+    return __builtin_fmaf(w, zh, w * zl);
+}
+// The fixed form's s_corr term from its ratio: -k (ratio^2)^2, floored at scorr_min.
+__device__ __forceinline__ float scorr_fixed(float ratio, const PairConsts& c) {
+    const float x2 = ratio * ratio;  // pow_scorr's n = 4 product
+    return fmaxf(c.neg_scorr_k * (x2 * x2), c.scorr_min);
+}
 
 // Quotients a / b with a in {0} or [amin, amax] and b in [bmin, bmax] (all positive)
 // meet div_unscaled's conditions, with margins of 2^6 and more on every bound.
@@ -170,6 +189,24 @@ __device__ __forceinline__ void with_pair_form(const PairConsts& c, F&& f) {
         f(std::true_type{});
     else
         f(std::false_type{});
+}
+
+// The delta kernels' forms: the common setting with the checked two-instruction s_corr
+// ratio, the common setting with the division, and the general form.
+template <bool kF, bool kR>
+struct DeltaForm {
+    static constexpr bool kFixed = kF, kRatio2 = kR;
+};
+template <class F>
+__device__ __forceinline__ void with_delta_form(const PairConsts& c, F&& f) {
+    if (pair_fixed(c)) {
+        if (c.ratio2)
+            f(DeltaForm<true, true>{});
+        else
+            f(DeltaForm<true, false>{});
+    } else {
+        f(DeltaForm<false, false>{});
+    }
 }
 
 __device__ __forceinline__ PairConsts make_pair_consts(const sim::DeviceParams& dp) {
@@ -268,8 +305,8 @@ __device__ __forceinline__ void lambda_adjoint_coefs(const LambdaAcc& a, const s
 }
 
 // One in-range neighbour of the position-correction sum.  kFixed: the caller has
-// established pair_fixed(c).
-template <bool kFixed = false>
+// established pair_fixed(c); kRatio2 (with kFixed): also c.ratio2.
+template <bool kFixed = false, bool kRatio2 = false>
 __device__ __forceinline__ void delta_pair(float& dx, float& dy, float& dz, float rx, float ry, float rz, float r2,
                                            float lam_i, float lam_j, const PairConsts& c) {
     float coeff;
@@ -289,9 +326,7 @@ __device__ __forceinline__ void delta_pair(float& dx, float& dy, float& dz, floa
     if (kFixed) {
         const float hr2 = c.h2 - r2;
         const float w = c.poly6 * hr2 * hr2 * hr2;
-        const float ratio = div_unscaled(w, c.w_q, c.rcp_w_q);
-        const float x2 = ratio * ratio;  // pow_scorr's n = 4 product
-        scorr = fmaxf(c.neg_scorr_k * (x2 * x2), c.scorr_min);
+        scorr = scorr_fixed(kRatio2 ? scorr_ratio2(w, c.rq_hi, c.rq_lo) : div_unscaled(w, c.w_q, c.rcp_w_q), c);
     } else if (c.scorr_enable) {
         const float hr2 = c.h2 - r2;
         const float w = c.poly6 * hr2 * hr2 * hr2;

@@ -221,7 +221,7 @@ void SortedSolve(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream
         pbfx::launch_delta_sorted(Pa, Pb, g.d_segOf, g.spans, mk, maskMode, dp, N, last, b.d_pos, g.d_indices_sorted,
                                   rec ? b.rec.Vs : b.d_vel_sorted, invDt, g.scenes, s,
                                   last ? b.rec.finalClamp : nullptr, rec ? b.rec.dispU[it] : nullptr, perm,
-                                  g.d_stepFlags);
+                                  g.d_stepFlags, g.d_ratioCoef);
         mark(last ? "delta_last" : "delta");
         cur ^= 1;
     }
@@ -351,7 +351,8 @@ bool GridBuffers::allocate(uint32_t cap, uint32_t nCells, GridKind k, uint32_t t
         mask.stride = cap;
         ok = ok && dalloc(mask.words, (size_t)pbfx::kCandMaskWords * cap) && dalloc(d_laneOrder, cap) &&
              dalloc(d_slotPairs, cap) && dalloc(d_slotPairCount, 1) && dalloc(d_stepFlags, 1) &&
-             hipMemset(d_stepFlags, 0xFF, sizeof(uint32_t)) == hipSuccess;
+             hipMemset(d_stepFlags, 0xFF, sizeof(uint32_t)) == hipSuccess && dalloc(d_ratioCoef, 1) &&
+             dalloc(d_ratioBad, 1) && hipMemset(d_ratioCoef, 0, sizeof(pbfx::RatioCoef)) == hipSuccess;
         if (k == GridKind::Hashed) {
             tableLog2 = tLog2;
             const size_t slots = size_t(1) << tLog2;
@@ -377,7 +378,7 @@ void GridBuffers::release() {
     dfree(d_cellKeys); dfree(d_cellKeys_sorted); dfree(d_indices); dfree(d_indices_sorted);
     dfree(d_cellStart); dfree(d_cellEnd); dfree(d_cellUniqueKeys); dfree(d_cellOffsets); dfree(d_compactCount);
     dfree(d_segOf); dfree(d_blockCounts); dfree(d_blockOffsets); dfree(d_laneOrder);
-    dfree(d_slotPairs); dfree(d_slotPairCount); dfree(d_stepFlags);
+    dfree(d_slotPairs); dfree(d_slotPairCount); dfree(d_stepFlags); dfree(d_ratioCoef); dfree(d_ratioBad);
     if (d_sortTemp) (void)hipFree(d_sortTemp);
     d_sortTemp = nullptr;
     sortTempBytes = 0;
@@ -540,7 +541,13 @@ bool Simulator::initCommon(const SimParams& p, const SimOptions& opt, uint64_t c
     m_frameIndex = 0;
     m_graphDirty = true;
     m_initialized = true;
+    checkScorrRatio();
     return true;
+}
+
+void Simulator::checkScorrRatio() {
+    if (m_grid.d_ratioCoef)
+        pbfx::launch_ratio_check(MakeDeviceParams(m_params), m_grid.d_ratioCoef, m_grid.d_ratioBad, m_stream);
 }
 
 void Simulator::releaseScenes() {
@@ -685,6 +692,7 @@ bool Simulator::restructure(const SimParams& p) {
     }
     m_params = p;
     m_graphDirty = true;
+    checkScorrRatio();
     return true;
 }
 
@@ -707,6 +715,7 @@ bool Simulator::setParams(const SimParams& p) {
     }
     m_params = p;
     m_graphDirty = true;
+    checkScorrRatio();
     return true;
 }
 

@@ -87,6 +87,9 @@ struct GridBuffers {
     // the step's flags word (reordered path; bits pbfx::kStepVelocityUnbounded and
     // pbfx::kStepNanPosition): cleared by the predict kernel, set by the solver passes
     uint32_t* d_stepFlags = nullptr;
+    // the s_corr ratio check's record and scratch word (reordered path; launch_ratio_check)
+    pbfx::RatioCoef* d_ratioCoef = nullptr;
+    uint32_t* d_ratioBad = nullptr;
     bool laneOrder = true;            // use d_laneOrder (tuning knob)
     int sortItems = 0;          // radix sort keys per thread (0: by N; tuning knob)
     int sortDigitBits = 8;      // radix sort widest digit of a pass (4..8 bits; tuning knob)
@@ -263,6 +266,8 @@ public:
 
 private:
     bool restructure(const SimParams& p);
+    // re-checks the two-instruction s_corr ratio for m_params (reordered path)
+    void checkScorrRatio();
     bool captureGraphs();
     void destroyGraphs();
     void runPipeline(hipStream_t s);

@@ -59,6 +59,14 @@ constexpr uint32_t kCandMaskWords = 64;
 
 // Per-step candidate mask (built and walked in walk.hpp): words[w * stride + s] holds
 // bits 32w..32w+31 of slot s's walk; rs2 is the squared skin radius.
+// The s_corr ratio check's record (launch_ratio_check): zh = RN(1 / w_q) and
+// zl = RN(1 / w_q - zh), ok = 1 when fma(W, zh, W * zl) gives the division's s_corr term
+// for every float W from +0 to the largest a pair can have, and the parameters the check
+// was made for (the delta kernels take the two-instruction ratio only when theirs match).
+struct RatioCoef {
+    float zh, zl, ok, w_q, poly6, h2, neg_k, smin;
+};
+
 struct CandMask {
     uint32_t* words = nullptr;
     uint32_t stride = 0;
@@ -130,6 +138,11 @@ void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st
                           float4* recA = nullptr, uint2* recB = nullptr, const uint32_t* perm = nullptr,
                           const uint32_t* stepFlags = nullptr);
 
+// Check the two-instruction s_corr ratio for the parameters dp on every W it can meet and
+// write the record `coef` (bad: one word of device scratch).  Runs on stream s; the
+// simulator calls it whenever its parameters change.
+void launch_ratio_check(const sim::DeviceParams& dp, RatioCoef* coef, uint32_t* bad, hipStream_t s);
+
 // Pb[s] = Pa[s] + delta(s).  On the last iteration also clamps to the box and
 // writes Vs[s] = (Pb[s] - X[idxS[s]]) * invDt.  maskMode 0: full walk; 1: full
 // walk that also builds the candidate mask; 2: walk of the mask.  Recording: clampBits[s]
@@ -140,7 +153,8 @@ void launch_delta_sorted(const float4* Pa, float4* Pb, const uint32_t* segOf, co
                          const CandMask& mk, int maskMode, const sim::DeviceParams& dp, uint32_t N, bool last,
                          const float4* X, const uint32_t* idxS, float4* Vs, float invDt, const Scenes& sc,
                          hipStream_t s, uint32_t* clampBits = nullptr, float4* recU = nullptr,
-                         const uint32_t* perm = nullptr, uint32_t* stepFlags = nullptr);
+                         const uint32_t* perm = nullptr, uint32_t* stepFlags = nullptr,
+                         const RatioCoef* ratioCoef = nullptr);
 
 // perm[t] = the slot lane t of the lambda and delta passes takes: each 128-slot group
 // dealt to its two waves by the particles' place inside their cells (positions Ps).

@@ -84,6 +84,7 @@ SIGNATURES = {
     "pbfDebugEchoFloat3": (None, [Float3, ctypes.POINTER(ctypes.c_float)]),
     "pbfDebugPairUnscaled": (_I, [DeviceParams]),
     "pbfDebugCheckUnscaledMath": (_I, [ctypes.c_uint64, ctypes.c_uint64, _P]),
+    "pbfDebugScorrRatio": (_I, [DeviceParams, _P]),
 }
 
 _lib = None

@@ -98,3 +98,37 @@ def test_general_pair_form_matches_oracle(oracle, dev, variant, path):
     eX, eV = oracle.run(X0, np.zeros_like(X0), sp, 4)
     np.testing.assert_array_equal(X, eX)
     np.testing.assert_array_equal(V, eV)
+
+
+def _ratio_record(sp):
+    out = (ctypes.c_float * 8)()
+    assert _lib.lib().pbfDebugScorrRatio(P.make_device_params(sp), ctypes.addressof(out)) == 0
+    return np.array(list(out), np.float32)
+
+
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_scenes_take_the_two_instruction_ratio(dev, name):
+    """Every scene's s_corr ratio passes the exhaustive check (fma(W, zh, W * zl) gives the
+    division's s_corr term for every float W from +0 to poly6 h^6), and the record holds
+    zh = RN(1 / w_q) and zl = RN(1 / w_q - zh) for the scene's w_q = poly6 (h^2 - dq^2)^3."""
+    sp = P.build_sim_params(SCENES[name]())
+    zh, zl, ok, wq, poly6, h2, negk, smin = _ratio_record(sp)
+    f = np.float32
+    hr2q = f(f(sp.kernel.h2) - f(f(f(sp.pbf.scorr_dq_h) * f(sp.kernel.h)) ** 2))
+    assert wq == f(f(f(f(sp.kernel.poly6) * hr2q) * hr2q) * hr2q)
+    assert ok == 1.0
+    # zh is the float nearest 1 / w_q: the exact error |1 - w_q zh| / w_q is at most half
+    # an ulp of zh (products of two floats are exact in float64)
+    e = abs(1.0 - float(wq) * float(zh)) / float(wq)
+    assert e <= 0.5 * float(np.spacing(zh))
+    assert zl == f((1.0 - float(wq) * float(zh)) / float(wq))
+    assert (poly6, h2, negk, smin) == (f(sp.kernel.poly6), f(sp.kernel.h2), f(-sp.pbf.scorr_k), f(sp.pbf.scorr_min))
+
+
+@pytest.mark.parametrize("field,value", [("scorr_enable", 0), ("scorr_n", 3.0), ("scorr_min", 0.0)])
+def test_ratio_check_declines_outside_the_common_form(dev, field, value):
+    """Outside the delta kernels' common form (s_corr off, another exponent, no floor below
+    zero) the check records no two-instruction ratio."""
+    sp = P.build_sim_params(P.small_block())
+    setattr(sp.pbf, field, value)
+    assert _ratio_record(sp)[2] == 0.0
