@@ -531,12 +531,13 @@ __global__ __launch_bounds__(kBS) void k_finalize_sorted(float4* __restrict__ Ps
 
 // XSPH's pair term is a few products, so its walk evaluates every candidate in place
 // rather than queueing the neighbours for a later pass: the sums take the same
-// neighbours in the same order, with no queue to fill and drain.  The vector memory
-// pipeline, not the VALU, bounds that walk, so each lane takes two particles of one
-// cell (a slot pair: the cell's slots two at a time), which share the cell's neighbour
-// rows: a round's four candidate positions are read once for both, and a candidate's
-// velocity once when it is a neighbour of either (a non-neighbour's read is sent past
-// the end of the buffer, where the hardware returns zeros without touching memory).
+// neighbours in the same order, with no queue to fill and drain.  With one particle a
+// lane the vector memory pipeline, not the VALU, bounded that walk, so each lane takes
+// two particles of one cell (a slot pair: the cell's slots two at a time), which share
+// the cell's neighbour rows: a round's four candidate positions are read once for both,
+// and a candidate's velocity once when it is a neighbour of either (a non-neighbour's
+// read is sent past the end of the buffer, where the hardware returns zeros without
+// touching memory).
 // Both arrays carry kSlotPad padding slots past the last particle.  The grid covers N
 // pairs, the most there can be; blocks past the step's pair count return at once, and
 // the XCD-aware order is taken over the blocks that have pairs.
