@@ -445,7 +445,8 @@ __global__ __launch_bounds__(BS) void k_lambda_sorted(float4* Ps, const uint32_t
 }
 
 // Delta of a sorted slot.  maskMode: 0 full walk, 1 full walk that builds the
-// candidate mask, 2 mask walk.
+// candidate mask (the later lambda passes walk the mask; every delta pass walks every
+// candidate, which measured faster than the mask's walk for them).
 template <int BS>
 __global__ __launch_bounds__(BS) void k_delta_sorted(const float4* __restrict__ Pa, float4* __restrict__ Pb,
                                                       const uint32_t* __restrict__ segOf, SpanTable st,
@@ -479,12 +480,10 @@ __global__ __launch_bounds__(BS) void k_delta_sorted(const float4* __restrict__ 
             delta_pair<decltype(form)::kFixed, decltype(form)::kRatio2>(dx, dy, dz, rx, ry, rz,
                                                                         pair_r2(rx, ry, rz), pi.w, pj.w, c);
         };
-        if (maskMode == 2 && !nan_position_seen(stepFlags))
-            mask_walk<BS>(valid, s, m, st, mk, PA, sQ + threadIdx.x, 0, c.h2, dist2, apply);
-        else if (maskMode == 1)
+        if (maskMode == 1)
             neighbor_walk<true, BS>(valid, s, SpanRows(st, m), mk, SortedFetch{PA}, buf, sQ + threadIdx.x, 0, c.h2,
                                     dist2, apply);
-        else  // maskMode 0, or 2 after a NaN position
+        else
             neighbor_walk<false, BS>(valid, s, SpanRows(st, m), mk, SortedFetch{PA}, buf, sQ + threadIdx.x, 0, c.h2,
                                      dist2, apply);
     });

@@ -105,8 +105,8 @@ struct Scenes {
 //    magnitude 2^126 or more (XSPH then takes its general form);
 //  - kStepNanPosition: a solver pass before the last wrote a NaN position.  The
 //    candidate mask's skin argument does not cover a NaN position (the reference's
-//    range test admits it at any distance), so the later lambda and delta passes then
-//    walk every candidate instead of the mask.
+//    range test admits it at any distance), so the later lambda passes then walk
+//    every candidate instead of the mask.
 constexpr uint32_t kStepVelocityUnbounded = 1u;
 constexpr uint32_t kStepNanPosition = 2u;
 
@@ -145,10 +145,9 @@ void launch_ratio_check(const sim::DeviceParams& dp, RatioCoef* coef, uint32_t* 
 
 // Pb[s] = Pa[s] + delta(s).  On the last iteration also clamps to the box and
 // writes Vs[s] = (Pb[s] - X[idxS[s]]) * invDt.  maskMode 0: full walk; 1: full
-// walk that also builds the candidate mask; 2: walk of the mask.  Recording: clampBits[s]
-// (last iteration) and recU[s] = the scaled correction before the displacement clamp.
-// stepFlags: the last iteration sets kStepVelocityUnbounded, the others kStepNanPosition
-// (see there); with kStepNanPosition set, maskMode 2 walks every candidate instead.
+// walk that also builds the candidate mask.  Recording: clampBits[s] (last iteration)
+// and recU[s] = the scaled correction before the displacement clamp.  stepFlags: the
+// last iteration sets kStepVelocityUnbounded, the others kStepNanPosition (see there).
 void launch_delta_sorted(const float4* Pa, float4* Pb, const uint32_t* segOf, const SpanTable& st,
                          const CandMask& mk, int maskMode, const sim::DeviceParams& dp, uint32_t N, bool last,
                          const float4* X, const uint32_t* idxS, float4* Vs, float invDt, const Scenes& sc,

@@ -91,11 +91,12 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 // Candidate mask: bit c of particle s is set when candidate c of its walk (the
 // concatenated neighbour-row spans, in walk order) was within the skin radius
 // h + 2 * (I - 1) * maxDisp when the first delta pass scanned the predicted
-// positions.  The passes that read the mask (the lambda and delta passes of
-// iterations 1 .. I-1) see positions at most I - 1 corrections later, each clamped
-// to maxDisp, so a pair outside the skin cannot have come within h.  Those passes
-// visit only the set bits, in walk order, and still test the range.  (XSPH, after
-// the last correction and the box clamp, walks its candidates directly.)
+// positions.  The passes that read the mask (the lambda passes of iterations
+// 1 .. I-1) see positions at most I - 1 corrections later, each clamped to maxDisp, so
+// a pair outside the skin cannot have come within h.  Those passes visit only the set
+// bits, in walk order, and still test the range.  (The later delta passes, which take
+// every neighbour, and XSPH, after the last correction and the box clamp, walk their
+// candidates directly.)
 // The mask covers the first 32 * kMaskWords candidates; candidates past that are
 // tested directly.
 constexpr uint32_t kMaskWords = kCandMaskWords;
