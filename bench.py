@@ -51,11 +51,13 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 VALU_PEAK = 157.3e12 / (64 * 2 * 2)
 
 # Algorithmic bytes per particle of each launch label (FP32 xyz = 12 B, each datum
-# counted once per kernel): predict+clamp 36 + hash 20; sort 16 per pass + 4;
-# lambda 16; delta+apply 28 (+36 for the fused velocity on the last iteration);
-# XSPH + clamp 36.
+# counted once per kernel): predict+clamp 36 + hash 20; sort 16 per pass + 4; the
+# candidate mask's build 16 (its mask words depend on the state, so the measured
+# traffic counts them); lambda 16; delta+apply 28 (+36 for the fused velocity on the
+# last iteration); XSPH + clamp 36.
 BYTES_PER_PARTICLE = {
     "predict_hash": 56.0,
+    "mask_build": 16.0,
     "lambda": 16.0,
     "delta": 28.0,
     "delta_last": 64.0,
@@ -66,6 +68,7 @@ BYTES_PER_PARTICLE = {
 # launches of one kernel).
 KERNELS = {
     "k_predict_hash": ["predict_hash"],
+    "k_mask_build": ["mask_build"],
     "k_lambda_sorted": ["lambda"],
     "k_delta_sorted": ["delta", "delta_last"],
     "k_xsph_scatter": ["xsph_scatter"],

@@ -398,7 +398,74 @@ __device__ __forceinline__ uint32_t lane_slot(uint32_t t, uint32_t N, const uint
     return (perm && t < N) ? perm[t] : t;
 }
 
-// Lambda of a sorted slot: the full walk (first iteration) or the mask walk.
+// The candidate mask of every sorted slot, from the positions Ps.  Each lane takes a slot
+// pair (two slots of one cell, which share the cell's rows, from build_slot_pairs): it
+// walks the rows kMaskRound candidates a round, each load serving both slots (reads past
+// the last particle's padding return zeros, and past the span are masked off), and
+// records each slot's skin bit of every candidate in walk order, its own slot included
+// (the walks that read the mask skip it), 32 to a word.  No pair arithmetic and no
+// queue: it is the candidate test alone, which the first delta pass would otherwise do
+// on top of its walk.
+// Eight loads a round kept more of them in flight than four (a 9% faster build), with no
+// pair arithmetic to hide their latency behind.
+constexpr uint32_t kMaskRound = 8;
+template <int BS>
+__global__ __launch_bounds__(BS) void k_mask_build(const float4* __restrict__ Ps, const uint32_t* __restrict__ segOf,
+                                                   SpanTable st, CandMask mk, uint32_t N,
+                                                   const uint2* __restrict__ pairs,
+                                                   const uint32_t* __restrict__ pairCount) {
+    const uint32_t np = *pairCount;
+    const uint32_t nb = (np + (uint32_t)BS - 1u) / (uint32_t)BS;
+    if (blockIdx.x >= nb) return;
+    const uint32_t p = xcd_tile(blockIdx.x, nb) * BS + threadIdx.x;
+    if (p >= np) return;
+    const uint2 ab = pairs[p];
+    const uint32_t a = ab.x;
+    const bool hasB = ab.y != 0xFFFFFFFFu;
+    const uint32_t b = hasB ? ab.y : a;
+    const float4 pa = Ps[a], pb = Ps[b];
+    const uint32_t m = segOf[a];
+    const SlotBuf PS(Ps, N + kSlotPad);
+    uint32_t cidx = 0, wa = 0, wb = 0;
+    auto put = [&](uint32_t slot, uint32_t wi, uint32_t v) {
+        if (wi < kMaskWords) mk.words[(uint64_t)wi * mk.stride + slot] = v;
+    };
+    for (int r = 0; r < st.rows; ++r) {
+        const uint2 sp = st.spans[(uint64_t)r * st.stride + m];
+        for (uint32_t k = sp.x; k < sp.y; k += kMaskRound) {
+            const uint32_t cnt = min(kMaskRound, sp.y - k);
+            float4 pj[kMaskRound];
+#pragma unroll
+            for (uint32_t u = 0; u < kMaskRound; ++u) pj[u] = PS.at(k, u);
+            uint32_t sa = 0u, sb = 0u;
+#pragma unroll
+            for (uint32_t u = 0; u < kMaskRound; ++u) {
+                sa |= in_range(pair_r2(pa.x - pj[u].x, pa.y - pj[u].y, pa.z - pj[u].z), mk.rs2) ? 1u << u : 0u;
+                sb |= in_range(pair_r2(pb.x - pj[u].x, pb.y - pj[u].y, pb.z - pj[u].z), mk.rs2) ? 1u << u : 0u;
+            }
+            const uint32_t inSpan = (1u << cnt) - 1u;
+            const uint32_t sh = cidx & 31u;
+            uint64_t xa = (uint64_t)wa | ((uint64_t)(sa & inSpan) << sh);
+            uint64_t xb = (uint64_t)wb | ((uint64_t)(sb & inSpan) << sh);
+            if (sh + cnt >= 32u) {
+                const uint32_t wi = cidx >> 5;
+                put(a, wi, (uint32_t)xa);
+                if (hasB) put(b, wi, (uint32_t)xb);
+                xa >>= 32;
+                xb >>= 32;
+            }
+            wa = (uint32_t)xa;
+            wb = (uint32_t)xb;
+            cidx += cnt;
+        }
+    }
+    if ((cidx & 31u) != 0u) {
+        put(a, cidx >> 5, wa);
+        if (hasB) put(b, cidx >> 5, wb);
+    }
+}
+
+// Lambda of a sorted slot: the full walk or the mask walk.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_lambda_sorted(float4* Ps, const uint32_t* __restrict__ segOf, SpanTable st,
                                                        CandMask mk, int useMask, sim::DeviceParams dp, uint32_t N,
@@ -647,8 +714,9 @@ inline uint32_t blocks_for(uint32_t n) { return ceil_div(n > 0 ? n : 1, kBS); }
 float mask_skin_radius2(const sim::SimParams& p) {
     if (!p.pbf.enable_disp_clamp || p.solverIters < 1 || !(p.pbf.disp_clamp_max_h >= 0.f)) return 0.f;
     const double h = p.kernel.h;
-    // the mask serves iterations 1 .. I-1, whose positions are at most I-1 clamped
-    // corrections away from the ones it was built on (XSPH walks without it)
+    // the mask serves the lambda passes of iterations 0 .. I-1, whose positions are at
+    // most I-1 clamped corrections away from the ones it was built on (the delta passes
+    // and XSPH walk without it)
     const double rs = h * (1.0 + 2.0 * (p.solverIters - 1) * (double)p.pbf.disp_clamp_max_h);
     if (rs > 2.0 * h) return 0.f;
     const double r = rs * (1.0 + 1e-3);  // margin for float rounding of positions and r^2
@@ -690,6 +758,13 @@ void launch_spans_gather(CellLookup lookup, const float4* Xn, const uint32_t* id
             break;
     }
     check_launch("spans_gather");
+}
+
+void launch_mask_build(const float4* Ps, const uint32_t* segOf, const SpanTable& st, const CandMask& mk, uint32_t N,
+                       const uint2* pairs, const uint32_t* pairCount, hipStream_t s) {
+    if (N == 0) return;
+    k_mask_build<128><<<ceil_div(N, 128u), 128, 0, s>>>(Ps, segOf, st, mk, N, pairs, pairCount);
+    check_launch("mask_build");
 }
 
 void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st, const CandMask& mk,

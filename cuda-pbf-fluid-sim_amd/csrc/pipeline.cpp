@@ -203,27 +203,30 @@ void SortedSolve(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream
         mark("finalize");
         return;
     }
-    // The first delta pass records the candidate mask the later lambda passes walk (with
-    // one iteration there is none, and no mask).  The later delta passes, which take
-    // every neighbour, walk every candidate: at the collapsed state that measured 3%
-    // faster than walking the mask, whose gain is in the lambda passes (they stop at the
-    // neighbour cap).
+    // The candidate mask, built from the predicted positions before the first iteration,
+    // serves every lambda pass (they stop at the neighbour cap, so skipping the candidates
+    // outside the skin pays).  The delta passes, which take every neighbour, walk every
+    // candidate: at the collapsed state that measured faster than walking the mask.
     pbfx::CandMask mk = g.mask;
     mk.rs2 = pbfx::mask_skin_radius2(p);
-    const bool useMask = g.candidateMask && g.mask.words != nullptr && mk.rs2 > 0.f && p.solverIters > 1;
+    const bool useMask = g.candidateMask && g.mask.words != nullptr && mk.rs2 > 0.f;
     const bool rec = b.rec.P[0] != nullptr;
     const uint32_t* perm = g.laneOrder ? g.d_laneOrder : nullptr;
+    if (useMask) {
+        pbfx::launch_mask_build(rec ? b.rec.P[0] : b.d_pos_sorted[0], g.d_segOf, g.spans, mk, N, g.d_slotPairs,
+                                g.d_slotPairCount, s);
+        mark("mask_build");
+    }
     int cur = 0;
     for (int it = 0; it < p.solverIters; ++it) {
         float4* Pa = rec ? b.rec.P[it] : b.d_pos_sorted[cur];
         float4* Pb = rec ? b.rec.P[it + 1] : b.d_pos_sorted[cur ^ 1];
         const bool last = it == p.solverIters - 1;
-        pbfx::launch_lambda_sorted(Pa, g.d_segOf, g.spans, mk, useMask && it > 0, dp, N, s,
+        pbfx::launch_lambda_sorted(Pa, g.d_segOf, g.spans, mk, useMask, dp, N, s,
                                    rec ? b.rec.lamA[it] : nullptr, rec ? b.rec.lamB[it] : nullptr, perm,
                                    g.d_stepFlags);
         mark("lambda");
-        const int maskMode = (useMask && it == 0) ? 1 : 0;
-        pbfx::launch_delta_sorted(Pa, Pb, g.d_segOf, g.spans, mk, maskMode, dp, N, last, b.d_pos, g.d_indices_sorted,
+        pbfx::launch_delta_sorted(Pa, Pb, g.d_segOf, g.spans, mk, 0, dp, N, last, b.d_pos, g.d_indices_sorted,
                                   rec ? b.rec.Vs : b.d_vel_sorted, invDt, g.scenes, s,
                                   last ? b.rec.finalClamp : nullptr, rec ? b.rec.dispU[it] : nullptr, perm,
                                   g.d_stepFlags, g.d_ratioCoef);

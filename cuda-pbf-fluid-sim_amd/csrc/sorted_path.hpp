@@ -132,6 +132,12 @@ void launch_spans_gather(CellLookup lookup, const float4* Xn, const uint32_t* id
 // Recording (non-null recA/recB): recA[s] = (sum of spiky gradients, b0) and recB[s] =
 // (bits of e0, slot of the neighbour that reached the cap or ~0), the reverse pass's
 // coefficients (lambda_adjoint_coefs).
+// Build the candidate mask from the positions Ps (one lane per slot pair of
+// build_slot_pairs, every candidate of the cell's rows): the skin bit of each candidate
+// in walk order, mk.words[w * stride + s].
+void launch_mask_build(const float4* Ps, const uint32_t* segOf, const SpanTable& st, const CandMask& mk, uint32_t N,
+                       const uint2* pairs, const uint32_t* pairCount, hipStream_t s);
+
 // stepFlags: with kStepNanPosition set, the full walk replaces the mask's.
 void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st, const CandMask& mk,
                           bool useMask, const sim::DeviceParams& dp, uint32_t N, hipStream_t s,
