@@ -205,7 +205,7 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k
                 accP = add3(accP, mul3(dW, dot3(sub3(mul3(gk, kk), mul3(gj, kj)), dvj)));
             }
         };
-        neighbor_walk<false, kBS>(valid, s, SpanRows(st, segOf[ss]), CandMask{}, SortedFetch{SlotBuf(Pf, N + kSlotPad)},
+        neighbor_walk<kBS>(valid, s, SpanRows(st, segOf[ss]), SortedFetch{SlotBuf(Pf, N + kSlotPad)},
                                   sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x, 0, h2, dist2, apply);
         if (valid) {
             gVn[s] = mk4(add3(mul3(gk, 1.0f - kk * sumW), accV), 0.f);
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k
             const float ckj = pk4.w + pj4.w + scorr;
             acc = add3(acc, add3(mul3(hess_apply(sp, r, da), ckj), mul3(g, dsr * dot3(r, da))));
         };
-        neighbor_walk<false, kBS>(valid, s, SpanRows(st, segOf[ss]), CandMask{}, SortedFetch{SlotBuf(P, N + kSlotPad)},
+        neighbor_walk<kBS>(valid, s, SpanRows(st, segOf[ss]), SortedFetch{SlotBuf(P, N + kSlotPad)},
                                   sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x, 0, c.h2, dist2, apply);
     });
     if (!valid) return;
@@ -387,7 +387,7 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(7))) void k
             if (inK) acc = add3(acc, add3(mul3(dW6, bk), mul3(hess_apply(sp, r, add3(g, Gk)), 2.0f * epk)));
             if (inJ) acc = add3(acc, add3(mul3(dW6, qj.w), mul3(hess_apply(sp, r, sub3(g, f3(qj))), 2.0f * epj)));
         };
-        neighbor_walk<false, kBS>(valid, s, SpanRows(st, segOf[ss]), CandMask{}, SortedFetch{SlotBuf(P, N + kSlotPad)},
+        neighbor_walk<kBS>(valid, s, SpanRows(st, segOf[ss]), SortedFetch{SlotBuf(P, N + kSlotPad)},
                                   sBuf + (threadIdx.x & ~63u), sQ + threadIdx.x, 0, c.h2, dist2, apply);
     });
     if (!valid) return;

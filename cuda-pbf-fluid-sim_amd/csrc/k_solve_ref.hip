@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kWalkBS) void k_lambda_ref(float* __restrict__ lamb
                 const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
                 lambda_pair<decltype(form)::value>(a, rx, ry, rz, pair_r2(rx, ry, rz), c);
             };
-            neighbor_walk<false, kWalkBS>(valid, s, rows, CandMask{}, fetch, sBuf + (threadIdx.x & ~63u),
+            neighbor_walk<kWalkBS>(valid, s, rows, fetch, sBuf + (threadIdx.x & ~63u),
                                       sQ + threadIdx.x, dp.maxNeighbors, c.h2, dist2, apply);
         });
     };
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kWalkBS) __attribute__((amdgpu_waves_per_eu(7))) vo
                 delta_pair<decltype(form)::value>(dx, dy, dz, rx, ry, rz, pair_r2(rx, ry, rz), lam_i,
                                                   lam_j, c);
             };
-            neighbor_walk<false, kWalkBS>(valid, s, rows, CandMask{}, fetch, sBuf + (threadIdx.x & ~63u),
+            neighbor_walk<kWalkBS>(valid, s, rows, fetch, sBuf + (threadIdx.x & ~63u),
                                       sQ + threadIdx.x, 0, c.h2, dist2, apply);
         });
     };
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(kWalkBS) void k_xsph_ref(float4* __restrict__ Vout,
             xsph_pair(sx, sy, sz, vj, vi, w);
             ++n;
         };
-        neighbor_walk<false, kWalkBS>(valid, s, rows, CandMask{}, IndexedFetch{P, idxS}, sBuf + (threadIdx.x & ~63u),
+        neighbor_walk<kWalkBS>(valid, s, rows, IndexedFetch{P, idxS}, sBuf + (threadIdx.x & ~63u),
                                   sQ + threadIdx.x, 0, h2, dist2, apply);
     }
     if (!valid) return;

@@ -404,8 +404,9 @@ __device__ __forceinline__ uint32_t lane_slot(uint32_t t, uint32_t N, const uint
 // the last particle's padding return zeros, and past the span are masked off), and
 // records each slot's skin bit of every candidate in walk order, its own slot included
 // (the walks that read the mask skip it), 32 to a word.  No pair arithmetic and no
-// queue: it is the candidate test alone, which the first delta pass would otherwise do
-// on top of its walk.
+// queue: the candidate test alone.  Built here rather than during the first delta pass
+// (as it once was), it also serves the first lambda pass, which then skips the
+// candidates outside the skin too.
 // Eight loads a round kept more of them in flight than four (a 9% faster build), with no
 // pair arithmetic to hide their latency behind.
 constexpr uint32_t kMaskRound = 8;
@@ -498,7 +499,7 @@ __global__ __launch_bounds__(BS) void k_lambda_sorted(float4* Ps, const uint32_t
         if (useMask && !nan_position_seen(stepFlags))
             mask_walk<BS>(valid, s, m, st, mk, PS, sQ + threadIdx.x, dp.maxNeighbors, c.h2, dist2, apply);
         else
-            neighbor_walk<false, BS>(valid, s, SpanRows(st, m), mk, SortedFetch{PS}, sBuf + (threadIdx.x & ~63u),
+            neighbor_walk<BS>(valid, s, SpanRows(st, m), SortedFetch{PS}, sBuf + (threadIdx.x & ~63u),
                                      sQ + threadIdx.x, dp.maxNeighbors, c.h2, dist2, apply);
     });
     if (!valid) return;
@@ -511,13 +512,12 @@ __global__ __launch_bounds__(BS) void k_lambda_sorted(float4* Ps, const uint32_t
     }
 }
 
-// Delta of a sorted slot.  maskMode: 0 full walk, 1 full walk that builds the
-// candidate mask (the later lambda passes walk the mask; every delta pass walks every
-// candidate, which measured faster than the mask's walk for them).
+// Delta of a sorted slot: every candidate (a delta pass takes every neighbour; for it
+// the candidate mask's walk measured slower than the full one).
 template <int BS>
 __global__ __launch_bounds__(BS) void k_delta_sorted(const float4* __restrict__ Pa, float4* __restrict__ Pb,
                                                       const uint32_t* __restrict__ segOf, SpanTable st,
-                                                      CandMask mk, int maskMode, sim::DeviceParams dp, uint32_t N,
+                                                      sim::DeviceParams dp, uint32_t N,
                                                       int last, const float4* __restrict__ X,
                                                       const uint32_t* __restrict__ idxS, float4* __restrict__ Vs,
                                                       float invDt, Scenes sc, uint32_t* __restrict__ clampBits,
@@ -547,12 +547,7 @@ __global__ __launch_bounds__(BS) void k_delta_sorted(const float4* __restrict__ 
             delta_pair<decltype(form)::kFixed, decltype(form)::kRatio2>(dx, dy, dz, rx, ry, rz,
                                                                         pair_r2(rx, ry, rz), pi.w, pj.w, c);
         };
-        if (maskMode == 1)
-            neighbor_walk<true, BS>(valid, s, SpanRows(st, m), mk, SortedFetch{PA}, buf, sQ + threadIdx.x, 0, c.h2,
-                                    dist2, apply);
-        else
-            neighbor_walk<false, BS>(valid, s, SpanRows(st, m), mk, SortedFetch{PA}, buf, sQ + threadIdx.x, 0, c.h2,
-                                     dist2, apply);
+        neighbor_walk<BS>(valid, s, SpanRows(st, m), SortedFetch{PA}, buf, sQ + threadIdx.x, 0, c.h2, dist2, apply);
     });
     if (!valid) return;
     const float3 e = delta_scaled(dx, dy, dz, dp);
@@ -786,18 +781,18 @@ void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st
 }
 
 void launch_delta_sorted(const float4* Pa, float4* Pb, const uint32_t* segOf, const SpanTable& st,
-                         const CandMask& mk, int maskMode, const sim::DeviceParams& dp, uint32_t N, bool last,
+                         const sim::DeviceParams& dp, uint32_t N, bool last,
                          const float4* X, const uint32_t* idxS, float4* Vs, float invDt, const Scenes& sc,
                          hipStream_t s, uint32_t* clampBits, float4* recU, const uint32_t* perm,
                          uint32_t* stepFlags, const RatioCoef* ratioCoef) {
     if (N == 0) return;
     const int lastPass = last ? 1 : 0;
     if (st.block == 128)
-        k_delta_sorted<128><<<ceil_div(N, 128u), 128, 0, s>>>(Pa, Pb, segOf, st, mk, maskMode, dp, N, lastPass, X,
+        k_delta_sorted<128><<<ceil_div(N, 128u), 128, 0, s>>>(Pa, Pb, segOf, st, dp, N, lastPass, X,
                                                               idxS, Vs, invDt, sc, clampBits, recU, perm, stepFlags,
                                                               ratioCoef);
     else
-        k_delta_sorted<256><<<ceil_div(N, 256u), 256, 0, s>>>(Pa, Pb, segOf, st, mk, maskMode, dp, N, lastPass, X,
+        k_delta_sorted<256><<<ceil_div(N, 256u), 256, 0, s>>>(Pa, Pb, segOf, st, dp, N, lastPass, X,
                                                               idxS, Vs, invDt, sc, clampBits, recU, perm, stepFlags,
                                                               ratioCoef);
     check_launch("delta_sorted");

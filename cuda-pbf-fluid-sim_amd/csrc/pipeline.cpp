@@ -226,7 +226,7 @@ void SortedSolve(DeviceBuffers& b, GridBuffers& g, const SimParams& p, hipStream
                                    rec ? b.rec.lamA[it] : nullptr, rec ? b.rec.lamB[it] : nullptr, perm,
                                    g.d_stepFlags);
         mark("lambda");
-        pbfx::launch_delta_sorted(Pa, Pb, g.d_segOf, g.spans, mk, 0, dp, N, last, b.d_pos, g.d_indices_sorted,
+        pbfx::launch_delta_sorted(Pa, Pb, g.d_segOf, g.spans, dp, N, last, b.d_pos, g.d_indices_sorted,
                                   rec ? b.rec.Vs : b.d_vel_sorted, invDt, g.scenes, s,
                                   last ? b.rec.finalClamp : nullptr, rec ? b.rec.dispU[it] : nullptr, perm,
                                   g.d_stepFlags, g.d_ratioCoef);

@@ -150,12 +150,12 @@ void launch_lambda_sorted(float4* Ps, const uint32_t* segOf, const SpanTable& st
 void launch_ratio_check(const sim::DeviceParams& dp, RatioCoef* coef, uint32_t* bad, hipStream_t s);
 
 // Pb[s] = Pa[s] + delta(s).  On the last iteration also clamps to the box and
-// writes Vs[s] = (Pb[s] - X[idxS[s]]) * invDt.  maskMode 0: full walk; 1: full
-// walk that also builds the candidate mask.  Recording: clampBits[s] (last iteration)
+// writes Vs[s] = (Pb[s] - X[idxS[s]]) * invDt.  Every candidate is walked.  Recording:
+// clampBits[s] (last iteration)
 // and recU[s] = the scaled correction before the displacement clamp.  stepFlags: the
 // last iteration sets kStepVelocityUnbounded, the others kStepNanPosition (see there).
 void launch_delta_sorted(const float4* Pa, float4* Pb, const uint32_t* segOf, const SpanTable& st,
-                         const CandMask& mk, int maskMode, const sim::DeviceParams& dp, uint32_t N, bool last,
+                         const sim::DeviceParams& dp, uint32_t N, bool last,
                          const float4* X, const uint32_t* idxS, float4* Vs, float invDt, const Scenes& sc,
                          hipStream_t s, uint32_t* clampBits = nullptr, float4* recU = nullptr,
                          const uint32_t* perm = nullptr, uint32_t* stepFlags = nullptr,

@@ -90,13 +90,13 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 
 // Candidate mask: bit c of particle s is set when candidate c of its walk (the
 // concatenated neighbour-row spans, in walk order) was within the skin radius
-// h + 2 * (I - 1) * maxDisp when the first delta pass scanned the predicted
-// positions.  The passes that read the mask (the lambda passes of iterations
-// 1 .. I-1) see positions at most I - 1 corrections later, each clamped to maxDisp, so
-// a pair outside the skin cannot have come within h.  Those passes visit only the set
-// bits, in walk order, and still test the range.  (The later delta passes, which take
-// every neighbour, and XSPH, after the last correction and the box clamp, walk their
-// candidates directly.)
+// h + 2 * (I - 1) * maxDisp of the predicted positions (k_mask_build, before the first
+// iteration).  The passes that read the mask (the lambda passes of iterations 0 .. I-1)
+// see positions at most I - 1 corrections later, each clamped to maxDisp, so a pair
+// outside the skin cannot have come within h.  Those passes visit only the set bits, in
+// walk order, and still test the range.  (The delta passes, which take every neighbour,
+// and XSPH, after the last correction and the box clamp, walk their candidates
+// directly.)
 // The mask covers the first 32 * kMaskWords candidates; candidates past that are
 // tested directly.
 constexpr uint32_t kMaskWords = kCandMaskWords;
@@ -117,13 +117,11 @@ __device__ __forceinline__ auto fetch_round(const Fetch& f, uint32_t k, uint32_t
 __device__ __forceinline__ uint64_t low_bits64(uint32_t n) { return n >= 64u ? ~0ull : ((1ull << n) - 1ull); }
 
 // kLanes: lanes per block of the calling kernel (the FIFO's stride in LDS).
-template <bool kBuild, int kLanes, class Rows, class Fetch, class Dist2, class Apply>
-__device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, const Rows& rows, const CandMask& mk,
-                                              const Fetch& fetch, float4* buf, uint32_t* q, int cap, float h2,
-                                              Dist2&& dist2, Apply&& apply) {
+template <int kLanes, class Rows, class Fetch, class Dist2, class Apply>
+__device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, const Rows& rows, const Fetch& fetch, float4* buf,
+                                              uint32_t* q, int cap, float h2, Dist2&& dist2, Apply&& apply) {
     const uint32_t lane = threadIdx.x & 63;
     int qh = 0, qn = 0, found = 0;
-    uint32_t cidx = 0, word = 0;  // mask building: own-candidate index and pending word
     auto push = [&](uint32_t k) {
         q[((qh + qn) & (kQD - 1)) * kLanes] = k;
         ++qn;
@@ -145,17 +143,8 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, const Rows
                 pop_one();
             } while (__any(qn > keep));
     };
-    // one own candidate: record its skin bit, queue it when in range
+    // one own candidate: queue it when in range
     auto visit = [&](uint32_t k, float r2, uint32_t& hi, uint32_t lo) {
-        if (kBuild) {
-            if (in_range(r2, mk.rs2)) word |= 1u << (cidx & 31);
-            ++cidx;
-            if ((cidx & 31) == 0) {
-                const uint32_t w = (cidx >> 5) - 1;
-                if (w < kMaskWords) mk.words[(uint64_t)w * mk.stride + s] = word;
-                word = 0;
-            }
-        }
         if (k != s && in_range(r2, h2)) {
             push(fetch.id(k));
             if (cap > 0 && found >= cap) hi = lo;
@@ -189,26 +178,10 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, const Rows
                 for (uint32_t u = 0; u < kUnroll; ++u) pj[u] = fetch_round(fetch, k, u, id[u]);
                 const uint32_t inSpan = (1u << cnt) - 1u;
                 const uint32_t self = s - k < kUnroll ? 1u << (s - k) : 0u;
-                uint32_t bits = 0u, skin = 0u;
+                uint32_t bits = 0u;
 #pragma unroll
-                for (uint32_t u = 0; u < kUnroll; ++u) {
-                    const float r2 = dist2(pj[u]);
-                    bits |= in_range(r2, h2) ? 1u << u : 0u;
-                    if (kBuild) skin |= in_range(r2, mk.rs2) ? 1u << u : 0u;
-                }
+                for (uint32_t u = 0; u < kUnroll; ++u) bits |= in_range(dist2(pj[u]), h2) ? 1u << u : 0u;
                 bits &= inSpan & ~self;
-                skin &= inSpan;
-                if (kBuild && cnt > 0) {
-                    const uint32_t sh = cidx & 31u;
-                    uint64_t w = (uint64_t)word | ((uint64_t)skin << sh);
-                    if (sh + cnt >= 32u) {
-                        const uint32_t wi = cidx >> 5;
-                        if (wi < kMaskWords) mk.words[(uint64_t)wi * mk.stride + s] = (uint32_t)w;
-                        w >>= 32;
-                    }
-                    word = (uint32_t)w;
-                    cidx += cnt;
-                }
                 if (cap > 0) {
                     const int room = cap - found;
                     while (__popc(bits) > room) bits &= ~(1u << (31 - __clz(bits)));
@@ -225,9 +198,8 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, const Rows
                 }
                 k += cnt;
             }
-        } else if (kBuild || rows.chunkBits == 0) {
-            // dense row, candidate by candidate (and building the candidate mask):
-            // stream the union through LDS
+        } else if (rows.chunkBits == 0) {
+            // dense row, candidate by candidate: stream the union through LDS
             for (uint32_t base = ulo; base < uhi; base += 64) {
                 const uint32_t c = base + lane;
                 if (c < uhi) buf[lane] = fetch.at(fetch.id(c));
@@ -278,10 +250,6 @@ __device__ __forceinline__ void neighbor_walk(bool valid, uint32_t s, const Rows
             }
             if (cap > 0 && found >= cap) hi = lo;
         }
-    }
-    if (kBuild && valid && (cidx & 31) != 0) {
-        const uint32_t w = cidx >> 5;
-        if (w < kMaskWords) mk.words[(uint64_t)w * mk.stride + s] = word;
     }
     pop_above(0);
 }

@@ -132,8 +132,8 @@ int pbfEnsembleSceneInfo(PbfSim* sim, uint32_t scene, uint32_t* first, uint32_t*
 // gives the same bits).  No reference counterpart; picked per workload by the
 // autotuner (pbfx/tune.py).
 typedef struct {
-    int candidateMask;     // 1: the first delta pass records which candidates can come within h
-                           // this step; the later lambda / delta passes walk only those
+    int candidateMask;     // 1: a pass before the first iteration records which candidates can
+                           // come within h this step; the lambda passes walk only those
     uint32_t streamRatio;  // a wave streams a neighbour row through LDS when the union of its
                            // lanes' spans is <= streamRatio x the longest span (>= 1)
     int chunkBits;         // streamed rows: 1 = test a 64-candidate chunk into a bit mask, then
