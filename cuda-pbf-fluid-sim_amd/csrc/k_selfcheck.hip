@@ -100,8 +100,8 @@ extern "C" int pbfDebugPairUnscaled(sim::DeviceParams dp) {
     return h;
 }
 
-extern "C" int pbfDebugScorrRatio(sim::DeviceParams dp, float* out8) {
-    if (!out8) return -1;
+extern "C" int pbfDebugScorrRatio(sim::DeviceParams dp, float* out12) {
+    if (!out12) return -1;
     pbfx::RatioCoef* d = nullptr;
     uint32_t* bad = nullptr;
     if (hipMalloc(&d, sizeof(pbfx::RatioCoef)) != hipSuccess || hipMalloc(&bad, sizeof(uint32_t)) != hipSuccess) {
@@ -119,8 +119,8 @@ extern "C" int pbfDebugScorrRatio(sim::DeviceParams dp, float* out8) {
         pbfx::set_error((int)e, "pbfDebugScorrRatio");
         return -1;
     }
-    const float v[8] = {h.zh, h.zl, h.ok, h.w_q, h.poly6, h.h2, h.neg_k, h.smin};
-    for (int i = 0; i < 8; ++i) out8[i] = v[i];
+    const float v[12] = {h.zh, h.zl, h.ok, h.w_q, h.poly6, h.h2, h.neg_k, h.smin, h.rsq_ok, h.h, h.eps, h.neg3_spiky};
+    for (int i = 0; i < 12; ++i) out12[i] = v[i];
     return 0;
 }
 

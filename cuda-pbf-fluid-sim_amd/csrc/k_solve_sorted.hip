@@ -40,11 +40,13 @@ __device__ __forceinline__ bool velocity_bounded(float x, float y, float z) {
 __device__ __forceinline__ void note_velocity(uint32_t* stepFlags, float x, float y, float z) {
     if (stepFlags && !velocity_bounded(x, y, z)) atomicOr(stepFlags, kStepVelocityUnbounded);
 }
-// The two-instruction s_corr ratio, when the record holds the check's verdict for this
-// launch's constants (a record left from other parameters keeps the division).
+// The checked shortcuts (the two-instruction s_corr ratio and the spiky coefficient's
+// rsq start), when the record holds both checks' verdicts for this launch's constants
+// (a record left from other parameters keeps the full sequences).
 __device__ __forceinline__ void take_ratio_coef(PairConsts& c, const RatioCoef* __restrict__ rc) {
-    if (rc && rc->ok != 0.f && rc->w_q == c.w_q && rc->poly6 == c.poly6 && rc->h2 == c.h2 &&
-        rc->neg_k == c.neg_scorr_k && rc->smin == c.scorr_min) {
+    if (rc && rc->ok != 0.f && rc->rsq_ok != 0.f && rc->w_q == c.w_q && rc->poly6 == c.poly6 &&
+        rc->h2 == c.h2 && rc->neg_k == c.neg_scorr_k && rc->smin == c.scorr_min && rc->h == c.h &&
+        rc->eps == c.grad_r_eps && rc->neg3_spiky == c.neg3_spiky) {
         c.rq_hi = rc->zh;
         c.rq_lo = rc->zl;
         c.ratio2 = true;
@@ -53,7 +55,9 @@ __device__ __forceinline__ void take_ratio_coef(PairConsts& c, const RatioCoef* 
 
 // Every float W from +0 to the largest a pair can have (poly6 h2^3: r = 0), in a
 // grid-stride loop: any W whose s_corr term from the two-instruction ratio differs from
-// the division's sets *bad.
+// the division's sets bit 0 of *bad.  Every float r2 from +0 to h2 (and a NaN):
+// any whose spiky coefficient from the rsq start differs from spiky_coef's (for NaN:
+// is not NaN) sets bit 1.
 __global__ __launch_bounds__(kBS) void k_ratio_check(sim::DeviceParams dp, uint32_t* __restrict__ bad) {
     const PairConsts c = make_pair_consts(dp);
     if (!pair_fixed(c)) return;
@@ -61,18 +65,31 @@ __global__ __launch_bounds__(kBS) void k_ratio_check(sim::DeviceParams dp, uint3
     const float zl = __builtin_fmaf(-c.w_q, zh, 1.0f) / c.w_q;  // 1 - w_q zh is exact
     const float wmax = c.poly6 * c.h2 * c.h2 * c.h2;
     if (!(wmax >= 0.f && wmax < 0x1p127f)) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(bad, 1u);
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(bad, 3u);
         return;
     }
     const uint64_t top = __float_as_uint(wmax);
+    const uint64_t first = (uint64_t)blockIdx.x * kBS + threadIdx.x, stride = (uint64_t)gridDim.x * kBS;
     uint32_t diff = 0;
-    for (uint64_t u = (uint64_t)blockIdx.x * kBS + threadIdx.x; u <= top; u += (uint64_t)gridDim.x * kBS) {
+    for (uint64_t u = first; u <= top; u += stride) {
         const float w = __uint_as_float((uint32_t)u);
         const float a = scorr_fixed(scorr_ratio2(w, zh, zl), c);
         const float b = scorr_fixed(div_unscaled(w, c.w_q, c.rcp_w_q), c);
         diff |= __float_as_uint(a) ^ __float_as_uint(b);
     }
     if (diff) atomicOr(bad, 1u);
+    // h <= 2^16 in the common form (pair_unscaled_ok): h2 is a positive finite float
+    const uint64_t topR = __float_as_uint(c.h2);
+    uint32_t diffR = 0;
+    for (uint64_t u = first; u <= topR; u += stride) {
+        const float r2 = __uint_as_float((uint32_t)u);
+        diffR |= __float_as_uint(spiky_coef_rsq(r2, c)) ^ __float_as_uint(spiky_coef(r2, c));
+    }
+    if (first == 0) {  // (r2 is never -0: a sum of squares from +0)
+        const float nan = __uint_as_float(0x7FC00000u);
+        diffR |= (__builtin_isnan(spiky_coef_rsq(nan, c)) && __builtin_isnan(spiky_coef(nan, c))) ? 0u : 1u;
+    }
+    if (diffR) atomicOr(bad, 2u);
 }
 
 __global__ void k_ratio_record(sim::DeviceParams dp, const uint32_t* __restrict__ bad, RatioCoef* __restrict__ rc) {
@@ -81,12 +98,16 @@ __global__ void k_ratio_record(sim::DeviceParams dp, const uint32_t* __restrict_
     RatioCoef r;
     r.zh = zh;
     r.zl = __builtin_fmaf(-c.w_q, zh, 1.0f) / c.w_q;
-    r.ok = (pair_fixed(c) && *bad == 0u) ? 1.0f : 0.0f;
+    r.ok = (pair_fixed(c) && (*bad & 1u) == 0u) ? 1.0f : 0.0f;
     r.w_q = c.w_q;
     r.poly6 = c.poly6;
     r.h2 = c.h2;
     r.neg_k = c.neg_scorr_k;
     r.smin = c.scorr_min;
+    r.rsq_ok = (pair_fixed(c) && (*bad & 2u) == 0u) ? 1.0f : 0.0f;
+    r.h = c.h;
+    r.eps = c.grad_r_eps;
+    r.neg3_spiky = c.neg3_spiky;
     *rc = r;
 }
 

@@ -122,8 +122,11 @@ struct PairConsts {
     float w_q, neg_scorr_k, scorr_n, scorr_min;
     int scorr_enable, scorr_ni;
     float rcp_w_q;  // rcp_refined(w_q)
+    float rcp_eps;  // rcp_refined(grad_r_eps)
     bool unscaled;  // pair_unscaled_ok: the pair arithmetic uses the unscaled sequences
-    // the two-instruction s_corr ratio, when checked for these constants (scorr_ratio2)
+    // the checked shortcuts, taken together when both were checked for these constants:
+    // the two-instruction s_corr ratio (scorr_ratio2) and the spiky coefficient from the
+    // square root's rsq (spiky_coef_rsq)
     float rq_hi = 0.f, rq_lo = 0.f;
     bool ratio2 = false;
 };
@@ -142,6 +145,34 @@ __device__ __forceinline__ float scorr_ratio2(float w, float zh, float zl) {
 __device__ __forceinline__ float scorr_fixed(float ratio, const PairConsts& c) {
     const float x2 = ratio * ratio;  // pow_scorr's n = 4 product
     return fmaxf(c.neg_scorr_k * (x2 * x2), c.scorr_min);
+}
+
+// The spiky gradient's coefficient neg3_spiky (h - r)^2 / max(r, eps), r = sqrt(r2), in
+// the unscaled sequences (reference: pbf_lambda.cu:76-79, pbf_delta_apply.cu:71-74).
+__device__ __forceinline__ float spiky_coef(float r2, const PairConsts& c) {
+    const float r = sqrt_unscaled(r2);  // +0 or positive: r2 is a sum of squares
+    const float t = c.h - r;
+    const float r_safe = max_nonneg(r, c.grad_r_eps);
+    return div_unscaled(c.neg3_spiky * (t * t), r_safe, rcp_refined(r_safe));
+}
+// The same coefficient with the reciprocal's Newton step started from the square root's
+// own rsq instead of a v_rcp: min(rsq(r2), 1 / eps) is within a few ulps of
+// 1 / max(r, eps) (r within half an ulp of sqrt(r2)), one transcendental fewer a pair.
+// A different start can change the refined reciprocal's last bit, so it is used only
+// where the simulator has checked, on the device, that it gives spiky_coef's bits for
+// every float r2 from +0 to h2 (launch_ratio_check).  A NaN r2 gives NaN in both.
+__device__ __forceinline__ float spiky_coef_rsq(float r2, const PairConsts& c) {
+    const float y = __builtin_amdgcn_rsqf(fmaxf(r2, 0x1p-96f));
+    const float g = r2 * y;  // sqrt_unscaled's steps, sharing y
+    const float half_y = 0.5f * y;
+    const float r = __builtin_fmaf(__builtin_fmaf(-g, g, r2), half_y, g);
+    const float t = c.h - r;
+    const float r_safe = max_nonneg(r, c.grad_r_eps);
+    // both positive and finite (y <= 2^48, eps >= 2^-40 in the unscaled form): their
+    // bit patterns order as their values
+    const float y0 = __uint_as_float(min(__float_as_uint(y), __float_as_uint(c.rcp_eps)));
+    const float yr = __builtin_fmaf(__builtin_fmaf(-r_safe, y0, 1.0f), y0, y0);
+    return div_unscaled(c.neg3_spiky * (t * t), r_safe, yr);
 }
 
 // Quotients a / b with a in {0} or [amin, amax] and b in [bmin, bmax] (all positive)
@@ -191,8 +222,9 @@ __device__ __forceinline__ void with_pair_form(const PairConsts& c, F&& f) {
         f(std::false_type{});
 }
 
-// The delta kernels' forms: the common setting with the checked two-instruction s_corr
-// ratio, the common setting with the division, and the general form.
+// The delta kernels' forms: the common setting with the checked shortcuts (the
+// two-instruction s_corr ratio and the spiky coefficient's rsq start), the common
+// setting with the full sequences, and the general form.
 template <bool kF, bool kR>
 struct DeltaForm {
     static constexpr bool kFixed = kF, kRatio2 = kR;
@@ -226,6 +258,7 @@ __device__ __forceinline__ PairConsts make_pair_consts(const sim::DeviceParams& 
     c.scorr_enable = dp.pbf.scorr_enable;
     c.scorr_ni = scorr_int_exponent(dp.pbf.scorr_n);
     c.rcp_w_q = rcp_refined(c.w_q);
+    c.rcp_eps = rcp_refined(c.grad_r_eps);
     c.unscaled = pair_unscaled_ok(c);
     return c;
 }
@@ -245,10 +278,7 @@ __device__ __forceinline__ void lambda_pair(LambdaAcc& a, float rx, float ry, fl
     a.density = __builtin_fmaf(c.m_poly6 * hr2 * hr2, hr2, a.density);
     float coeff;
     if (kFixed || c.unscaled) {
-        const float r = sqrt_unscaled(r2);  // +0 or positive: r2 is a sum of squares
-        const float r_safe = max_nonneg(r, c.grad_r_eps);
-        const float t = c.h - r;
-        coeff = div_unscaled(c.neg3_spiky * (t * t), r_safe, rcp_refined(r_safe));
+        coeff = spiky_coef(r2, c);
     } else {
         const float r = sqrtf(r2);
         const float r_safe = fmaxf(r, c.grad_r_eps);
@@ -305,16 +335,13 @@ __device__ __forceinline__ void lambda_adjoint_coefs(const LambdaAcc& a, const s
 }
 
 // One in-range neighbour of the position-correction sum.  kFixed: the caller has
-// established pair_fixed(c); kRatio2 (with kFixed): also c.ratio2.
+// established pair_fixed(c); kRatio2 (with kFixed): also c.ratio2 (both checked shortcuts).
 template <bool kFixed = false, bool kRatio2 = false>
 __device__ __forceinline__ void delta_pair(float& dx, float& dy, float& dz, float rx, float ry, float rz, float r2,
                                            float lam_i, float lam_j, const PairConsts& c) {
     float coeff;
     if (kFixed || c.unscaled) {
-        const float r = sqrt_unscaled(r2);  // +0 or positive: r2 is a sum of squares
-        const float t = c.h - r;
-        const float r_safe = max_nonneg(r, c.grad_r_eps);
-        coeff = div_unscaled(c.neg3_spiky * (t * t), r_safe, rcp_refined(r_safe));
+        coeff = kRatio2 ? spiky_coef_rsq(r2, c) : spiky_coef(r2, c);
     } else {
         const float r = sqrtf(r2);
         const float t = c.h - r;

@@ -59,12 +59,15 @@ constexpr uint32_t kCandMaskWords = 64;
 
 // Per-step candidate mask (built and walked in walk.hpp): words[w * stride + s] holds
 // bits 32w..32w+31 of slot s's walk; rs2 is the squared skin radius.
-// The s_corr ratio check's record (launch_ratio_check): zh = RN(1 / w_q) and
+// The pair-arithmetic check's record (launch_ratio_check): zh = RN(1 / w_q) and
 // zl = RN(1 / w_q - zh), ok = 1 when fma(W, zh, W * zl) gives the division's s_corr term
-// for every float W from +0 to the largest a pair can have, and the parameters the check
-// was made for (the delta kernels take the two-instruction ratio only when theirs match).
+// for every float W from +0 to the largest a pair can have; rsq_ok = 1 when
+// spiky_coef_rsq gives spiky_coef's bits for every float r2 from +0 to h2; and the
+// parameters the check was made for (the delta kernels take the two shortcuts, together,
+// only when both checks passed and theirs match).
 struct RatioCoef {
     float zh, zl, ok, w_q, poly6, h2, neg_k, smin;
+    float rsq_ok, h, eps, neg3_spiky;
 };
 
 struct CandMask {

@@ -216,12 +216,14 @@ int pbfDebugPairUnscaled(sim::DeviceParams dp);
 // in the admitted operand range, zero numerators whose quotient is not +0, integer
 // max results that differ from fmaxf}.  Synchronous; 0 on success, -1 on error.
 int pbfDebugCheckUnscaledMath(uint64_t divSamples, uint64_t seed, unsigned long long* out5);
-// pbfDebugScorrRatio: the s_corr ratio check a simulator runs whenever its parameters
-// change, for dp: out8 = {zh, zl, ok, w_q, poly6, h2, -scorr_k, scorr_min}, where ok is 1
-// when fma(W, zh, W * zl) gives the division's s_corr term for every float W from +0 to
-// poly6 h2^3 (the delta kernels then take it), else 0.  Synchronous; 0 on success, -1 on
-// error.
-int pbfDebugScorrRatio(sim::DeviceParams dp, float* out8);
+// pbfDebugScorrRatio: the pair-arithmetic check a simulator runs whenever its parameters
+// change, for dp: out12 = {zh, zl, ok, w_q, poly6, h2, -scorr_k, scorr_min, rsq_ok, h,
+// grad_r_eps, -3 spiky}, where ok is 1 when fma(W, zh, W * zl) gives the division's s_corr
+// term for every float W from +0 to poly6 h2^3, and rsq_ok is 1 when the spiky gradient's
+// coefficient with its reciprocal started from the square root's rsq equals the one
+// started from v_rcp for every float r2 from +0 to h2 (with both at 1 the delta kernels take both),
+// else 0.  Synchronous; 0 on success, -1 on error.
+int pbfDebugScorrRatio(sim::DeviceParams dp, float* out12);
 
 #ifdef __cplusplus
 }

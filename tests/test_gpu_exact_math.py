@@ -101,7 +101,7 @@ def test_general_pair_form_matches_oracle(oracle, dev, variant, path):
 
 
 def _ratio_record(sp):
-    out = (ctypes.c_float * 8)()
+    out = (ctypes.c_float * 12)()
     assert _lib.lib().pbfDebugScorrRatio(P.make_device_params(sp), ctypes.addressof(out)) == 0
     return np.array(list(out), np.float32)
 
@@ -112,7 +112,7 @@ def test_scenes_take_the_two_instruction_ratio(dev, name):
     division's s_corr term for every float W from +0 to poly6 h^6), and the record holds
     zh = RN(1 / w_q) and zl = RN(1 / w_q - zh) for the scene's w_q = poly6 (h^2 - dq^2)^3."""
     sp = P.build_sim_params(SCENES[name]())
-    zh, zl, ok, wq, poly6, h2, negk, smin = _ratio_record(sp)
+    zh, zl, ok, wq, poly6, h2, negk, smin = _ratio_record(sp)[:8]
     f = np.float32
     hr2q = f(f(sp.kernel.h2) - f(f(f(sp.pbf.scorr_dq_h) * f(sp.kernel.h)) ** 2))
     assert wq == f(f(f(f(sp.kernel.poly6) * hr2q) * hr2q) * hr2q)
@@ -125,10 +125,25 @@ def test_scenes_take_the_two_instruction_ratio(dev, name):
     assert (poly6, h2, negk, smin) == (f(sp.kernel.poly6), f(sp.kernel.h2), f(-sp.pbf.scorr_k), f(sp.pbf.scorr_min))
 
 
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_scenes_take_the_rsq_started_reciprocal(dev, name):
+    """Every scene's spiky coefficient passes the exhaustive check: with the reciprocal of
+    max(r, eps) started from the square root's rsq (min(rsq(r2), 1/eps)) it equals the one
+    started from v_rcp for every float r2 from +0 to h2, and the record holds the
+    constants it was made for."""
+    sp = P.build_sim_params(SCENES[name]())
+    rec = _ratio_record(sp)
+    rsq_ok, h, eps, neg3 = rec[8:]
+    f = np.float32
+    assert rsq_ok == 1.0
+    assert (h, eps, neg3) == (f(sp.kernel.h), f(sp.pbf.grad_r_eps), f(f(-3.0) * f(sp.kernel.spiky)))
+
+
 @pytest.mark.parametrize("field,value", [("scorr_enable", 0), ("scorr_n", 3.0), ("scorr_min", 0.0)])
 def test_ratio_check_declines_outside_the_common_form(dev, field, value):
     """Outside the delta kernels' common form (s_corr off, another exponent, no floor below
     zero) the check records no two-instruction ratio."""
     sp = P.build_sim_params(P.small_block())
     setattr(sp.pbf, field, value)
-    assert _ratio_record(sp)[2] == 0.0
+    rec = _ratio_record(sp)
+    assert rec[2] == 0.0 and rec[8] == 0.0
